@@ -1,0 +1,250 @@
+"""ctypes binding of libemu_mi355x.so (C-ABI in include/emustack_mi355x.h).
+
+The library is built in-tree by emustack_amd.build; importing this module
+never builds and never falls back to anything else: if the library is
+missing the error says how to build it.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libemu_mi355x.so')
+
+EMU_OK = 0
+ERRORS = {
+    -1: ValueError,
+    -2: RuntimeError,
+    -3: RuntimeError,
+    -4: RuntimeError,
+    -5: ValueError,
+    -6: RuntimeError,
+    -7: MemoryError,
+}
+
+_lib = None
+_lock = threading.Lock()
+
+c_i64 = ctypes.c_int64
+c_dp = ctypes.POINTER(ctypes.c_double)
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+c_i32p = ctypes.POINTER(ctypes.c_int32)
+c_vp = ctypes.c_void_p
+
+EXPORTS = [
+    'emu_last_error', 'emu_ctx_create', 'emu_ctx_destroy', 'emu_set_lapack', 'emu_mesh_load',
+    'emu_mesh_analyze', 'emu_mesh_symbolic', 'emu_mesh_free', 'emu_mesh_info', 'emu_mesh_arrays',
+    'emu_mesh_geometry', 'emu_calc_modes_2d', 'emu_calc_modes_2d_batch', 'emu_assemble', 'emu_lu_solve',
+]
+
+
+class EmuError(RuntimeError):
+    """Failure reported by the native library (status code + message)."""
+
+    def __init__(self, code, msg):
+        super().__init__('%s (status %d)' % (msg, code))
+        self.code = code
+
+
+def _lapack_pointers():
+    """Addresses of zgees/ztrsen/ztrevc from scipy's LAPACK build (used for
+    the small projected Schur problem of the eigensolver, on the host)."""
+    import scipy.linalg.cython_lapack as cl
+    get_name = ctypes.pythonapi.PyCapsule_GetName
+    get_name.restype = ctypes.c_char_p
+    get_name.argtypes = [ctypes.py_object]
+    get_ptr = ctypes.pythonapi.PyCapsule_GetPointer
+    get_ptr.restype = ctypes.c_void_p
+    get_ptr.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    out = []
+    for name in ('zgees', 'ztrsen', 'ztrevc'):
+        cap = cl.__pyx_capi__[name]
+        out.append(get_ptr(cap, get_name(cap)))
+    return out
+
+
+def lib():
+    """Load the library once (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError('%s is missing: build it with `python -m emustack_amd.build` '
+                              '(or __graft_entry__.build())' % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        L.emu_last_error.restype = ctypes.c_char_p
+        L.emu_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(c_vp)]
+        L.emu_ctx_destroy.argtypes = [c_vp]
+        L.emu_ctx_destroy.restype = None
+        L.emu_set_lapack.argtypes = [c_vp, c_vp, c_vp]
+        L.emu_mesh_load.argtypes = [c_vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(c_vp)]
+        L.emu_mesh_analyze.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(c_vp)]
+        L.emu_mesh_symbolic.argtypes = [c_vp, c_i32p, c_i64p, c_i64p, c_i32p, c_i32p, c_i64p]
+        L.emu_mesh_free.argtypes = [c_vp]
+        L.emu_mesh_free.restype = None
+        L.emu_mesh_info.argtypes = [c_vp, c_i64p]
+        L.emu_mesh_arrays.argtypes = [c_vp, c_i64p, c_i32p, c_i32p, c_i32p, c_dp]
+        L.emu_mesh_geometry.argtypes = [c_vp, c_i64p, c_i64p, c_dp]
+        L.emu_calc_modes_2d.argtypes = [c_vp, c_vp, ctypes.c_double, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp,
+                                        c_i64, c_i64, c_i64, c_dp, c_dp, c_dp]
+        L.emu_calc_modes_2d_batch.argtypes = [c_vp, c_vp, c_i64, c_dp, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp,
+                                              c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_i64p]
+        L.emu_assemble.argtypes = [c_vp, c_vp, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp]
+        L.emu_lu_solve.argtypes = [c_vp, c_vp, c_i64, c_dp, c_i64, c_dp, c_dp, c_i64p]
+        for f in EXPORTS:
+            getattr(L, f)
+        rc = L.emu_set_lapack(*_lapack_pointers())
+        if rc != EMU_OK:
+            raise EmuError(rc, L.emu_last_error().decode())
+        _lib = L
+        return L
+
+
+def check(rc):
+    if rc != EMU_OK:
+        msg = lib().emu_last_error().decode()
+        raise EmuError(rc, msg)
+
+
+def ptr(a, kind=c_dp):
+    if a is None:
+        return None
+    return a.ctypes.data_as(kind)
+
+
+def as_f64(a):
+    """Complex array -> contiguous interleaved float64 view."""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a.view(np.float64)
+
+
+class Context:
+    """One HIP device + stream (emu_ctx)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = c_vp()
+        check(L.emu_ctx_create(int(device), ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def assemble(self, mesh, pp, qq, shift, bloch):
+        """mat1 = K - shift*M, mat2 = M for nb instances (CSR order of mesh.arrays())."""
+        pp = np.atleast_2d(np.asarray(pp, np.complex128))
+        qq = np.atleast_2d(np.asarray(qq, np.complex128))
+        nb, ntyp = pp.shape
+        shift = np.asarray(shift, np.complex128).reshape(nb)
+        bloch = np.ascontiguousarray(np.asarray(bloch, np.float64).reshape(nb, 2))
+        m1 = np.zeros((nb, mesh.nnz), np.complex128)
+        m2 = np.zeros((nb, mesh.nnz), np.complex128)
+        check(lib().emu_assemble(self.h, mesh.h, nb, ntyp, ptr(as_f64(pp)), ptr(as_f64(qq)), ptr(as_f64(shift)),
+                                 ptr(bloch), ptr(m1.view(np.float64)), ptr(m2.view(np.float64))))
+        return m1, m2
+
+    def lu_solve(self, mesh, mat1, b):
+        """Factor mat1 [nb, nnz] on the GPU and solve for b [nb, nrhs, neq]."""
+        mat1 = np.ascontiguousarray(np.atleast_2d(mat1), np.complex128)
+        nb = mat1.shape[0]
+        b = np.ascontiguousarray(np.asarray(b, np.complex128).reshape(nb, -1, mesh.neq))
+        x = np.zeros_like(b)
+        st = np.zeros(8, np.int64)
+        check(lib().emu_lu_solve(self.h, mesh.h, nb, ptr(mat1.view(np.float64)), b.shape[1],
+                                 ptr(b.view(np.float64)), ptr(x.view(np.float64)), ptr(st, c_i64p)))
+        return x, int(st[0])
+
+    def calc_modes_batch(self, mesh, lam, nval, ordre, n_eff, bloch, shift, neq_pw, E_H_field=1, itermax=30,
+                         tol=0.0):
+        """Batched calc_modes_2d: returns beta [nl, nval], J [nl, 2P, nval], Jd [nl, nval, 2P], stats."""
+        lam = np.ascontiguousarray(np.atleast_1d(np.asarray(lam, np.float64)))
+        nl = lam.shape[0]
+        n_eff = np.ascontiguousarray(np.asarray(n_eff, np.complex128).reshape(nl, -1))
+        ntyp = n_eff.shape[1]
+        bloch = np.ascontiguousarray(np.asarray(bloch, np.float64).reshape(nl, 2))
+        shift = np.ascontiguousarray(np.asarray(shift, np.complex128).reshape(nl))
+        beta = np.zeros((nl, nval), np.complex128)
+        J = np.zeros((nl, nval, 2 * neq_pw), np.complex128)    # C-order view of F-order (2P x nval)
+        Jd = np.zeros((nl, 2 * neq_pw, nval), np.complex128)   # C-order view of F-order (nval x 2P)
+        opts = np.array([tol, 0.0])
+        st = np.zeros(8, np.int64)
+        check(lib().emu_calc_modes_2d_batch(self.h, mesh.h, nl, ptr(lam), nval, ordre, ntyp,
+                                            ptr(n_eff.view(np.float64)), ptr(bloch),
+                                            ptr(shift.view(np.float64)), E_H_field, itermax, neq_pw, ptr(opts),
+                                            ptr(beta.view(np.float64)), ptr(J.view(np.float64)),
+                                            ptr(Jd.view(np.float64)), ptr(st, c_i64p)))
+        return beta, J.transpose(0, 2, 1), Jd.transpose(0, 2, 1), st
+
+    def close(self):
+        if self.h:
+            lib().emu_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MeshHandle:
+    """A loaded .mail mesh with cached topology, pattern and symbolic LU."""
+
+    def __init__(self, path, ctx=None, leaf_size=0):
+        L = lib()
+        h = c_vp()
+        if ctx is None:
+            check(L.emu_mesh_analyze(path.encode(), int(leaf_size), ctypes.byref(h)))
+        else:
+            check(L.emu_mesh_load(ctx.h, path.encode(), int(leaf_size), ctypes.byref(h)))
+        self.h = h
+        self.path = path
+        self.ctx = ctx
+        info = np.zeros(16, np.int64)
+        check(L.emu_mesh_info(h, ptr(info, c_i64p)))
+        (self.npt, self.nel, self.n_ddl, self.neq, self.nnz, self.nfronts, self.nlevels, self.pool,
+         self.max_f, self.factor_entries, self.lu_flops, self.n_edge, self.npt_p3, self.nw) = [int(v) for v in info[:14]]
+
+    def arrays(self):
+        L = lib()
+        rp = np.zeros(self.neq + 1, np.int64)
+        ci = np.zeros(self.nnz, np.int32)
+        ld = np.zeros(self.nel * 22, np.int32)
+        pm = np.zeros(self.neq, np.int32)
+        lat = np.zeros(4)
+        check(L.emu_mesh_arrays(self.h, ptr(rp, c_i64p), ptr(ci, c_i32p), ptr(ld, c_i32p), ptr(pm, c_i32p),
+                                ptr(lat)))
+        return dict(row_ptr=rp, col_ind=ci, ldof=ld.reshape(self.nel, 22), perm=pm, lat=lat)
+
+    def geometry(self):
+        L = lib()
+        tn = np.zeros(6 * self.nel, np.int64)
+        te = np.zeros(self.nel, np.int64)
+        x = np.zeros(2 * self.npt)
+        check(L.emu_mesh_geometry(self.h, ptr(tn, c_i64p), ptr(te, c_i64p), ptr(x)))
+        return tn.reshape(self.nel, 6).T.copy(order='F'), te, x.reshape(self.npt, 2).T.copy(order='F')
+
+    def symbolic(self):
+        L = lib()
+        fr = np.zeros(8 * self.nfronts, np.int32)
+        fo = np.zeros(self.nfronts, np.int64)
+        uo = np.zeros(self.nfronts, np.int64)
+        ui = np.zeros(max(self.nw, 1), np.int32)
+        pmp = np.zeros(max(self.nw, 1), np.int32)
+        nf = np.zeros(self.nnz, np.int64)
+        check(L.emu_mesh_symbolic(self.h, ptr(fr, c_i32p), ptr(fo, c_i64p), ptr(uo, c_i64p), ptr(ui, c_i32p),
+                                  ptr(pmp, c_i32p), ptr(nf, c_i64p)))
+        return dict(fronts=fr.reshape(-1, 8), foff=fo, uoff=uo, uidx=ui[:self.nw], pmap=pmp[:self.nw],
+                    nnz2front=nf)
+
+    def close(self):
+        if self.h:
+            lib().emu_mesh_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

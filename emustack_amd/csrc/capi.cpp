@@ -1,0 +1,195 @@
+// extern "C" entry points of libemu_mi355x.so (declared in
+// include/emustack_mi355x.h).  Converts C++ exceptions to status codes and a
+// thread-local message.
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+
+#include "../../include/emustack_mi355x.h"
+#include "common.h"
+#include "engine_api.h"
+
+struct emu_ctx {
+  emu::Ctx *c;
+};
+struct emu_mesh {
+  emu::DevMesh *m;
+};
+
+static thread_local std::string g_err = "no error";
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define EMU_GUARD(body)                                                       \
+  try {                                                                       \
+    body;                                                                     \
+    return EMU_OK;                                                            \
+  } catch (const emu::NoConvergence &e) {                                     \
+    return fail(EMU_E_NOCONV, e.what());                                      \
+  } catch (const emu::Homogeneous &e) {                                       \
+    return fail(EMU_E_HOMOGENEOUS, e.what());                                 \
+  } catch (const emu::LapackMissing &e) {                                     \
+    return fail(EMU_E_LAPACK, e.what());                                      \
+  } catch (const emu::HipError &e) {                                          \
+    std::string w = e.what();                                                 \
+    return fail(w.find("hipMalloc") != std::string::npos ? EMU_E_OOM : EMU_E_HIP, w); \
+  } catch (const std::invalid_argument &e) {                                  \
+    return fail(EMU_E_ARG, e.what());                                         \
+  } catch (const std::bad_alloc &) {                                          \
+    return fail(EMU_E_OOM, "host allocation failed");                         \
+  } catch (const std::exception &e) {                                         \
+    std::string w = e.what();                                                 \
+    bool mesh = w.find("periodic") != std::string::npos || w.find("mesh") != std::string::npos || \
+                w.find("list_") != std::string::npos || w.find("asmbly") != std::string::npos ||  \
+                w.find("basis_ls") != std::string::npos || w.find("geometry") != std::string::npos; \
+    return fail(mesh ? EMU_E_MESH : EMU_E_LAPACK, w);                         \
+  }
+
+extern "C" {
+
+const char *emu_last_error(void) { return g_err.c_str(); }
+
+int emu_ctx_create(int device, emu_ctx **out) {
+  if (!out) return fail(EMU_E_ARG, "emu_ctx_create: out is NULL");
+  EMU_GUARD({
+    emu_ctx *c = new emu_ctx;
+    try {
+      c->c = emu::ctx_create(device);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  })
+}
+
+void emu_ctx_destroy(emu_ctx *ctx) {
+  if (!ctx) return;
+  emu::ctx_destroy(ctx->c);
+  delete ctx;
+}
+
+int emu_set_lapack(void *zgees, void *ztrsen, void *ztrevc) {
+  if (!zgees || !ztrsen || !ztrevc) return fail(EMU_E_ARG, "emu_set_lapack: NULL routine");
+  emu::set_lapack(zgees, ztrsen, ztrevc);
+  return EMU_OK;
+}
+
+int emu_mesh_load(emu_ctx *ctx, const char *path, int leaf_size, emu_mesh **out) {
+  if (!ctx || !path || !out) return fail(EMU_E_ARG, "emu_mesh_load: NULL argument");
+  EMU_GUARD({
+    emu_mesh *m = new emu_mesh;
+    try {
+      m->m = emu::mesh_load(ctx->c, path, leaf_size);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  })
+}
+
+int emu_mesh_analyze(const char *path, int leaf_size, emu_mesh **out) {
+  if (!path || !out) return fail(EMU_E_ARG, "emu_mesh_analyze: NULL argument");
+  EMU_GUARD({
+    emu_mesh *m = new emu_mesh;
+    try {
+      m->m = emu::mesh_analyze(path, leaf_size);
+    } catch (...) {
+      delete m;
+      throw;
+    }
+    *out = m;
+  })
+}
+
+int emu_mesh_symbolic(const emu_mesh *mesh, int32_t *fronts, int64_t *foff, int64_t *uoff, int32_t *uidx,
+                      int32_t *pmap, int64_t *nnz2front) {
+  if (!mesh) return fail(EMU_E_ARG, "emu_mesh_symbolic: NULL mesh");
+  EMU_GUARD(emu::mesh_symbolic(mesh->m, fronts, foff, uoff, uidx, pmap, nnz2front))
+}
+
+void emu_mesh_free(emu_mesh *mesh) {
+  if (!mesh) return;
+  emu::mesh_free(mesh->m);
+  delete mesh;
+}
+
+int emu_mesh_info(const emu_mesh *mesh, int64_t *info) {
+  if (!mesh || !info) return fail(EMU_E_ARG, "emu_mesh_info: NULL argument");
+  EMU_GUARD(emu::mesh_info(mesh->m, info))
+}
+
+int emu_mesh_arrays(const emu_mesh *mesh, int64_t *row_ptr, int32_t *col_ind, int32_t *ldof, int32_t *perm,
+                    double *lat) {
+  if (!mesh) return fail(EMU_E_ARG, "emu_mesh_arrays: NULL mesh");
+  EMU_GUARD(emu::mesh_arrays(mesh->m, row_ptr, col_ind, ldof, perm, lat))
+}
+
+int emu_mesh_geometry(const emu_mesh *mesh, int64_t *table_nod, int64_t *type_el, double *x_arr) {
+  if (!mesh) return fail(EMU_E_ARG, "emu_mesh_geometry: NULL mesh");
+  EMU_GUARD(emu::mesh_geometry(mesh->m, table_nod, type_el, x_arr))
+}
+
+int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *lambda, int64_t nval,
+                            int64_t ordre_ls, int64_t nb_typ_el, const double *n_eff, const double *bloch_vec,
+                            const double *shift, int64_t E_H_field, int64_t itermax, int64_t neq_PW,
+                            const double *opts, double *beta1, double *overlap_J, double *overlap_J_dagger,
+                            int64_t *stats) {
+  if (!ctx || !mesh || !lambda || !n_eff || !bloch_vec || !shift || !beta1 || !overlap_J || !overlap_J_dagger)
+    return fail(EMU_E_ARG, "emu_calc_modes_2d_batch: NULL argument");
+  if (nbatch < 1 || nval < 1 || nb_typ_el < 1 || neq_PW < 1 || itermax < 0)
+    return fail(EMU_E_ARG, "emu_calc_modes_2d_batch: invalid size");
+  if (E_H_field != 1 && E_H_field != 2) return fail(EMU_E_ARG, "E_H_field must be 1 or 2");
+  for (int64_t i = 0; i < nbatch; ++i)
+    if (!(lambda[i] > 0)) return fail(EMU_E_ARG, "lambda must be positive");
+  if (opts && opts[1] != 0) return fail(EMU_E_ARG, "only the complex128 precision form is available");
+  EMU_GUARD({
+    emu::ModesArgs a;
+    a.nl = nbatch;
+    a.lambda = lambda;
+    a.nval = nval;
+    a.ordre = ordre_ls;
+    a.ntyp = nb_typ_el;
+    a.eh = E_H_field;
+    a.itermax = itermax;
+    a.neq_pw = neq_PW;
+    a.n_eff = n_eff;
+    a.bloch = bloch_vec;
+    a.shift = shift;
+    if (opts && opts[0] > 0) a.tol = opts[0];
+    a.beta = beta1;
+    a.J = overlap_J;
+    a.Jd = overlap_J_dagger;
+    a.stats = stats;
+    emu::calc_modes(ctx->c, *mesh->m, a);
+  })
+}
+
+int emu_calc_modes_2d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval, int64_t ordre_ls,
+                      int64_t nb_typ_el, const double *n_eff, const double *bloch_vec, const double *shift,
+                      int64_t E_H_field, int64_t itermax, int64_t neq_PW, double *beta1, double *overlap_J,
+                      double *overlap_J_dagger) {
+  return emu_calc_modes_2d_batch(ctx, mesh, 1, &lambda, nval, ordre_ls, nb_typ_el, n_eff, bloch_vec, shift,
+                                 E_H_field, itermax, neq_PW, nullptr, beta1, overlap_J, overlap_J_dagger, nullptr);
+}
+
+int emu_assemble(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, int64_t nb_typ_el, const double *pp,
+                 const double *qq, const double *shift, const double *bloch, double *mat1, double *mat2) {
+  if (!ctx || !mesh || !pp || !qq || !shift || !bloch) return fail(EMU_E_ARG, "emu_assemble: NULL argument");
+  if (nbatch < 1 || nb_typ_el < 1) return fail(EMU_E_ARG, "emu_assemble: invalid size");
+  EMU_GUARD(emu::assemble(ctx->c, *mesh->m, (int)nbatch, (int)nb_typ_el, pp, qq, shift, bloch, mat1, mat2))
+}
+
+int emu_lu_solve(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *mat1, int64_t nrhs, const double *b,
+                 double *x, int64_t *stats) {
+  if (!ctx || !mesh || !mat1 || !b || !x) return fail(EMU_E_ARG, "emu_lu_solve: NULL argument");
+  if (nbatch < 1 || nrhs < 1) return fail(EMU_E_ARG, "emu_lu_solve: invalid size");
+  EMU_GUARD(emu::lu_solve_host(ctx->c, *mesh->m, (int)nbatch, mat1, (int)nrhs, b, x, stats))
+}
+
+}  // extern "C"

@@ -1,0 +1,155 @@
+// Device-side state of a mesh and of a batch of solve instances, and the
+// kernel launchers of the FEM / LU / Krylov / overlap stages.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+#include "emu_host.h"
+
+namespace emu {
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count <= n && p) return;
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw HipError("hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    }
+    n = count;
+  }
+  // synchronous: the host source may be a temporary
+  void upload(const T *h, size_t count, hipStream_t s) {
+    alloc(count);
+    EMU_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    EMU_HIP(hipStreamSynchronize(s));
+  }
+  void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+// Kernel task lists of one factorisation / solve level.
+struct StepPlan {
+  int32_t k0;
+  int64_t panel_off, panel_cnt;  // front ids
+  int64_t trsm_off, trsm_cnt;    // (front, col0)
+  int64_t gemm_off, gemm_cnt;    // (front, row0, col0)
+};
+struct LevelSched {
+  std::vector<StepPlan> steps;
+  int64_t fronts_off, fronts_cnt;     // fronts of the level
+  int64_t ea_off[2], ea_cnt[2];       // extend-add tasks (child front, col0) by slot
+  int32_t max_f;
+};
+
+struct DevMesh {
+  Mesh host;
+  int64_t ntyp = 0;
+  // FEM constants
+  DBuf<double> geo, det;       // nel x 8, nel
+  DBuf<signed char> orient;    // nel x 6
+  DBuf<double> dphase;         // nel x 28
+  DBuf<int32_t> type_el;       // nel (0-based)
+  DBuf<int32_t> ldof;          // nel x 22
+  DBuf<int32_t> cptr, citem;   // assembly gather lists
+  DBuf<double> eka, ekc, emd;  // element integrals nel x 484 (K pp-part, K qq-part, M pp-part)
+  DBuf<int64_t> row_ptr;       // CSR of the pattern (original numbering)
+  DBuf<int32_t> col_ind;
+  // permuted CSR of M (new numbering) and map from original nnz
+  DBuf<int64_t> prow_ptr;
+  DBuf<int32_t> pcol_ind, pnnz_src;  // pnnz_src[k'] = original nnz index
+  // transverse DOF contributor lists for the plane-wave load (J)
+  DBuf<int32_t> dcptr, dcitem;       // per dof: (e*12 + l)
+  DBuf<double> gt;                   // nel x 12 x 12: transverse field at P2 nodes from coefs
+  // symbolic LU
+  DBuf<int32_t> perm, iperm;
+  DBuf<int32_t> fs, fu, ff, fb, fparent, fslot;
+  DBuf<int64_t> foff, fuoff, fpivoff;
+  DBuf<int32_t> uidx, pmap;
+  DBuf<int64_t> nnz2front;
+  DBuf<int32_t> tasks;               // all task lists (+ child arrays)
+  int64_t child_off[2] = {0, 0};     // child0[t], child1[t] inside tasks
+  std::vector<LevelSched> sched;     // sched[0] = deepest level
+  int32_t max_f = 0;
+  // geometry for the plane-wave integrals
+  DBuf<double> xnod;                 // npt x 2
+  DBuf<int64_t> table_nod;           // nel x 6 (1-based)
+};
+
+void devmesh_build(DevMesh &dm, hipStream_t s);
+
+// ---------------- FEM kernels (kernels_fem.hip) ----------------
+void launch_elem_integrals(DevMesh &dm, hipStream_t s);
+// ve[nb][nel][14]
+void launch_phase(DevMesh &dm, int nb, const double *bloch_dev, zd *ve, hipStream_t s);
+// mat1, mat2 [nb][nnz]
+void launch_gather(DevMesh &dm, int nb, int ntyp, const zd *pp, const zd *qq, const zd *shift,
+                   const zd *ve, zd *mat1, zd *mat2, hipStream_t s);
+void launch_permute_vals(DevMesh &dm, int nb, const zd *src, zd *dst, hipStream_t s);
+
+// ---------------- LU kernels (kernels_lu.hip) ----------------
+void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int32_t *pflag,
+               hipStream_t s);
+// x = A^{-1} b, both in the new (nested-dissection) numbering, [nb][nrhs][n]
+void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *pool, const int32_t *piv, const zd *b,
+              zd *y, zd *x, zd *w, hipStream_t s);
+// permute vectors old <-> new numbering: dst[new] = src[perm[new]] or inverse
+void launch_permute_vec(DevMesh &dm, int nb, int ncol, const zd *src, zd *dst, bool to_new,
+                        hipStream_t s);
+
+// ---------------- dense / vector kernels (kernels_krylov.hip) ----------------
+// y[b][r] = sum_k A[b][r][k] x[b][col k] for CSR (n x n), ncol columns
+void launch_spmm(int nb, int64_t n, int ncol, const int64_t *rp, const int32_t *ci, const zd *vals,
+                 int64_t vstride, const zd *x, int64_t xstride, zd *y, int64_t ystride,
+                 hipStream_t s);
+// C[b] (M x N, ldc) = op(A[b])^T B[b] with K rows; op = conj if conjA.  Split-K.
+void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64_t sA,
+                const zd *B, int64_t ldb, int64_t sB, zd *C, int64_t ldc, int64_t sC,
+                bool conjA, zd *scratch, size_t scratch_elems, hipStream_t s);
+// C[b] (M x N) = alpha * A[b] (M x K) * B[b] (K x N) + beta*C[b], column-major
+void launch_gemm_nn(int nb, int64_t M, int N, int K, zd alpha, const zd *A, int64_t lda, int64_t sA,
+                    const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,
+                    hipStream_t s);
+// w[b] -= V[b][:, 0:k] h[b]  (h device, [b][hstride])
+void launch_sub_vh(int nb, int64_t n, int k, const zd *V, int64_t ldv, int64_t sV, const zd *h,
+                   int64_t sh, zd *w, int64_t sw, hipStream_t s);
+// nrm[b] = ||w[b]||; then w[b] *= 1/nrm[b] written to out[b]
+void launch_norm_scale(int nb, int64_t n, const zd *w, int64_t sw, zd *out, int64_t so,
+                       double *nrm, double *scratch, hipStream_t s);
+void launch_axpy_h(int nb, int k, const zd *h1, const zd *h2, int64_t sh, zd *H, int64_t sH,
+                   hipStream_t s);
+void launch_copy(int nb, int64_t n, const zd *src, int64_t ss, zd *dst, int64_t sd, hipStream_t s);
+void launch_fill(zd *p, int64_t n, zd v, hipStream_t s);
+
+// ---------------- post-processing (kernels_post.hip) ----------------
+// coefficient fields: max |field| over (e, node, comp) per mode, first index
+void launch_field_max(DevMesh &dm, int nb, int nval, const zd *X, int64_t ldx, int64_t sX,
+                      const zd *ve, bool conj_phase, zd *zmax, double *scratch, hipStream_t s);
+void launch_scale_cols(int nb, int64_t n, int ncol, zd *X, int64_t ldx, int64_t sX, const zd *f,
+                       int64_t sf, bool invert, hipStream_t s);
+// plane-wave load matrices G[b] (n x 2P, col-major): J = G^T X
+// per instance b: sign[b] = -1 (J, exp(-ikx)) or +1 (J^dagger, exp(+ikx)); the
+// DOF phase is the instance's own ve[b]
+void launch_pw_load(DevMesh &dm, int nb, int P, const double *alpha, const double *beta,
+                    const int32_t *pwpos, const int32_t *sign, const zd *ve, zd *JB, zd *G,
+                    hipStream_t s);
+void launch_permute_cols(int nb, int64_t n, int ncol, const zd *src, int64_t ss, const int32_t *idx,
+                         zd *dst, int64_t sd, hipStream_t s);
+
+}  // namespace emu

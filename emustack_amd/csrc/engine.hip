@@ -1,0 +1,913 @@
+// Host orchestration of the per-wavelength mode solve on one MI355X:
+// assembly -> multifrontal LU -> Krylov-Schur shift-invert eigensolve ->
+// normalisation / orthogonality / J overlaps, batched over instances that
+// share a mesh.  Mirrors calc_modes_2d (backend/fortran/py_calc_modes.f:1-958).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <random>
+#include <thread>
+
+#include "emu_host.h"
+#include "engine.h"
+#include "fem_basis.h"
+#include "engine_api.h"
+
+namespace emu {
+
+typedef std::complex<double> cplx;
+
+// ------------------------------------------------------------------ LAPACK
+typedef int (*zselect_t)(cplx *);
+typedef void (*zgees_t)(char *, char *, zselect_t, int *, cplx *, int *, int *, cplx *, cplx *, int *,
+                        cplx *, int *, double *, int *, int *);
+typedef void (*ztrsen_t)(char *, char *, int *, int *, cplx *, int *, cplx *, int *, cplx *, int *, double *,
+                         double *, cplx *, int *, int *);
+typedef void (*ztrevc_t)(char *, char *, int *, int *, cplx *, int *, cplx *, int *, cplx *, int *, int *, int *,
+                         cplx *, double *, int *);
+static zgees_t g_zgees = nullptr;
+static ztrsen_t g_ztrsen = nullptr;
+static ztrevc_t g_ztrevc = nullptr;
+
+void set_lapack(void *a, void *b, void *c) {
+  g_zgees = (zgees_t)a;
+  g_ztrsen = (ztrsen_t)b;
+  g_ztrevc = (ztrevc_t)c;
+}
+static int dummy_select(cplx *) { return 0; }
+bool lapack_ready() { return g_zgees && g_ztrsen && g_ztrevc; }
+
+// ------------------------------------------------------------ mesh upload
+void devmesh_build(DevMesh &dm, hipStream_t s) {
+  Mesh &m = dm.host;
+  const Symbolic &sy = m.sym;
+  const int64_t nel = m.nel, n = m.neq;
+  dm.geo.upload(m.geo, s);
+  dm.det.upload(m.det, s);
+  {
+    std::vector<signed char> o(m.orient.begin(), m.orient.end());
+    dm.orient.upload(o, s);
+  }
+  dm.dphase.upload(m.dphase, s);
+  {
+    std::vector<int32_t> t(nel);
+    int64_t mx = 0;
+    for (int64_t e = 0; e < nel; ++e) {
+      t[e] = (int32_t)(m.type_el[e] - 1);
+      mx = std::max<int64_t>(mx, m.type_el[e]);
+    }
+    dm.ntyp = mx;
+    dm.type_el.upload(t, s);
+  }
+  dm.ldof.upload(m.ldof, s);
+  dm.cptr.upload(m.cptr, s);
+  dm.citem.upload(m.citem, s);
+  dm.row_ptr.upload(m.row_ptr, s);
+  dm.col_ind.upload(m.col_ind, s);
+  dm.xnod.upload(m.x, s);
+  dm.table_nod.upload(m.table_nod, s);
+  // permuted CSR (new numbering) for M
+  {
+    std::vector<int64_t> prp(n + 1, 0);
+    std::vector<int32_t> pci(m.nnz), psrc(m.nnz);
+    std::vector<std::pair<int32_t, int32_t>> row;
+    for (int64_t i = 0; i < n; ++i) {
+      int32_t old = sy.perm[i];
+      row.clear();
+      for (int64_t k = m.row_ptr[old]; k < m.row_ptr[old + 1]; ++k)
+        row.push_back({sy.iperm[m.col_ind[k]], (int32_t)k});
+      std::sort(row.begin(), row.end());
+      prp[i + 1] = prp[i] + (int64_t)row.size();
+      for (size_t q = 0; q < row.size(); ++q) {
+        pci[prp[i] + q] = row[q].first;
+        psrc[prp[i] + q] = row[q].second;
+      }
+    }
+    dm.prow_ptr.upload(prp, s);
+    dm.pcol_ind.upload(pci, s);
+    dm.pnnz_src.upload(psrc, s);
+  }
+  // transverse DOF contributors
+  {
+    std::vector<int32_t> cnt(n + 1, 0);
+    for (int64_t e = 0; e < nel; ++e)
+      for (int l = 0; l < 12; ++l) cnt[m.ldof[22 * e + l] + 1]++;
+    std::vector<int32_t> ptr(n + 1, 0);
+    for (int64_t i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + cnt[i + 1];
+    std::vector<int32_t> item(ptr[n]);
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t e = 0; e < nel; ++e)
+      for (int l = 0; l < 12; ++l) item[fill[m.ldof[22 * e + l]]++] = (int32_t)(12 * e + l);
+    dm.dcptr.upload(ptr, s);
+    dm.dcitem.upload(item, s);
+  }
+  // transverse field at the 6 P2 nodes from the 12 transverse coefficients
+  {
+    std::vector<double> gt(144 * nel);
+    const double xn[6][2] = {{0, 0}, {1, 0}, {0, 1}, {0.5, 0}, {0.5, 0.5}, {0, 0.5}};
+    for (int64_t e = 0; e < nel; ++e) {
+      const double *T = &m.geo[8 * e + 4];
+      const signed char *ori = reinterpret_cast<const signed char *>(&m.orient[6 * e]);
+      for (int inod = 0; inod < 6; ++inod)
+        for (int l = 0; l < 12; ++l) {
+          double vx, vy, c;
+          transverse_basis(l, xn[inod][0], xn[inod][1], T, ori, vx, vy, c);
+          gt[144 * e + (2 * inod) * 12 + l] = vx;
+          gt[144 * e + (2 * inod + 1) * 12 + l] = vy;
+        }
+    }
+    dm.gt.upload(gt, s);
+  }
+  // symbolic LU arrays
+  dm.perm.upload(sy.perm, s);
+  dm.iperm.upload(sy.iperm, s);
+  const int32_t nf = (int32_t)sy.fronts.size();
+  std::vector<int32_t> fs(nf), fu(nf), ff(nf), fb(nf), fp(nf), fsl(nf);
+  std::vector<int64_t> foff(nf), fuoff(nf), fpiv(nf);
+  for (int32_t i = 0; i < nf; ++i) {
+    const FrontInfo &F = sy.fronts[i];
+    fs[i] = F.s; fu[i] = F.u; ff[i] = F.f; fb[i] = F.b; fp[i] = F.parent; fsl[i] = F.slot;
+    foff[i] = F.off; fuoff[i] = F.uoff; fpiv[i] = F.pivoff;
+  }
+  dm.fs.upload(fs, s); dm.fu.upload(fu, s); dm.ff.upload(ff, s); dm.fb.upload(fb, s);
+  dm.fparent.upload(fp, s); dm.fslot.upload(fsl, s);
+  dm.foff.upload(foff, s); dm.fuoff.upload(fuoff, s); dm.fpivoff.upload(fpiv, s);
+  dm.uidx.upload(sy.uidx, s);
+  dm.pmap.upload(sy.pmap, s);
+  dm.nnz2front.upload(sy.nnz2front, s);
+  dm.max_f = sy.max_f;
+  // launch schedule: deepest level first
+  std::vector<int32_t> tasks;
+  dm.sched.clear();
+  const int NBP = 32, TL = 64, EAC = 8;
+  for (int lv = (int)sy.levels.size() - 1; lv >= 0; --lv) {
+    const LevelPlan &LP = sy.levels[lv];
+    LevelSched L;
+    L.max_f = LP.max_f;
+    L.fronts_off = (int64_t)tasks.size();
+    for (int32_t t : LP.fronts) tasks.push_back(t);
+    L.fronts_cnt = (int64_t)LP.fronts.size();
+    for (int32_t k0 = 0; k0 < LP.max_s; k0 += NBP) {
+      StepPlan st;
+      st.k0 = k0;
+      st.panel_off = (int64_t)tasks.size();
+      for (int32_t t : LP.fronts)
+        if (sy.fronts[t].s > k0) tasks.push_back(t);
+      st.panel_cnt = (int64_t)tasks.size() - st.panel_off;
+      st.trsm_off = (int64_t)tasks.size();
+      for (int32_t t : LP.fronts) {
+        const FrontInfo &F = sy.fronts[t];
+        if (F.s <= k0) continue;
+        int32_t ke = k0 + std::min(NBP, F.s - k0);
+        for (int32_t c = ke; c < F.f; c += 64) { tasks.push_back(t); tasks.push_back(c); }
+      }
+      st.trsm_cnt = ((int64_t)tasks.size() - st.trsm_off) / 2;
+      st.gemm_off = (int64_t)tasks.size();
+      for (int32_t t : LP.fronts) {
+        const FrontInfo &F = sy.fronts[t];
+        if (F.s <= k0) continue;
+        int32_t ke = k0 + std::min(NBP, F.s - k0);
+        for (int32_t c = ke; c < F.f; c += TL)
+          for (int32_t r = ke; r < F.f; r += TL) { tasks.push_back(t); tasks.push_back(r); tasks.push_back(c); }
+      }
+      st.gemm_cnt = ((int64_t)tasks.size() - st.gemm_off) / 3;
+      L.steps.push_back(st);
+    }
+    for (int sl = 0; sl < 2; ++sl) {
+      L.ea_off[sl] = (int64_t)tasks.size();
+      for (int32_t t : LP.fronts) {
+        const FrontInfo &F = sy.fronts[t];
+        if (F.parent < 0 || F.slot != sl || F.u == 0) continue;
+        for (int32_t c = 0; c < F.u; c += EAC) { tasks.push_back(t); tasks.push_back(c); }
+      }
+      L.ea_cnt[sl] = ((int64_t)tasks.size() - L.ea_off[sl]) / 2;
+    }
+    dm.sched.push_back(L);
+  }
+  for (int c = 0; c < 2; ++c) {
+    dm.child_off[c] = (int64_t)tasks.size();
+    for (int32_t i = 0; i < nf; ++i) tasks.push_back(sy.fronts[i].child[c]);
+  }
+  dm.tasks.upload(tasks, s);
+  launch_elem_integrals(dm, s);
+  EMU_HIP(hipStreamSynchronize(s));
+  // the forward/backward solve kernels keep one front column in LDS
+  size_t lds = (size_t)dm.max_f * sizeof(zd);
+  if (lds > 160 * 1024) throw std::runtime_error("front order too large for the LDS-resident solve");
+}
+
+// ------------------------------------------------------------- workspace
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<void *> bufs;
+  std::vector<size_t> sizes;
+  std::vector<std::string> names;
+  void *get(const std::string &name, size_t bytes) {
+    for (size_t i = 0; i < names.size(); ++i)
+      if (names[i] == name) {
+        if (sizes[i] >= bytes) return bufs[i];
+        (void)hipFree(bufs[i]);
+        bufs[i] = nullptr;
+        if (hipMalloc(&bufs[i], bytes) != hipSuccess) throw HipError("hipMalloc failed for " + name);
+        sizes[i] = bytes;
+        return bufs[i];
+      }
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) throw HipError("hipMalloc failed for " + name);
+    names.push_back(name);
+    bufs.push_back(p);
+    sizes.push_back(bytes);
+    return p;
+  }
+  template <typename T>
+  T *buf(const std::string &name, size_t count) {
+    return reinterpret_cast<T *>(get(name, count * sizeof(T)));
+  }
+  ~Ctx() {
+    for (void *p : bufs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+Ctx *ctx_create(int device) {
+  int ndev = 0;
+  EMU_HIP(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) throw HipError("no HIP device " + std::to_string(device));
+  EMU_HIP(hipSetDevice(device));
+  Ctx *c = new Ctx();
+  c->device = device;
+  EMU_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  return c;
+}
+void ctx_destroy(Ctx *c) { delete c; }
+
+DevMesh *mesh_load(Ctx *c, const std::string &path, int leaf) {
+  EMU_HIP(hipSetDevice(c->device));
+  DevMesh *dm = new DevMesh();
+  try {
+    build_mesh(dm->host, path);
+    build_symbolic(dm->host, leaf > 0 ? leaf : 96);
+    devmesh_build(*dm, c->stream);
+  } catch (...) {
+    delete dm;
+    throw;
+  }
+  return dm;
+}
+
+// ---------------------------------------------------------------- assembly
+__global__ void k_store_norm(int nb, const double *__restrict__ nrm, zd *__restrict__ dst, int64_t sd) {
+  int b = threadIdx.x + blockIdx.x * blockDim.x;
+  if (b < nb) dst[b * sd] = zmk(nrm[b], 0.0);
+}
+
+static void upload_params(Ctx *c, DevMesh &dm, int nb, int ntyp, const double *pp, const double *qq,
+                          const double *shift, const double *bloch, zd **dpp, zd **dqq, zd **dsh, double **dbl) {
+  *dpp = c->buf<zd>("pp", (size_t)nb * ntyp);
+  *dqq = c->buf<zd>("qq", (size_t)nb * ntyp);
+  *dsh = c->buf<zd>("shift", nb);
+  *dbl = c->buf<double>("bloch", 2 * (size_t)nb);
+  EMU_HIP(hipMemcpyAsync(*dpp, pp, sizeof(zd) * nb * ntyp, hipMemcpyHostToDevice, c->stream));
+  EMU_HIP(hipMemcpyAsync(*dqq, qq, sizeof(zd) * nb * ntyp, hipMemcpyHostToDevice, c->stream));
+  EMU_HIP(hipMemcpyAsync(*dsh, shift, sizeof(zd) * nb, hipMemcpyHostToDevice, c->stream));
+  EMU_HIP(hipMemcpyAsync(*dbl, bloch, sizeof(double) * 2 * nb, hipMemcpyHostToDevice, c->stream));
+  EMU_HIP(hipStreamSynchronize(c->stream));
+}
+
+void assemble(Ctx *c, DevMesh &dm, int nb, int ntyp, const double *pp, const double *qq, const double *shift,
+              const double *bloch, double *mat1, double *mat2) {
+  EMU_HIP(hipSetDevice(c->device));
+  if (ntyp < dm.ntyp) throw std::invalid_argument("nb_typ_el smaller than the mesh's material count");
+  zd *dpp, *dqq, *dsh;
+  double *dbl;
+  upload_params(c, dm, nb, ntyp, pp, qq, shift, bloch, &dpp, &dqq, &dsh, &dbl);
+  const int64_t nnz = dm.host.nnz;
+  zd *ve = c->buf<zd>("ve", (size_t)nb * dm.host.nel * 14);
+  zd *m1 = c->buf<zd>("mat1", (size_t)nb * nnz);
+  zd *m2 = c->buf<zd>("mat2", (size_t)nb * nnz);
+  launch_phase(dm, nb, dbl, ve, c->stream);
+  launch_gather(dm, nb, ntyp, dpp, dqq, dsh, ve, m1, m2, c->stream);
+  if (mat1) EMU_HIP(hipMemcpyAsync(mat1, m1, sizeof(zd) * nb * nnz, hipMemcpyDeviceToHost, c->stream));
+  if (mat2) EMU_HIP(hipMemcpyAsync(mat2, m2, sizeof(zd) * nb * nnz, hipMemcpyDeviceToHost, c->stream));
+  EMU_HIP(hipStreamSynchronize(c->stream));
+}
+
+void lu_solve_host(Ctx *c, DevMesh &dm, int nb, const double *mat1, int nrhs, const double *b, double *x,
+                   int64_t *stats) {
+  EMU_HIP(hipSetDevice(c->device));
+  const Symbolic &sy = dm.host.sym;
+  const int64_t nnz = dm.host.nnz, n = dm.host.neq;
+  zd *m1 = c->buf<zd>("lu_mat1", (size_t)nb * nnz);
+  EMU_HIP(hipMemcpyAsync(m1, mat1, sizeof(zd) * nb * nnz, hipMemcpyHostToDevice, c->stream));
+  EMU_HIP(hipStreamSynchronize(c->stream));
+  zd *pool = c->buf<zd>("pool", (size_t)nb * sy.pool);
+  int32_t *piv = c->buf<int32_t>("piv", (size_t)nb * sy.npiv);
+  int32_t *pfl = c->buf<int32_t>("pflag", (size_t)nb * sy.npiv);
+  lu_factor(dm, nb, m1, pool, piv, pfl, c->stream);
+  zd *bo = c->buf<zd>("lu_b", (size_t)nb * nrhs * n);
+  zd *bn = c->buf<zd>("lu_bn", (size_t)nb * nrhs * n);
+  zd *yn = c->buf<zd>("lu_y", (size_t)nb * nrhs * n);
+  zd *xn = c->buf<zd>("lu_x", (size_t)nb * nrhs * n);
+  zd *w = c->buf<zd>("lu_w", (size_t)nb * nrhs * std::max<int64_t>(sy.nw, 1));
+  EMU_HIP(hipMemcpyAsync(bo, b, sizeof(zd) * nb * nrhs * n, hipMemcpyHostToDevice, c->stream));
+  EMU_HIP(hipStreamSynchronize(c->stream));
+  launch_permute_vec(dm, nb, nrhs, bo, bn, true, c->stream);
+  lu_solve(dm, nb, nrhs, pool, piv, bn, yn, xn, w, c->stream);
+  launch_permute_vec(dm, nb, nrhs, xn, bo, false, c->stream);
+  EMU_HIP(hipMemcpyAsync(x, bo, sizeof(zd) * nb * nrhs * n, hipMemcpyDeviceToHost, c->stream));
+  std::vector<int32_t> pf((size_t)nb * sy.npiv);
+  EMU_HIP(hipMemcpyAsync(pf.data(), pfl, sizeof(int32_t) * pf.size(), hipMemcpyDeviceToHost, c->stream));
+  EMU_HIP(hipStreamSynchronize(c->stream));
+  if (stats) stats[0] = std::accumulate(pf.begin(), pf.end(), (int64_t)0);
+}
+
+// ------------------------------------------------------ Krylov-Schur solve
+// Host-side state of the projected problem of one instance.
+struct KSState {
+  std::vector<cplx> H;   // (m+1) x m, column-major, ld = m+1
+  std::vector<cplx> T, Q;  // m x m Schur form and vectors
+  std::vector<cplx> theta;
+  int keep = 0;
+  bool conv = false;
+  int restarts = 0;
+};
+
+// Schur-decompose H[:m,:m], order the `keep` largest |theta| first and the
+// `nev` largest among them first of all, test residuals of the leading nev
+// Ritz pairs.  Returns true if converged.
+static bool ks_project(KSState &st, int m, int nev, int keep, double tol, std::vector<cplx> *ritzvec) {
+  int ld = m + 1;
+  st.T.assign((size_t)m * m, 0);
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) st.T[i + (size_t)j * m] = st.H[i + (size_t)j * ld];
+  st.Q.assign((size_t)m * m, 0);
+  st.theta.assign(m, 0);
+  int n = m, lda = m, sdim = 0, ldvs = m, info = 0, lwork = -1;
+  char jobvs = 'V', sort = 'N';
+  std::vector<double> rwork(m);
+  std::vector<int> bwork(m);
+  cplx wq;
+  g_zgees(&jobvs, &sort, dummy_select, &n, st.T.data(), &lda, &sdim, st.theta.data(), st.Q.data(), &ldvs, &wq,
+          &lwork, rwork.data(), bwork.data(), &info);
+  lwork = std::max(1, (int)wq.real());
+  std::vector<cplx> work(lwork);
+  g_zgees(&jobvs, &sort, dummy_select, &n, st.T.data(), &lda, &sdim, st.theta.data(), st.Q.data(), &ldvs,
+          work.data(), &lwork, rwork.data(), bwork.data(), &info);
+  if (info != 0) throw std::runtime_error("zgees failed, info=" + std::to_string(info));
+  auto reorder = [&](int count) {
+    std::vector<int> idx(m);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int b) { return std::abs(st.theta[a]) > std::abs(st.theta[b]); });
+    std::vector<int> sel(m, 0);
+    for (int i = 0; i < count; ++i) sel[idx[i]] = 1;
+    char job = 'N', compq = 'V';
+    int mm = 0, lw = std::max(1, m), inf = 0;
+    double sv = 0, sep = 0;
+    std::vector<cplx> wk(lw);
+    std::vector<cplx> w(m);
+    g_ztrsen(&job, &compq, sel.data(), &n, st.T.data(), &lda, st.Q.data(), &ldvs, w.data(), &mm, &sv, &sep,
+             wk.data(), &lw, &inf);
+    if (inf != 0) throw std::runtime_error("ztrsen failed, info=" + std::to_string(inf));
+    st.theta = w;
+  };
+  reorder(keep);
+  reorder(nev);
+  // residuals of the leading nev Ritz pairs
+  std::vector<cplx> Tn((size_t)nev * nev);
+  for (int j = 0; j < nev; ++j)
+    for (int i = 0; i < nev; ++i) Tn[i + (size_t)j * nev] = st.T[i + (size_t)j * m];
+  std::vector<cplx> VR((size_t)nev * nev), VL(1), wk(2 * nev);
+  std::vector<double> rw(nev);
+  char side = 'R', howmny = 'A';
+  int nn = nev, ldt = nev, ldvl = 1, ldvr = nev, mmv = nev, mout = 0, inf = 0;
+  g_ztrevc(&side, &howmny, nullptr, &nn, Tn.data(), &ldt, VL.data(), &ldvl, VR.data(), &ldvr, &mmv, &mout,
+           wk.data(), rw.data(), &inf);
+  if (inf != 0) throw std::runtime_error("ztrevc failed");
+  // b^T Q : last row of H times Q
+  std::vector<cplx> bq(nev, 0);
+  for (int j = 0; j < nev; ++j) {
+    cplx acc = 0;
+    for (int i = 0; i < m; ++i) acc += st.H[m + (size_t)i * ld] * st.Q[i + (size_t)j * m];
+    bq[j] = acc;
+  }
+  bool conv = true;
+  for (int k = 0; k < nev; ++k) {
+    cplx r = 0;
+    double ny = 0;
+    for (int j = 0; j <= k; ++j) {
+      r += bq[j] * VR[j + (size_t)k * nev];
+      ny += std::norm(VR[j + (size_t)k * nev]);
+    }
+    double res = std::abs(r) / std::sqrt(ny);
+    if (!(res <= tol * std::abs(st.theta[k]))) conv = false;
+  }
+  if (ritzvec) {
+    // W = Q[:, :nev] * VR (m x nev)
+    ritzvec->assign((size_t)m * nev, 0);
+    for (int k = 0; k < nev; ++k)
+      for (int j = 0; j <= k; ++j) {
+        cplx v = VR[j + (size_t)k * nev];
+        for (int i = 0; i < m; ++i) (*ritzvec)[i + (size_t)k * m] += st.Q[i + (size_t)j * m] * v;
+      }
+  }
+  return conv;
+}
+
+struct EigResult {
+  std::vector<cplx> mu;  // nb x nev
+  int64_t ops = 0, restarts = 0, steps = 0;
+};
+
+// nb instances in lockstep.  Operator: x -> A^{-1} M x in the ND numbering.
+// Output: Ritz vectors X (nb x n x nev, new numbering) in `Xout`.
+static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int itermax, double tol,
+                              const zd *pool, const int32_t *piv, const zd *mvals, zd *Xout) {
+  hipStream_t s = c->stream;
+  const Symbolic &sy = dm.host.sym;
+  const int64_t n = dm.host.neq;
+  const int64_t sV = (int64_t)(m + 1) * n;
+  zd *V = c->buf<zd>("ks_V", (size_t)nb * sV);
+  zd *V2 = c->buf<zd>("ks_V2", (size_t)nb * sV);
+  zd *t1 = c->buf<zd>("ks_t1", (size_t)nb * n);
+  zd *ty = c->buf<zd>("ks_ty", (size_t)nb * n);
+  zd *tw = c->buf<zd>("ks_w", (size_t)nb * std::max<int64_t>(sy.nw, 1));
+  zd *wv = c->buf<zd>("ks_wv", (size_t)nb * n);
+  zd *h1 = c->buf<zd>("ks_h1", (size_t)nb * (m + 1));
+  zd *h2 = c->buf<zd>("ks_h2", (size_t)nb * (m + 1));
+  const int64_t sH = (int64_t)(m + 1) * m;
+  zd *Hd = c->buf<zd>("ks_H", (size_t)nb * sH);
+  double *nrm = c->buf<double>("ks_nrm", nb);
+  double *nsc = c->buf<double>("ks_nsc", (size_t)nb * 64);
+  size_t atb_elems = (size_t)nb * (m + 1) * 256;
+  zd *atbs = c->buf<zd>("ks_atb", atb_elems);
+  zd *Qd = c->buf<zd>("ks_Q", (size_t)nb * m * m);
+  // start vector: deterministic pseudo-random
+  {
+    std::vector<zd> v0((size_t)nb * n);
+    std::mt19937_64 rng(12345);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (int b = 0; b < nb; ++b)
+      for (int64_t i = 0; i < n; ++i) v0[(size_t)b * n + i] = zmk(U(rng), U(rng));
+    EMU_HIP(hipMemcpyAsync(t1, v0.data(), sizeof(zd) * v0.size(), hipMemcpyHostToDevice, s));
+    EMU_HIP(hipStreamSynchronize(s));
+    launch_norm_scale(nb, n, t1, n, V, sV, nrm, nsc, s);
+  }
+  launch_fill(Hd, (int64_t)nb * sH, zmk(0, 0), s);
+  std::vector<KSState> st(nb);
+  for (auto &x : st) x.H.assign((size_t)sH, 0);
+  EigResult res;
+  res.mu.assign((size_t)nb * nev, 0);
+  int k = 0;
+  const int keep = std::min(m - 1, nev + (m - nev) / 2);
+  bool all_conv = false;
+  std::vector<zd> Hh((size_t)nb * sH);
+  for (int it = 0; it <= itermax; ++it) {
+    for (int j = k; j < m; ++j) {
+      zd *vj = V + (int64_t)j * n;
+      launch_spmm(nb, n, 1, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, vj, sV, t1, n, s);
+      lu_solve(dm, nb, 1, pool, piv, t1, ty, wv, tw, s);
+      // CGS with one re-orthogonalisation
+      launch_atb(nb, n, j + 1, 1, V, n, sV, wv, n, n, h1, m + 1, m + 1, true, atbs, atb_elems, s);
+      launch_sub_vh(nb, n, j + 1, V, n, sV, h1, m + 1, wv, n, s);
+      launch_atb(nb, n, j + 1, 1, V, n, sV, wv, n, n, h2, m + 1, m + 1, true, atbs, atb_elems, s);
+      launch_sub_vh(nb, n, j + 1, V, n, sV, h2, m + 1, wv, n, s);
+      launch_axpy_h(nb, j + 1, h1, h2, m + 1, Hd + (int64_t)j * (m + 1), sH, s);
+      launch_norm_scale(nb, n, wv, n, V + (int64_t)(j + 1) * n, sV, nrm, nsc, s);
+      // H(j+1, j) = ||w||
+      hipLaunchKernelGGL(k_store_norm, dim3((nb + 63) / 64), dim3(64), 0, s, nb, nrm,
+                         Hd + (int64_t)j * (m + 1) + j + 1, sH);
+      res.ops += nb;
+      res.steps += 1;
+    }
+    EMU_HIP(hipMemcpyAsync(Hh.data(), Hd, sizeof(zd) * Hh.size(), hipMemcpyDeviceToHost, s));
+    EMU_HIP(hipStreamSynchronize(s));
+    bool last = (it == itermax);
+    all_conv = true;
+    std::vector<std::thread> th;
+    std::vector<std::string> errs(nb);
+    std::vector<char> convflag(nb, 0);
+    for (int b = 0; b < nb; ++b) {
+      th.emplace_back([&, b]() {
+        try {
+          KSState &S = st[b];
+          for (int64_t i = 0; i < sH; ++i) S.H[i] = cplx(Hh[(size_t)b * sH + i].x, Hh[(size_t)b * sH + i].y);
+          convflag[b] = ks_project(S, m, nev, keep, tol, nullptr) ? 1 : 0;
+        } catch (std::exception &e) {
+          errs[b] = e.what();
+        }
+      });
+    }
+    for (auto &t : th) t.join();
+    for (int b = 0; b < nb; ++b) {
+      if (!errs[b].empty()) throw std::runtime_error(errs[b]);
+      st[b].conv = convflag[b];
+      if (!convflag[b]) all_conv = false;
+    }
+    res.restarts = it;
+    if (all_conv || last) break;
+    // restart: V2[:, :keep] = V[:, :m] Q[:, :keep]; V2[:, keep] = V[:, m]
+    std::vector<zd> Qh((size_t)nb * m * keep);
+    std::vector<zd> Hn((size_t)nb * sH, zmk(0, 0));
+    for (int b = 0; b < nb; ++b) {
+      KSState &S = st[b];
+      for (int j = 0; j < keep; ++j)
+        for (int i = 0; i < m; ++i) {
+          cplx q = S.Q[i + (size_t)j * m];
+          Qh[(size_t)b * m * keep + i + (size_t)j * m] = zmk(q.real(), q.imag());
+        }
+      for (int j = 0; j < keep; ++j) {
+        for (int i = 0; i <= j; ++i) {
+          cplx t = S.T[i + (size_t)j * m];
+          Hn[(size_t)b * sH + i + (size_t)j * (m + 1)] = zmk(t.real(), t.imag());
+        }
+        cplx acc = 0;
+        for (int i = 0; i < m; ++i) acc += S.H[m + (size_t)i * (m + 1)] * S.Q[i + (size_t)j * m];
+        Hn[(size_t)b * sH + keep + (size_t)j * (m + 1)] = zmk(acc.real(), acc.imag());
+      }
+    }
+    EMU_HIP(hipMemcpyAsync(Qd, Qh.data(), sizeof(zd) * Qh.size(), hipMemcpyHostToDevice, s));
+    EMU_HIP(hipMemcpyAsync(Hd, Hn.data(), sizeof(zd) * Hn.size(), hipMemcpyHostToDevice, s));
+    EMU_HIP(hipStreamSynchronize(s));
+    launch_gemm_nn(nb, n, keep, m, zmk(1, 0), V, n, sV, Qd, m, (int64_t)m * keep, zmk(0, 0), V2, n, sV, s);
+    launch_copy(nb, n, V + (int64_t)m * n, sV, V2 + (int64_t)keep * n, sV, s);
+    std::swap(V, V2);
+    k = keep;
+  }
+  if (!all_conv) throw NoConvergence("Krylov-Schur: eigenpairs did not converge within itermax restarts");
+  // Ritz vectors X = V[:, :m] W, W = Q[:, :nev] Y
+  std::vector<zd> Wh((size_t)nb * m * nev);
+  for (int b = 0; b < nb; ++b) {
+    std::vector<cplx> W;
+    ks_project(st[b], m, nev, keep, tol, &W);
+    for (size_t i = 0; i < W.size(); ++i) Wh[(size_t)b * m * nev + i] = zmk(W[i].real(), W[i].imag());
+    for (int i = 0; i < nev; ++i) res.mu[(size_t)b * nev + i] = st[b].theta[i];
+  }
+  zd *Wd = c->buf<zd>("ks_W", (size_t)nb * m * nev);
+  EMU_HIP(hipMemcpyAsync(Wd, Wh.data(), sizeof(zd) * Wh.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipStreamSynchronize(s));
+  launch_gemm_nn(nb, n, nev, m, zmk(1, 0), V, n, sV, Wd, m, (int64_t)m * nev, zmk(0, 0), Xout, n, (int64_t)n * nev, s);
+  EMU_HIP(hipStreamSynchronize(s));
+  return res;
+}
+
+// -------------------------------------------------------- host utilities
+
+// Numerical-Recipes quicksort index (z_indexx.f / sort_n.f): ascending key,
+// not stable; reproduced so that ties break as in the reference.
+static std::vector<int> nr_indexx(const std::vector<double> &key) {
+  const int n = (int)key.size();
+  std::vector<int> indx(n);
+  std::iota(indx.begin(), indx.end(), 0);
+  if (n < 2) return indx;
+  const int M = 7;
+  std::vector<int> istack;
+  int l = 0, ir = n - 1;
+  for (;;) {
+    if (ir - l < M) {
+      for (int j = l + 1; j <= ir; ++j) {
+        int indxt = indx[j];
+        double a = key[indxt];
+        int i = j - 1;
+        for (; i >= l; --i) {
+          if (key[indx[i]] <= a) break;
+          indx[i + 1] = indx[i];
+        }
+        indx[i + 1] = indxt;
+      }
+      if (istack.empty()) return indx;
+      ir = istack.back(); istack.pop_back();
+      l = istack.back(); istack.pop_back();
+    } else {
+      int k = (l + ir) / 2;
+      std::swap(indx[k], indx[l + 1]);
+      if (key[indx[l]] > key[indx[ir]]) std::swap(indx[l], indx[ir]);
+      if (key[indx[l + 1]] > key[indx[ir]]) std::swap(indx[l + 1], indx[ir]);
+      if (key[indx[l]] > key[indx[l + 1]]) std::swap(indx[l], indx[l + 1]);
+      int i = l + 1, j = ir;
+      int indxt = indx[l + 1];
+      double a = key[indxt];
+      for (;;) {
+        do ++i; while (key[indx[i]] < a);
+        do --j; while (key[indx[j]] > a);
+        if (j < i) break;
+        std::swap(indx[i], indx[j]);
+      }
+      indx[l + 1] = indx[j];
+      indx[j] = indxt;
+      if (ir - i + 1 >= j - l) {
+        istack.push_back(i);
+        istack.push_back(ir);
+        ir = j - 1;
+      } else {
+        istack.push_back(l);
+        istack.push_back(j - 1);
+        l = i;
+      }
+    }
+  }
+}
+
+// beta = sqrt(1/mu + shift), forward branch (py_calc_modes.f:664-679)
+static cplx beta_from_mu(cplx mu, cplx shift) {
+  cplx b = std::sqrt(1.0 / mu + shift);
+  if (std::abs(b.imag()) < 1e-8) {
+    if (b.real() < 0) b = -b;
+  } else if (b.imag() < 0) {
+    b = -b;
+  }
+  return b;
+}
+
+// Conjugate-pair reordering of the adjoint modes (orthogonal.f:251-301):
+// returns the swaps (j, j2) in the order the reference applies them.
+static std::vector<std::pair<int, int>> pair_swaps(const std::vector<cplx> &L, int nval) {
+  std::vector<std::pair<int, int>> sw;
+  int j = 0, skip = 0;
+  while (j < nval) {
+    if (std::abs(L[j + (size_t)j * nval]) > 1e-7) {
+      ++j;
+    } else if (skip == 0) {
+      for (int j2 = j + 1; j2 < nval; ++j2)
+        if (std::abs(L[j2 + (size_t)j * nval]) > 1e-7) sw.push_back({j, j2});
+      skip = 1;
+      ++j;
+    } else {
+      skip = 0;
+      ++j;
+    }
+  }
+  return sw;
+}
+
+// ------------------------------------------------------------ calc_modes
+void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
+  EMU_HIP(hipSetDevice(c->device));
+  if (!lapack_ready()) throw LapackMissing("host LAPACK routines not registered (emu_set_lapack)");
+  hipStream_t s = c->stream;
+  const Mesh &mh = dm.host;
+  const int64_t n = mh.neq, nnz = mh.nnz, nel = mh.nel;
+  const int nl = (int)a.nl, nb = 2 * nl, ntyp = (int)a.ntyp, nval = (int)a.nval;
+  if (nl < 1 || nval < 1 || ntyp < dm.ntyp) throw std::invalid_argument("calc_modes: bad sizes");
+  const int m = (int)std::min<int64_t>(2 * nval + nval / 2 + 3, n - 1);  // py_calc_modes.f:256
+  if (nval >= m) throw std::invalid_argument("calc_modes: num_BMs too large for the mesh");
+  // material coefficients (py_calc_modes.f:583-599); prime = +bloch, adjoint = -bloch
+  std::vector<cplx> pp((size_t)nb * ntyp), qq((size_t)nb * ntyp), sh(nb);
+  std::vector<double> bl(2 * (size_t)nb);
+  std::vector<double> k0(nl);
+  for (int i = 0; i < nl; ++i) {
+    k0[i] = 2.0 * M_PI / a.lambda[i];
+    std::vector<cplx> eps(ntyp);
+    for (int t = 0; t < ntyp; ++t) {
+      cplx ne(a.n_eff[2 * ((size_t)i * ntyp + t)], a.n_eff[2 * ((size_t)i * ntyp + t) + 1]);
+      eps[t] = ne * ne;
+    }
+    bool homog = true;
+    for (int t = 0; t + 1 < ntyp; ++t)
+      if (eps[t].real() != eps[t + 1].real() || eps[t].imag() != eps[t + 1].imag()) homog = false;
+    if (homog) throw Homogeneous("FEM routine cannot handle homogeneous layers; define the layer as a ThinFilm");
+    for (int h = 0; h < 2; ++h) {
+      int b = 2 * i + h;
+      for (int t = 0; t < ntyp; ++t) {
+        if (a.eh == 1) {
+          qq[(size_t)b * ntyp + t] = eps[t] * k0[i] * k0[i];
+          pp[(size_t)b * ntyp + t] = 1.0;
+        } else {
+          qq[(size_t)b * ntyp + t] = k0[i] * k0[i];
+          pp[(size_t)b * ntyp + t] = 1.0 / eps[t];
+        }
+      }
+      sh[b] = cplx(a.shift[2 * i], a.shift[2 * i + 1]);
+      double sg = h == 0 ? 1.0 : -1.0;
+      bl[2 * b] = sg * a.bloch[2 * i];
+      bl[2 * b + 1] = sg * a.bloch[2 * i + 1];
+    }
+  }
+  zd *dpp, *dqq, *dsh;
+  double *dbl;
+  upload_params(c, dm, nb, ntyp, reinterpret_cast<const double *>(pp.data()),
+                reinterpret_cast<const double *>(qq.data()), reinterpret_cast<const double *>(sh.data()), bl.data(),
+                &dpp, &dqq, &dsh, &dbl);
+  // 1. assembly
+  zd *ve = c->buf<zd>("ve", (size_t)nb * nel * 14);
+  zd *m1 = c->buf<zd>("mat1", (size_t)nb * nnz);
+  zd *m2 = c->buf<zd>("mat2", (size_t)nb * nnz);
+  zd *m2p = c->buf<zd>("mat2p", (size_t)nb * nnz);
+  launch_phase(dm, nb, dbl, ve, s);
+  launch_gather(dm, nb, ntyp, dpp, dqq, dsh, ve, m1, m2, s);
+  launch_permute_vals(dm, nb, m2, m2p, s);
+  // 2. LU of K - shift M
+  const Symbolic &sy = mh.sym;
+  zd *pool = c->buf<zd>("pool", (size_t)nb * sy.pool);
+  int32_t *piv = c->buf<int32_t>("piv", (size_t)nb * sy.npiv);
+  int32_t *pfl = c->buf<int32_t>("pflag", (size_t)nb * sy.npiv);
+  lu_factor(dm, nb, m1, pool, piv, pfl, s);
+  // 3. eigensolve
+  zd *Xn = c->buf<zd>("Xn", (size_t)nb * n * nval);
+  EigResult er = krylov_schur(c, dm, nb, nval, m, (int)a.itermax, a.tol, pool, piv, m2p, Xn);
+  zd *Xo = c->buf<zd>("Xo", (size_t)nb * n * nval);
+  launch_permute_vec(dm, nb, nval, Xn, Xo, false, s);
+  // 4. beta, sort (z_indexx on -Re(beta^2))
+  std::vector<cplx> beta((size_t)nb * nval);
+  std::vector<int32_t> order((size_t)nb * nval);
+  for (int b = 0; b < nb; ++b) {
+    std::vector<cplx> bb(nval);
+    std::vector<double> key(nval);
+    for (int k = 0; k < nval; ++k) {
+      bb[k] = beta_from_mu(er.mu[(size_t)b * nval + k], sh[b]);
+      key[k] = -(bb[k] * bb[k]).real();
+    }
+    std::vector<int> idx = nr_indexx(key);
+    for (int k = 0; k < nval; ++k) {
+      beta[(size_t)b * nval + k] = bb[idx[k]];
+      order[(size_t)b * nval + k] = idx[k];
+    }
+  }
+  int32_t *dord = c->buf<int32_t>("order", (size_t)nb * nval);
+  EMU_HIP(hipMemcpyAsync(dord, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipStreamSynchronize(s));
+  launch_permute_cols(nb, n, nval, Xo, n * nval, dord, Xn, n * nval, s);  // sorted modes in Xn
+  // 5. field normalisation (array_sol.f:297-339)
+  zd *zmax = c->buf<zd>("zmax", (size_t)nb * nval);
+  size_t nparts = (size_t)((nel + 255) / 256);
+  double *fms = c->buf<double>("fmax_scratch", (size_t)nb * nval * nparts * 4);
+  launch_field_max(dm, nb, nval, Xn, n, n * nval, ve, false, zmax, fms, s);
+  launch_scale_cols(nb, n, nval, Xn, n, n * nval, zmax, nval, true, s);
+  // 6. orthogonality overlap L = -(1/k0) X2^T M_prime X1 diag(beta1) (orthogonal.f:95-248)
+  zd *Y = c->buf<zd>("ovl_Y", (size_t)nl * n * nval);
+  launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, m2, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s);
+  zd *Ld = c->buf<zd>("ovl_L", (size_t)nl * nval * nval);
+  size_t atb_elems = (size_t)nl * nval * nval * 64 + (size_t)nb * 2 * 113 * nval * 64;
+  zd *atbs = c->buf<zd>("post_atb", atb_elems);
+  launch_atb(nl, n, nval, nval, Xn + n * nval, n, 2 * n * nval, Y, n, n * nval, Ld, nval, (int64_t)nval * nval,
+             false, atbs, atb_elems, s);
+  std::vector<cplx> Lh((size_t)nl * nval * nval);
+  EMU_HIP(hipMemcpyAsync(Lh.data(), Ld, sizeof(zd) * Lh.size(), hipMemcpyDeviceToHost, s));
+  EMU_HIP(hipStreamSynchronize(s));
+  std::vector<int32_t> perm2((size_t)nb * nval);
+  std::vector<zd> scl((size_t)nb * nval, zmk(1, 0));
+  for (int i = 0; i < nl; ++i) {
+    cplx *L = &Lh[(size_t)i * nval * nval];
+    for (int j = 0; j < nval; ++j) {
+      cplx f = beta[(size_t)(2 * i) * nval + j] * (-1.0 / k0[i]);
+      for (int r = 0; r < nval; ++r) L[r + (size_t)j * nval] *= f;
+    }
+    std::vector<int> p(nval);
+    std::iota(p.begin(), p.end(), 0);
+    std::vector<cplx> Lv(L, L + (size_t)nval * nval);
+    for (auto &sw : pair_swaps(Lv, nval)) std::swap(p[sw.first], p[sw.second]);
+    for (int k = 0; k < nval; ++k) {
+      perm2[(size_t)(2 * i) * nval + k] = k;
+      perm2[(size_t)(2 * i + 1) * nval + k] = p[k];
+      cplx d = std::sqrt(L[p[k] + (size_t)k * nval]);  // diagonal after the swaps
+      if (std::abs(d) > 1e-8) {
+        cplx f = 1.0 / (d * d);
+        scl[(size_t)(2 * i + 1) * nval + k] = zmk(f.real(), f.imag());
+      }
+    }
+  }
+  EMU_HIP(hipMemcpyAsync(dord, perm2.data(), sizeof(int32_t) * perm2.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipMemcpyAsync(zmax, scl.data(), sizeof(zd) * scl.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipStreamSynchronize(s));
+  launch_permute_cols(nb, n, nval, Xn, n * nval, dord, Xo, n * nval, s);
+  launch_scale_cols(nb, n, nval, Xo, n, n * nval, zmax, nval, false, s);
+  // 7. plane-wave ordering (pw_ordering.f:48-78) and J / J^dagger
+  const double d = mh.lat[0], dy = mh.lat[3];
+  std::vector<std::pair<int, int>> pws;
+  for (int px = -(int)a.ordre; px <= (int)a.ordre; ++px)
+    for (int py = -(int)a.ordre; py <= (int)a.ordre; ++py)
+      if ((px / d) * (px / d) + (py / dy) * (py / dy) <= (a.ordre / std::max(d, dy)) * (a.ordre / std::max(d, dy)))
+        pws.push_back({px, py});
+  const int P = (int)pws.size();
+  if (P != a.neq_pw) throw std::invalid_argument("neq_PW does not match the circular plane-wave truncation");
+  std::vector<double> al((size_t)nb * P), be((size_t)nb * P);
+  std::vector<int32_t> pos((size_t)nb * P), sgn(nb);
+  for (int i = 0; i < nl; ++i) {
+    std::vector<double> key(P);
+    for (int q = 0; q < P; ++q) {
+      double aa = a.bloch[2 * i] + 2 * M_PI / d * pws[q].first;
+      double bb = a.bloch[2 * i + 1] + 2 * M_PI / dy * pws[q].second;
+      cplx kz = std::sqrt(cplx(k0[i] * k0[i] - aa * aa - bb * bb, 0.0));
+      key[q] = -(kz * kz).real();
+      for (int h = 0; h < 2; ++h) {
+        al[(size_t)(2 * i + h) * P + q] = aa;
+        be[(size_t)(2 * i + h) * P + q] = bb;
+      }
+    }
+    std::vector<int> idx = nr_indexx(key);
+    for (int r = 0; r < P; ++r) {
+      pos[(size_t)(2 * i) * P + idx[r]] = r;
+      pos[(size_t)(2 * i + 1) * P + idx[r]] = r;
+    }
+    sgn[2 * i] = -1;
+    sgn[2 * i + 1] = 1;
+  }
+  double *dal = c->buf<double>("pw_al", (size_t)nb * P);
+  double *dbe = c->buf<double>("pw_be", (size_t)nb * P);
+  int32_t *dpos = c->buf<int32_t>("pw_pos", (size_t)nb * P);
+  int32_t *dsg = c->buf<int32_t>("pw_sg", nb);
+  EMU_HIP(hipMemcpyAsync(dal, al.data(), sizeof(double) * al.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipMemcpyAsync(dbe, be.data(), sizeof(double) * be.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipMemcpyAsync(dpos, pos.data(), sizeof(int32_t) * pos.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipMemcpyAsync(dsg, sgn.data(), sizeof(int32_t) * sgn.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipStreamSynchronize(s));
+  zd *JB = c->buf<zd>("pw_JB", (size_t)nb * nel * 12 * 2 * P);
+  zd *G = c->buf<zd>("pw_G", (size_t)nb * 2 * P * n);
+  launch_pw_load(dm, nb, P, dal, dbe, dpos, dsg, ve, JB, G, s);
+  zd *R = c->buf<zd>("pw_R", (size_t)nb * 2 * P * nval);
+  launch_atb(nb, n, 2 * P, nval, G, n, 2 * P * n, Xo, n, n * nval, R, 2 * P, (int64_t)2 * P * nval, false, atbs,
+             atb_elems, s);
+  std::vector<zd> Rh((size_t)nb * 2 * P * nval);
+  EMU_HIP(hipMemcpyAsync(Rh.data(), R, sizeof(zd) * Rh.size(), hipMemcpyDeviceToHost, s));
+  std::vector<int32_t> pf((size_t)nb * sy.npiv);
+  EMU_HIP(hipMemcpyAsync(pf.data(), pfl, sizeof(int32_t) * pf.size(), hipMemcpyDeviceToHost, s));
+  EMU_HIP(hipStreamSynchronize(s));
+  // outputs
+  const int P2 = 2 * P;
+  for (int i = 0; i < nl; ++i) {
+    for (int k = 0; k < nval; ++k) {
+      a.beta[2 * ((size_t)i * nval + k)] = beta[(size_t)(2 * i) * nval + k].real();
+      a.beta[2 * ((size_t)i * nval + k) + 1] = beta[(size_t)(2 * i) * nval + k].imag();
+    }
+    const zd *Rp = &Rh[(size_t)(2 * i) * P2 * nval];
+    const zd *Ra = &Rh[(size_t)(2 * i + 1) * P2 * nval];
+    double *J = a.J + (size_t)i * 2 * P2 * nval;
+    double *Jd = a.Jd + (size_t)i * 2 * P2 * nval;
+    for (int k = 0; k < nval; ++k)
+      for (int q = 0; q < P2; ++q) {
+        J[2 * (q + (size_t)k * P2)] = Rp[q + (size_t)k * P2].x;
+        J[2 * (q + (size_t)k * P2) + 1] = Rp[q + (size_t)k * P2].y;
+        Jd[2 * (k + (size_t)q * nval)] = Ra[q + (size_t)k * P2].x;
+        Jd[2 * (k + (size_t)q * nval) + 1] = Ra[q + (size_t)k * P2].y;
+      }
+  }
+  if (a.stats) {
+    a.stats[0] = er.ops;
+    a.stats[1] = er.restarts;
+    a.stats[2] = std::accumulate(pf.begin(), pf.end(), (int64_t)0);
+    a.stats[3] = er.steps;
+  }
+}
+
+void mesh_free(DevMesh *m) { delete m; }
+
+DevMesh *mesh_analyze(const std::string &path, int leaf) {
+  DevMesh *dm = new DevMesh();
+  try {
+    build_mesh(dm->host, path);
+    build_symbolic(dm->host, leaf > 0 ? leaf : 96);
+  } catch (...) {
+    delete dm;
+    throw;
+  }
+  return dm;
+}
+
+void mesh_symbolic(const DevMesh *dm, int32_t *fronts, int64_t *foff, int64_t *uoff, int32_t *uidx, int32_t *pmap,
+                   int64_t *nnz2front) {
+  const Symbolic &sy = dm->host.sym;
+  for (size_t i = 0; i < sy.fronts.size(); ++i) {
+    const FrontInfo &F = sy.fronts[i];
+    if (fronts) {
+      int32_t *r = fronts + 8 * i;
+      r[0] = F.s; r[1] = F.u; r[2] = F.f; r[3] = F.b; r[4] = F.parent; r[5] = F.level;
+      r[6] = F.child[0]; r[7] = F.child[1];
+    }
+    if (foff) foff[i] = F.off;
+    if (uoff) uoff[i] = F.uoff;
+  }
+  if (uidx) std::copy(sy.uidx.begin(), sy.uidx.end(), uidx);
+  if (pmap) std::copy(sy.pmap.begin(), sy.pmap.end(), pmap);
+  if (nnz2front) std::copy(sy.nnz2front.begin(), sy.nnz2front.end(), nnz2front);
+}
+
+void mesh_info(const DevMesh *dm, int64_t *info) {
+  const Mesh &m = dm->host;
+  const Symbolic &sy = m.sym;
+  int64_t v[16] = {m.npt, m.nel, m.n_ddl, m.neq, m.nnz, (int64_t)sy.fronts.size(), (int64_t)sy.levels.size(),
+                   sy.pool, sy.max_f, sy.factor_entries, (int64_t)sy.flops, m.n_edge, m.npt_p3, sy.nw, 0, 0};
+  std::memcpy(info, v, sizeof(v));
+}
+
+void mesh_arrays(const DevMesh *dm, int64_t *row_ptr, int32_t *col_ind, int32_t *ldof, int32_t *perm, double *lat) {
+  const Mesh &m = dm->host;
+  if (row_ptr) std::copy(m.row_ptr.begin(), m.row_ptr.end(), row_ptr);
+  if (col_ind) std::copy(m.col_ind.begin(), m.col_ind.end(), col_ind);
+  if (ldof) std::copy(m.ldof.begin(), m.ldof.end(), ldof);
+  if (perm) std::copy(m.sym.perm.begin(), m.sym.perm.end(), perm);
+  if (lat) std::copy(m.lat, m.lat + 4, lat);
+}
+
+void mesh_geometry(const DevMesh *dm, int64_t *table_nod, int64_t *type_el, double *x_arr) {
+  const Mesh &m = dm->host;
+  if (table_nod) std::copy(m.table_nod.begin(), m.table_nod.end(), table_nod);
+  if (type_el) std::copy(m.type_el.begin(), m.type_el.end(), type_el);
+  if (x_arr) std::copy(m.x.begin(), m.x.end(), x_arr);
+}
+
+}  // namespace emu

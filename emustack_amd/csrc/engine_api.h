@@ -1,0 +1,55 @@
+// Internal interface between the C-ABI layer (capi.cpp) and the engine.
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace emu {
+
+struct Ctx;
+struct DevMesh;
+
+struct NoConvergence : std::runtime_error {
+  explicit NoConvergence(const std::string &s) : std::runtime_error(s) {}
+};
+struct Homogeneous : std::runtime_error {
+  explicit Homogeneous(const std::string &s) : std::runtime_error(s) {}
+};
+struct LapackMissing : std::runtime_error {
+  explicit LapackMissing(const std::string &s) : std::runtime_error(s) {}
+};
+
+void set_lapack(void *zgees, void *ztrsen, void *ztrevc);
+bool lapack_ready();
+Ctx *ctx_create(int device);
+void ctx_destroy(Ctx *c);
+DevMesh *mesh_load(Ctx *c, const std::string &path, int leaf);
+void mesh_free(DevMesh *m);
+DevMesh *mesh_analyze(const std::string &path, int leaf);
+void mesh_symbolic(const DevMesh *m, int32_t *fronts, int64_t *foff, int64_t *uoff, int32_t *uidx, int32_t *pmap,
+                   int64_t *nnz2front);
+void mesh_info(const DevMesh *m, int64_t *info);
+void mesh_arrays(const DevMesh *m, int64_t *row_ptr, int32_t *col_ind, int32_t *ldof, int32_t *perm, double *lat);
+void mesh_geometry(const DevMesh *m, int64_t *table_nod, int64_t *type_el, double *x_arr);
+
+void assemble(Ctx *c, DevMesh &dm, int nb, int ntyp, const double *pp, const double *qq, const double *shift,
+              const double *bloch, double *mat1, double *mat2);
+void lu_solve_host(Ctx *c, DevMesh &dm, int nb, const double *mat1, int nrhs, const double *b, double *x,
+                   int64_t *stats);
+
+struct ModesArgs {
+  int64_t nl = 1;           // number of (lambda, bloch) points
+  const double *lambda = nullptr;
+  int64_t nval = 0, ordre = 0, ntyp = 0, eh = 1, itermax = 30, neq_pw = 0;
+  const double *n_eff = nullptr;   // nl x ntyp complex
+  const double *bloch = nullptr;   // nl x 2
+  const double *shift = nullptr;   // nl complex
+  double tol = 1e-12;
+  double *beta = nullptr;          // nl x nval complex
+  double *J = nullptr;             // nl x (2P x nval) complex
+  double *Jd = nullptr;            // nl x (nval x 2P) complex
+  int64_t *stats = nullptr;
+};
+void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a);
+
+}  // namespace emu

@@ -1,0 +1,141 @@
+/* C-ABI of libemu_mi355x.so: EMUstack's per-wavelength FEM + scattering
+ * path on one MI355X (gfx950).
+ *
+ * The reference exposes this path to Python as the f2py module
+ * `fortran.EMUstack` (backend/fortran/Makefile:16-18), imported at
+ * backend/mode_calcs.py:30 and backend/objects.py:29.  Its hot entry point
+ * is EMUstack.calc_modes_2d (backend/fortran/py_calc_modes.f:1-11), called
+ * positionally at backend/mode_calcs.py:371-378.  The entry points below
+ * replace it.  Plain pointers and sizes only; complex numbers are
+ * interleaved (re, im) doubles; matrices are column-major (Fortran order)
+ * exactly as f2py hands them to Python.
+ *
+ * Every function returns 0 on success or a negative EMU_E_* status; the
+ * message of the last failure on the calling thread is emu_last_error().
+ * A context owns one HIP stream on one device; calls on one context are
+ * serialised by the caller (one context per host thread).
+ */
+#ifndef EMUSTACK_MI355X_H
+#define EMUSTACK_MI355X_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  EMU_OK = 0,
+  EMU_E_ARG = -1,         /* invalid argument (sizes, null pointers) */
+  EMU_E_MESH = -2,        /* unreadable / non-periodic mesh (periodic_node.f:108-128) */
+  EMU_E_HIP = -3,         /* HIP runtime error or no device */
+  EMU_E_NOCONV = -4,      /* eigensolver did not converge (py_calc_modes.f:647-656) */
+  EMU_E_HOMOGENEOUS = -5, /* all materials equal (py_calc_modes.f:553-567) */
+  EMU_E_LAPACK = -6,      /* host LAPACK for the projected problem missing/failed */
+  EMU_E_OOM = -7          /* device allocation failed */
+};
+
+typedef struct emu_ctx emu_ctx;
+typedef struct emu_mesh emu_mesh;
+
+/* Message of the last failure on this thread (never NULL). */
+const char *emu_last_error(void);
+
+/* Context on HIP device `device` with its own stream. */
+int emu_ctx_create(int device, emu_ctx **out);
+void emu_ctx_destroy(emu_ctx *ctx);
+
+/* Host LAPACK routines (zgees, ztrsen, ztrevc; LAPACK calling convention,
+ * all arguments by pointer, no hidden string lengths) used for the small
+ * ncv x ncv projected Schur problem of the Krylov-Schur eigensolver -- the
+ * role zlahqr/ztrevc play inside ARPACK (zarpack.f:3960 zneigh).  Must be set
+ * once per process before the first eigensolve. */
+int emu_set_lapack(void *zgees, void *ztrsen, void *ztrevc);
+
+/* Load a .mail mesh: topology, DOF numbering, periodic merge, sparse
+ * pattern (py_calc_modes.f:297-522), nested-dissection symbolic LU analysis
+ * and upload of the per-mesh element integrals.  The reference redoes this
+ * per wavelength inside calc_modes_2d; here it is cached in the handle.
+ * leaf_size <= 0 selects the default. */
+int emu_mesh_load(emu_ctx *ctx, const char *path, int leaf_size, emu_mesh **out);
+void emu_mesh_free(emu_mesh *mesh);
+
+/* Host-only analysis of a .mail mesh (no device needed): the same
+ * preprocessing and symbolic analysis as emu_mesh_load, without upload.
+ * The handle serves emu_mesh_info / _arrays / _geometry / _symbolic only. */
+int emu_mesh_analyze(const char *path, int leaf_size, emu_mesh **out);
+
+/* Symbolic multifrontal structure (any pointer may be NULL):
+ * fronts[nfronts*8] = (s, u, f, b, parent, level, child0, child1) per front,
+ * foff[nfronts] = offset of the f x f column-major front in the pool,
+ * uoff[nfronts] = offset of its update indices, uidx/pmap[total update
+ * entries] = update rows (nested-dissection numbering) and their positions
+ * in the parent front, nnz2front[nnz] = pool offset of every CSR entry. */
+int emu_mesh_symbolic(const emu_mesh *mesh, int32_t *fronts, int64_t *foff, int64_t *uoff,
+                      int32_t *uidx, int32_t *pmap, int64_t *nnz2front);
+
+/* info[0..15]: npt, nel, n_ddl, neq, nnz, nfronts, nlevels, pool entries,
+ * max front order, factor entries, LU flops (as int64), n_edge, npt_p3,
+ * total update entries, 0, 0 */
+int emu_mesh_info(const emu_mesh *mesh, int64_t *info);
+/* Host copies of per-mesh arrays (any pointer may be NULL):
+ * row_ptr[neq+1], col_ind[nnz], ldof[nel*22], perm[neq] (new -> old),
+ * lat[4] (lattice vectors v1, v2). */
+int emu_mesh_arrays(const emu_mesh *mesh, int64_t *row_ptr, int32_t *col_ind,
+                    int32_t *ldof, int32_t *perm, double *lat);
+/* Mesh arrays returned by f2py's calc_modes_2d: table_nod[6*nel] (1-based,
+ * column-major 6 x nel), type_el[nel], x_arr[2*npt]. */
+int emu_mesh_geometry(const emu_mesh *mesh, int64_t *table_nod, int64_t *type_el,
+                      double *x_arr);
+
+/* Drop-in for EMUstack.calc_modes_2d (py_calc_modes.f:1-11, E-field,
+ * periodic boundary).  Inputs as the f2py call at mode_calcs.py:371-378:
+ * lambda = wl/period, nval = num_BMs, ordre_ls = max_order_PWs,
+ * n_eff[2*nb_typ_el], bloch_vec[2], shift[2] (complex), E_H_field (1 or 2),
+ * itermax (restart limit), neq_PW.  Outputs (caller-allocated):
+ * beta1[2*nval], overlap_J[2*(2*neq_PW*nval)] (2neq_PW x nval, col-major),
+ * overlap_J_dagger[2*(nval*2*neq_PW)] (nval x 2neq_PW, col-major). */
+int emu_calc_modes_2d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval,
+                      int64_t ordre_ls, int64_t nb_typ_el, const double *n_eff,
+                      const double *bloch_vec, const double *shift, int64_t E_H_field,
+                      int64_t itermax, int64_t neq_PW, double *beta1, double *overlap_J,
+                      double *overlap_J_dagger);
+
+/* Batched form: nbatch independent (lambda, n_eff, bloch, shift) instances on
+ * one mesh, solved together (one launch sequence; prime and adjoint problems
+ * of every instance share the symbolic factorisation).  Arrays are
+ * instance-major: lambda[nbatch], n_eff[nbatch*2*nb_typ_el],
+ * bloch_vec[nbatch*2], shift[nbatch*2]; outputs beta1[nbatch*2*nval],
+ * overlap_J[nbatch*2*2neq_PW*nval], overlap_J_dagger[nbatch*2*nval*2neq_PW].
+ * opts[0] = eigen tolerance (<= 0: default 1e-12); opts[1] = precision
+ * (0 = complex128 everywhere).  stats[0..7] (may be NULL): OP applications,
+ * restarts (max over instances), pivot perturbations, Arnoldi steps,
+ * 0, 0, 0, 0. */
+int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch,
+                            const double *lambda, int64_t nval, int64_t ordre_ls,
+                            int64_t nb_typ_el, const double *n_eff, const double *bloch_vec,
+                            const double *shift, int64_t E_H_field, int64_t itermax,
+                            int64_t neq_PW, const double *opts, double *beta1,
+                            double *overlap_J, double *overlap_J_dagger, int64_t *stats);
+
+/* Stage entry points (parity tests and benchmarks). */
+
+/* Assembly of mat1 = K - shift*M and mat2 = M (asmbly.f:3-341) for nbatch
+ * instances: pp, qq [nbatch*2*nb_typ_el], shift [nbatch*2], bloch [nbatch*2].
+ * Outputs mat1, mat2 [nbatch*2*nnz] in the CSR order of emu_mesh_arrays
+ * (entry (row, col_ind[k]) at position k; the reference's CSC with
+ * row = ind_ip, col = ind_jp). */
+int emu_assemble(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, int64_t nb_typ_el,
+                 const double *pp, const double *qq, const double *shift,
+                 const double *bloch, double *mat1, double *mat2);
+
+/* Factor mat1 (host, nbatch*2*nnz, CSR order) with the GPU multifrontal LU and
+ * solve mat1 x = b for nrhs right-hand sides per instance
+ * (b, x: nbatch*nrhs*2*neq, column-major per instance).  stats[0] = pivot
+ * perturbations. */
+int emu_lu_solve(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *mat1,
+                 int64_t nrhs, const double *b, double *x, int64_t *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
