@@ -1,0 +1,226 @@
+#!/usr/bin/env python
+"""Throughput of the per-wavelength FEM + scattering-matrix path on one MI355X.
+
+Workload (stacked gratings, 8k-triangle meshes): square period 500 nm; three
+structured layers with circular inclusions r = 100, 150, 80 nm (heights
+100 / 150 / 120 nm, inclusion index drawn from U[2,4] + i U[0,0.05] with
+seed 3, air host) separated by two 50 nm films of n = 1.46, on a substrate of
+n = 1.5 under air.  Each layer mesh has about 8,000 P2 triangles (project
+mesher, seeded); num_BMs = 200, max_order_PWs = 4 (49 plane waves per
+polarisation, S-matrix dimension 98); 512 wavelengths in 400-1000 nm at
+normal incidence.  Data are synthetic.
+
+A step solves the modes of all three layers for a batch of wavelength
+points (prime and adjoint problems, one batched library call per layer)
+and runs the star-product cascade with fluxes for every point.  `value` is
+wavelength-points per second.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PERIOD = 500.0
+RADII = (100.0, 150.0, 80.0)
+HEIGHTS = (100.0, 150.0, 120.0)
+NTRI = 8000
+NUM_BMS = 200
+ORDER = 4
+NWL = 512
+HBM_PEAK_GBS = 8000.0
+
+
+def inclusion_indices():
+    rng = np.random.default_rng(3)
+    return [complex(rng.uniform(2.0, 4.0), rng.uniform(0.0, 0.05)) for _ in RADII]
+
+
+def build_meshes(workdir):
+    from emustack_amd import mesher
+    paths = []
+    for i, r in enumerate(RADII):
+        p = os.path.join(workdir, 'stacked_r%d_s%d.mail' % (int(r), i + 1))
+        if not os.path.exists(p):
+            mesher.write_mail(p, mesher.ellipse_cell_ntri(PERIOD, PERIOD, r, r, NTRI, seed=i + 1))
+        paths.append(p)
+    return paths
+
+
+def layers_for(mesh_paths):
+    from emustack_amd import objects, materials
+    nincl = inclusion_indices()
+    ns = [objects.NanoStruct('2D_array', PERIOD, 2 * r, height_nm=h, background=materials.Air,
+                             inclusion_a=materials.Material(n), loss=True, make_mesh_now=False, mesh_file=p)
+          for r, h, n, p in zip(RADII, HEIGHTS, nincl, mesh_paths)]
+    film = objects.ThinFilm(PERIOD, height_nm=50.0, material=materials.Material(1.46 + 0j))
+    sub = objects.ThinFilm(PERIOD, height_nm='semi_inf', material=materials.Material(1.5 + 0j), loss=False)
+    sup = objects.ThinFilm(PERIOD, height_nm='semi_inf', material=materials.Air, loss=False)
+    return ns, film, sub, sup
+
+
+def run_step(ns, film, sub, sup, wls):
+    """All three layers for the wavelength batch `wls`, then the cascade."""
+    from emustack_amd import objects, Stack
+    from emustack_amd.batch import calc_modes_many
+    lights = [objects.Light(wl, max_order_PWs=ORDER, theta=0.0, phi=0.0) for wl in wls]
+    sims, ops = [], 0
+    for s in ns:
+        sims.append(calc_modes_many(s, lights, num_BMs=NUM_BMS))
+        ops += int(calc_modes_many.last_stats[0])
+    out = []
+    for i, light in enumerate(lights):
+        st = Stack((sub.calc_modes(light), sims[0][i], film.calc_modes(light), sims[1][i],
+                    film.calc_modes(light), sims[2][i], sup.calc_modes(light)))
+        st.calc_scat(pol='TE')
+        out.append((float(st.a_list[-1]), float(st.r_list[-1]), float(st.t_list[-1])))
+    return out, ops
+
+
+CPU_SAMPLE = r'''
+import sys, time, json
+import numpy as np
+sys.path.insert(0, ROOT)
+from oracle import fem2d
+from emustack_amd import _lib
+import scipy.sparse.linalg as spla
+mesh_path, wl, n_incl, nops = sys.argv[1], float(sys.argv[2]), complex(sys.argv[3]), int(sys.argv[4])
+t0 = time.perf_counter()
+m = fem2d.build_mesh(mesh_path)
+t_mesh = time.perf_counter() - t0
+perm = _lib.MeshHandle(mesh_path).arrays()['perm']
+wln = wl / PERIOD; k0 = 2 * np.pi / wln
+n_eff = np.array([1.0, n_incl]); th = np.deg2rad(1e-5)
+kpn = 2 * np.pi / wl * np.sin(th) * np.array([np.cos(th), np.sin(th)]) * PERIOD
+shift = 1.1 * np.real(n_eff).max() ** 2 * k0 ** 2 - kpn @ kpn
+t0 = time.perf_counter()
+A, B = fem2d.assemble(m, np.ones(2, complex), n_eff ** 2 * k0 ** 2, shift, kpn)
+t_asm = time.perf_counter() - t0
+Ap = A.tocsr()[perm][:, perm].tocsc(); Bp = B.tocsr()[perm][:, perm].tocsr()
+t0 = time.perf_counter()
+lu = spla.splu(Ap, permc_spec='NATURAL', diag_pivot_thresh=0.0)
+t_lu = time.perf_counter() - t0
+v = np.random.default_rng(0).standard_normal(m.neq) + 0j
+t0 = time.perf_counter()
+for i in range(nops):
+    v = lu.solve(Bp @ v); v /= np.linalg.norm(v)
+t_op = (time.perf_counter() - t0) / nops
+mcv = 2 * NBM + NBM // 2 + 3; j = (NBM + mcv) // 2
+V = np.random.default_rng(1).standard_normal((j, m.neq)) + 0j
+w = v.copy()
+t0 = time.perf_counter()
+for i in range(4):
+    for p in range(2):
+        h = V.conj() @ w; w = w - V.T @ h
+t_orth = (time.perf_counter() - t0) / 4
+print(json.dumps(dict(t_mesh=t_mesh, t_asm=t_asm, t_lu=t_lu, t_op=t_op, t_orth=t_orth, neq=m.neq)))
+'''
+
+
+def cpu_baseline(mesh_path, wl, n_incl, ops_per_instance, nops=60):
+    """The oracle (numpy assembly + scipy SuperLU on the nested-dissection
+    order + ARPACK-style OP applications and CGS2 steps) timed on one host
+    core for a bounded sample of one eigensolve; extrapolated to a
+    wavelength point (6 eigensolves) with the GPU's measured OP count."""
+    env = dict(os.environ, OMP_NUM_THREADS='1', OPENBLAS_NUM_THREADS='1', MKL_NUM_THREADS='1')
+    code = 'ROOT=%r; PERIOD=%r; NBM=%r\n' % (ROOT, PERIOD, NUM_BMS) + CPU_SAMPLE
+    r = subprocess.run([sys.executable, '-c', code, mesh_path, str(wl), str(n_incl), str(nops)],
+                       capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr[-2000:])
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    t_eig = d['t_asm'] + d['t_lu'] + ops_per_instance * (d['t_op'] + d['t_orth'])
+    value = 1.0 / (6.0 * t_eig)
+    sample = ('one eigensolve of layer 1 at %.0f nm on 1 core: assembly %.1f s + SuperLU (ND order) %.1f s + '
+              '%d OP applications x (%.1f ms solve + %.1f ms CGS2) timed over %d OPs; x6 eigensolves per point'
+              % (wl, d['t_asm'], d['t_lu'], ops_per_instance, 1e3 * d['t_op'], 1e3 * d['t_orth'], nops))
+    return dict(value=value, unit='wavelength-points/s', cores=1, kind='port', sample=sample)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=2)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--batch', type=int, default=2, help='wavelength points per step')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline sample')
+    ap.add_argument('--workdir', default=os.path.join(ROOT, 'gpurun_out', 'bench_meshes'))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py: this build runs on one GPU; the workload is independent wavelength points and '
+              'would shard by wavelength across GPUs (see DESIGN.md), but only --gpus 1 is measured.',
+              file=sys.stderr)
+        sys.exit(2)
+    os.makedirs(args.workdir, exist_ok=True)
+    from emustack_amd import EMUstack
+    mesh_paths = build_meshes(args.workdir)
+    ns, film, sub, sup = layers_for(mesh_paths)
+    wavelengths = np.linspace(400.0, 1000.0, NWL)
+    ctx = EMUstack.context()
+    for p in mesh_paths:
+        EMUstack.mesh(p)
+    step_i = 0
+
+    def next_batch():
+        nonlocal step_i
+        idx = [(step_i * args.batch + i) % NWL for i in range(args.batch)]
+        step_i += 1
+        return wavelengths[idx]
+    for _ in range(args.warmup):
+        wl = next_batch()
+        print('warmup batch %s' % np.round(wl, 1), file=sys.stderr, flush=True)
+        run_step(ns, film, sub, sup, wl)
+    ctx.profile(True, every=4)
+    ops = 0
+    results = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        wl = next_batch()
+        res, o = run_step(ns, film, sub, sup, wl)
+        ops += o
+        results += res
+        print('step %d done (%.1f s)' % (k, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    T = time.perf_counter() - t0
+    prof = ctx.profile(False)
+    npts = args.steps * args.batch
+    value = npts / T
+    dom = max(prof, key=lambda k: prof[k][1])
+    cnt, ms, byt = prof[dom]
+    achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    kernel_name = {'solve_fwd': 'k_fwd', 'solve_bwd': 'k_bwd', 'gs_project': 'k_atb_partial',
+                   'gs_update': 'k_sub_vh', 'spmv_M': 'k_spmm'}[dom]
+    line = {
+        'metric': 'wavelength-points/s at 8k-tri mesh; mixed vs FP64 GPU at <=1e-6 R/T/A error',
+        'value': value, 'unit': 'wavelength-points/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': 1e3 * T / args.steps, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'c128', 'data': 'synthetic',
+        'config': {'workload': 'stacked gratings: 3 structured layers (r=100/150/80 nm, ~%d-triangle P2 meshes) '
+                               '+ 2 films, period 500 nm, num_BMs=%d, max_order_PWs=%d (98-dim S-matrix), '
+                               '%d wavelengths 400-1000 nm, normal incidence' % (NTRI, NUM_BMS, ORDER, NWL),
+                   'batch_wavelengths': args.batch, 'precision': 'fp64 (complex128) everywhere',
+                   'op_applications_per_instance': ops / (npts * 6.0)},
+        'roofline': {'bound': 'hbm', 'kernel': kernel_name, 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'launches_timed': int(cnt), 'avg_launch_us': 1e3 * ms / max(cnt, 1)},
+        'kernel_profile': {k: {'launches': int(v[0]), 'ms': v[1], 'GB': v[2] / 1e9} for k, v in prof.items()},
+        'rta_sample': results[:2],
+    }
+    if not args.no_cpu:
+        print('cpu baseline sample...', file=sys.stderr, flush=True)
+        line['cpu_baseline'] = cpu_baseline(mesh_paths[0], float(wavelengths[0]), inclusion_indices()[0],
+                                            max(1, int(round(ops / (npts * 6.0)))))
+    else:
+        line['cpu_baseline'] = None
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

@@ -37,6 +37,7 @@ EXPORTS = [
     'emu_last_error', 'emu_ctx_create', 'emu_ctx_destroy', 'emu_set_lapack', 'emu_mesh_load',
     'emu_mesh_analyze', 'emu_mesh_symbolic', 'emu_mesh_free', 'emu_mesh_info', 'emu_mesh_arrays',
     'emu_mesh_geometry', 'emu_calc_modes_2d', 'emu_calc_modes_2d_batch', 'emu_assemble', 'emu_lu_solve',
+    'emu_profile',
 ]
 
 
@@ -94,6 +95,7 @@ def lib():
                                               c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_i64p]
         L.emu_assemble.argtypes = [c_vp, c_vp, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp]
         L.emu_lu_solve.argtypes = [c_vp, c_vp, c_i64, c_dp, c_i64, c_dp, c_dp, c_i64p]
+        L.emu_profile.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_dp]
         for f in EXPORTS:
             getattr(L, f)
         rc = L.emu_set_lapack(*_lapack_pointers())
@@ -175,6 +177,16 @@ class Context:
                                             ptr(beta.view(np.float64)), ptr(J.view(np.float64)),
                                             ptr(Jd.view(np.float64)), ptr(st, c_i64p)))
         return beta, J.transpose(0, 2, 1), Jd.transpose(0, 2, 1), st
+
+    PROFILE_CLASSES = ('solve_fwd', 'solve_bwd', 'gs_project', 'gs_update', 'spmv_M')
+
+    def profile(self, enable=None, every=8):
+        """enable=True: reset and start timing; False: stop; None: read.
+        Returns {class: (launches, ms, algorithmic bytes)}."""
+        out = np.zeros(24)
+        flag = -1 if enable is None else (1 if enable else 0)
+        check(lib().emu_profile(self.h, flag, int(every), ptr(out)))
+        return {name: tuple(out[3 * k:3 * k + 3]) for k, name in enumerate(self.PROFILE_CLASSES)}
 
     def close(self):
         if self.h:

@@ -73,6 +73,11 @@ void emu_ctx_destroy(emu_ctx *ctx) {
   delete ctx;
 }
 
+int emu_profile(emu_ctx *ctx, int enable, int every, double *out) {
+  if (!ctx) return fail(EMU_E_ARG, "emu_profile: NULL context");
+  EMU_GUARD(emu::profile(ctx->c, enable, every, out))
+}
+
 int emu_set_lapack(void *zgees, void *ztrsen, void *ztrevc) {
   if (!zgees || !ztrsen || !ztrevc) return fail(EMU_E_ARG, "emu_set_lapack: NULL routine");
   emu::set_lapack(zgees, ztrsen, ztrevc);

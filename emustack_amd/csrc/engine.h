@@ -44,6 +44,61 @@ struct DBuf {
   void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
 };
 
+// Live kernel timing with HIP events (enabled by emu_profile).  A "sweep" is
+// one solve or one Gram-Schmidt pass; every `every`-th sweep is recorded.
+struct Profiler {
+  bool on = false;
+  int every = 8;
+  int64_t sweep = 0;
+  bool active = false;
+  struct Pend {
+    int cls;
+    double bytes;
+    int a, b;
+  };
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  std::vector<Pend> pend;
+  double stat[8][3] = {};
+  void start_sweep() { active = on && (sweep++ % every == 0); }
+  int mark(hipStream_t s) {
+    if (used == ev.size()) {
+      hipEvent_t e;
+      EMU_HIP(hipEventCreate(&e));
+      ev.push_back(e);
+    }
+    EMU_HIP(hipEventRecord(ev[used], s));
+    return (int)used++;
+  }
+  // returns the begin mark (or -1 when not recording)
+  int begin(hipStream_t s) { return active ? mark(s) : -1; }
+  void end(hipStream_t s, int a, int cls, double bytes) {
+    if (a < 0) return;
+    pend.push_back({cls, bytes, a, mark(s)});
+  }
+  // call after the stream has been synchronised
+  void collect() {
+    for (auto &p : pend) {
+      float ms = 0;
+      EMU_HIP(hipEventElapsedTime(&ms, ev[p.a], ev[p.b]));
+      stat[p.cls][0] += 1;
+      stat[p.cls][1] += ms;
+      stat[p.cls][2] += p.bytes;
+    }
+    pend.clear();
+    used = 0;
+  }
+  void reset() {
+    for (auto &r : stat) r[0] = r[1] = r[2] = 0;
+    pend.clear();
+    used = 0;
+    sweep = 0;
+  }
+  ~Profiler() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
+};
+
 // Kernel task lists of one factorisation / solve level.
 struct StepPlan {
   int32_t k0;
@@ -56,6 +111,7 @@ struct LevelSched {
   int64_t fronts_off, fronts_cnt;     // fronts of the level
   int64_t ea_off[2], ea_cnt[2];       // extend-add tasks (child front, col0) by slot
   int32_t max_f;
+  double fwd_bytes = 0, bwd_bytes = 0;  // algorithmic bytes per instance and RHS
 };
 
 struct DevMesh {
@@ -108,7 +164,7 @@ void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int3
                hipStream_t s);
 // x = A^{-1} b, both in the new (nested-dissection) numbering, [nb][nrhs][n]
 void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *pool, const int32_t *piv, const zd *b,
-              zd *y, zd *x, zd *w, hipStream_t s);
+              zd *y, zd *x, zd *w, hipStream_t s, Profiler *prof = nullptr);
 // permute vectors old <-> new numbering: dst[new] = src[perm[new]] or inverse
 void launch_permute_vec(DevMesh &dm, int nb, int ncol, const zd *src, zd *dst, bool to_new,
                         hipStream_t s);

@@ -186,6 +186,14 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
       }
       L.ea_cnt[sl] = ((int64_t)tasks.size() - L.ea_off[sl]) / 2;
     }
+    // algorithmic bytes of one solve sweep of this level (per instance, per RHS)
+    for (int32_t t : LP.fronts) {
+      const FrontInfo &F = sy.fronts[t];
+      double ss = F.s, uu = F.u, wc = 0;
+      for (int c = 0; c < F.nchild; ++c) wc += sy.fronts[F.child[c]].u;
+      L.fwd_bytes += 16.0 * (ss * (ss - 1) / 2 + uu * ss + 2 * ss + uu + wc) + 4.0 * ss;
+      L.bwd_bytes += 16.0 * (ss * (ss + 1) / 2 + ss * uu + 2 * ss + uu) + 4.0 * uu;
+    }
     dm.sched.push_back(L);
   }
   for (int c = 0; c < 2; ++c) {
@@ -204,6 +212,7 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  Profiler prof;
   std::vector<void *> bufs;
   std::vector<size_t> sizes;
   std::vector<std::string> names;
@@ -245,6 +254,21 @@ Ctx *ctx_create(int device) {
   return c;
 }
 void ctx_destroy(Ctx *c) { delete c; }
+
+void profile(Ctx *c, int enable, int every, double *out) {
+  if (enable == 1) {
+    c->prof.reset();
+    c->prof.on = true;
+    c->prof.every = std::max(1, every);
+  } else if (enable == 0) {
+    c->prof.on = false;
+  }
+  EMU_HIP(hipStreamSynchronize(c->stream));
+  c->prof.collect();
+  if (out)
+    for (int k = 0; k < 8; ++k)
+      for (int q = 0; q < 3; ++q) out[3 * k + q] = c->prof.stat[k][q];
+}
 
 DevMesh *mesh_load(Ctx *c, const std::string &path, int leaf) {
   EMU_HIP(hipSetDevice(c->device));
@@ -470,13 +494,22 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int i
   for (int it = 0; it <= itermax; ++it) {
     for (int j = k; j < m; ++j) {
       zd *vj = V + (int64_t)j * n;
+      Profiler *pf = c->prof.on ? &c->prof : nullptr;
+      const double nbd = nb, nd = (double)n, nzd = (double)dm.host.nnz;
+      int e0 = pf ? (pf->start_sweep(), pf->begin(s)) : -1;
       launch_spmm(nb, n, 1, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, vj, sV, t1, n, s);
-      lu_solve(dm, nb, 1, pool, piv, t1, ty, wv, tw, s);
+      if (pf) pf->end(s, e0, 4, nbd * (20.0 * nzd + 8.0 * (nd + 1) + 32.0 * nd));
+      lu_solve(dm, nb, 1, pool, piv, t1, ty, wv, tw, s, pf);
       // CGS with one re-orthogonalisation
-      launch_atb(nb, n, j + 1, 1, V, n, sV, wv, n, n, h1, m + 1, m + 1, true, atbs, atb_elems, s);
-      launch_sub_vh(nb, n, j + 1, V, n, sV, h1, m + 1, wv, n, s);
-      launch_atb(nb, n, j + 1, 1, V, n, sV, wv, n, n, h2, m + 1, m + 1, true, atbs, atb_elems, s);
-      launch_sub_vh(nb, n, j + 1, V, n, sV, h2, m + 1, wv, n, s);
+      for (int pass = 0; pass < 2; ++pass) {
+        zd *hh = pass == 0 ? h1 : h2;
+        int e1 = pf ? pf->begin(s) : -1;
+        launch_atb(nb, n, j + 1, 1, V, n, sV, wv, n, n, hh, m + 1, m + 1, true, atbs, atb_elems, s);
+        if (pf) pf->end(s, e1, 2, nbd * 16.0 * nd * (j + 2));
+        int e2 = pf ? pf->begin(s) : -1;
+        launch_sub_vh(nb, n, j + 1, V, n, sV, hh, m + 1, wv, n, s);
+        if (pf) pf->end(s, e2, 3, nbd * 16.0 * nd * (j + 3));
+      }
       launch_axpy_h(nb, j + 1, h1, h2, m + 1, Hd + (int64_t)j * (m + 1), sH, s);
       launch_norm_scale(nb, n, wv, n, V + (int64_t)(j + 1) * n, sV, nrm, nsc, s);
       // H(j+1, j) = ||w||
@@ -487,6 +520,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int i
     }
     EMU_HIP(hipMemcpyAsync(Hh.data(), Hd, sizeof(zd) * Hh.size(), hipMemcpyDeviceToHost, s));
     EMU_HIP(hipStreamSynchronize(s));
+    if (c->prof.on) c->prof.collect();
     bool last = (it == itermax);
     all_conv = true;
     std::vector<std::thread> th;

@@ -23,6 +23,7 @@ void set_lapack(void *zgees, void *ztrsen, void *ztrevc);
 bool lapack_ready();
 Ctx *ctx_create(int device);
 void ctx_destroy(Ctx *c);
+void profile(Ctx *c, int enable, int every, double *out);
 DevMesh *mesh_load(Ctx *c, const std::string &path, int leaf);
 void mesh_free(DevMesh *m);
 DevMesh *mesh_analyze(const std::string &path, int leaf);

@@ -354,20 +354,28 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ fronts,
 }
 
 void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *pool, const int32_t *piv, const zd *bnew,
-              zd *y, zd *x, zd *w, hipStream_t s) {
+              zd *y, zd *x, zd *w, hipStream_t s, Profiler *prof) {
   const Symbolic &sy = dm.host.sym;
   FrontArrays A = fa(dm);
   size_t lds = (size_t)dm.max_f * sizeof(zd);
   const int32_t *c0 = dm.tasks.p + dm.child_off[0];
   const int32_t *c1 = dm.tasks.p + dm.child_off[1];
-  for (const LevelSched &L : dm.sched)
+  if (prof) prof->start_sweep();
+  const double scale = (double)nb * nrhs;
+  for (const LevelSched &L : dm.sched) {
+    int ev = prof ? prof->begin(s) : -1;
     hipLaunchKernelGGL(k_fwd, dim3((unsigned)L.fronts_cnt, nb), dim3(256), lds, s,
                        dm.tasks.p + L.fronts_off, A, c0, c1, pool, sy.pool, piv, sy.npiv, nrhs,
                        sy.n, sy.nw, bnew, y, w);
+    if (prof) prof->end(s, ev, 0, L.fwd_bytes * scale);
+  }
   EMU_LAUNCH_CHECK();
-  for (auto it = dm.sched.rbegin(); it != dm.sched.rend(); ++it)
+  for (auto it = dm.sched.rbegin(); it != dm.sched.rend(); ++it) {
+    int ev = prof ? prof->begin(s) : -1;
     hipLaunchKernelGGL(k_bwd, dim3((unsigned)it->fronts_cnt, nb), dim3(256), lds, s,
                        dm.tasks.p + it->fronts_off, A, pool, sy.pool, nrhs, sy.n, y, x);
+    if (prof) prof->end(s, ev, 1, it->bwd_bytes * scale);
+  }
   EMU_LAUNCH_CHECK();
 }
 

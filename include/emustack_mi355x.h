@@ -117,6 +117,16 @@ int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch,
                             int64_t neq_PW, const double *opts, double *beta1,
                             double *overlap_J, double *overlap_J_dagger, int64_t *stats);
 
+/* Live kernel timing with HIP events on the context's stream (benchmarks).
+ * enable = 1 resets the counters and starts recording every `every`-th
+ * solve sweep; enable = 0 stops.  out[k*3 + {0,1,2}] (may be NULL) for kernel
+ * class k: launches timed, total milliseconds, total algorithmic bytes of
+ * those launches.  Classes: 0 forward-solve level sweep (k_fwd), 1 backward
+ * level sweep (k_bwd), 2 Gram-Schmidt projections (k_atb_partial),
+ * 3 Gram-Schmidt updates (k_sub_vh), 4 mass-matrix SpMV (k_spmm),
+ * 5..7 reserved. */
+int emu_profile(emu_ctx *ctx, int enable, int every, double *out);
+
 /* Stage entry points (parity tests and benchmarks). */
 
 /* Assembly of mat1 = K - shift*M and mat2 = M (asmbly.f:3-341) for nbatch
