@@ -158,7 +158,7 @@ class Context:
         return x, int(st[0])
 
     def calc_modes_batch(self, mesh, lam, nval, ordre, n_eff, bloch, shift, neq_pw, E_H_field=1, itermax=30,
-                         tol=0.0):
+                         tol=0.0, block=0):
         """Batched calc_modes_2d: returns beta [nl, nval], J [nl, 2P, nval], Jd [nl, nval, 2P], stats."""
         lam = np.ascontiguousarray(np.atleast_1d(np.asarray(lam, np.float64)))
         nl = lam.shape[0]
@@ -169,7 +169,7 @@ class Context:
         beta = np.zeros((nl, nval), np.complex128)
         J = np.zeros((nl, nval, 2 * neq_pw), np.complex128)    # C-order view of F-order (2P x nval)
         Jd = np.zeros((nl, 2 * neq_pw, nval), np.complex128)   # C-order view of F-order (nval x 2P)
-        opts = np.array([tol, 0.0])
+        opts = np.array([tol, 0.0, float(block), 0.0])
         st = np.zeros(8, np.int64)
         check(lib().emu_calc_modes_2d_batch(self.h, mesh.h, nl, ptr(lam), nval, ordre, ntyp,
                                             ptr(n_eff.view(np.float64)), ptr(bloch),

@@ -167,6 +167,9 @@ int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const 
     a.bloch = bloch_vec;
     a.shift = shift;
     if (opts && opts[0] > 0) a.tol = opts[0];
+    if (opts && opts[2] > 0) a.block = (int64_t)opts[2];
+    if (a.block != 1 && a.block != 2 && a.block != 4 && a.block != 8)
+      return fail(EMU_E_ARG, "Krylov block size must be 1, 2, 4 or 8");
     a.beta = beta1;
     a.J = overlap_J;
     a.Jd = overlap_J_dagger;

@@ -191,6 +191,17 @@ void launch_norm_scale(int nb, int64_t n, const zd *w, int64_t sw, zd *out, int6
 void launch_axpy_h(int nb, int k, const zd *h1, const zd *h2, int64_t sh, zd *H, int64_t sH,
                    hipStream_t s);
 void launch_copy(int nb, int64_t n, const zd *src, int64_t ss, zd *dst, int64_t sd, hipStream_t s);
+// block Gram-Schmidt: W[:, :bq] -= V[:, :k] H[:k, :bq]
+void launch_sub_vh_blk(int nb, int64_t n, int k, int bq, const zd *V, int64_t ldv, int64_t sV, const zd *H,
+                       int64_t ldh, int64_t sH, zd *W, int64_t ldw, int64_t sW, hipStream_t s);
+// Cholesky-QR of a bq-column block: G = R^H R, Rinv = R^-1, Racc = R * Rprev
+void launch_cholqr(int nb, int bq, const zd *G, zd *Rinv, const zd *Rprev, zd *Racc, int32_t *bad,
+                   hipStream_t s);
+// O = W * Rinv (bq x bq upper triangular), in place allowed
+void launch_trmm_right(int nb, int64_t n, int bq, const zd *W, int64_t ldw, int64_t sW, const zd *Ri, zd *O,
+                       int64_t ldo, int64_t sO, hipStream_t s);
+void launch_block_copy(int nb, int rows, int cols, const zd *S, int64_t lds, int64_t sS, zd *D, int64_t ldd,
+                       int64_t sD, bool add, hipStream_t s);
 void launch_fill(zd *p, int64_t n, zd v, hipStream_t s);
 
 // ---------------- post-processing (kernels_post.hip) ----------------

@@ -351,21 +351,27 @@ void lu_solve_host(Ctx *c, DevMesh &dm, int nb, const double *mat1, int nrhs, co
 }
 
 // ------------------------------------------------------ Krylov-Schur solve
-// Host-side state of the projected problem of one instance.
+// Block Krylov-Schur (block size bq) for the nev largest-|mu| eigenpairs of
+// OP = (K - shift M)^-1 M, the operator ARPACK iterates on in the reference
+// (valpr_umf_64.f:291-459, which = 'LM').  One block step applies OP to bq
+// vectors (one multi-RHS solve), orthogonalises them against the basis with
+// two block classical Gram-Schmidt passes and Cholesky-QR2 inside the block.
+// The projected (m x m) Schur problem is solved on the host with LAPACK.
+//
+// Relation after the expansion:  OP V[:, :m] = V[:, :m+bq] H[:m+bq, :m].
 struct KSState {
-  std::vector<cplx> H;   // (m+1) x m, column-major, ld = m+1
+  std::vector<cplx> H;     // (m+bq) x m, column-major, ld = m+bq
   std::vector<cplx> T, Q;  // m x m Schur form and vectors
   std::vector<cplx> theta;
-  int keep = 0;
-  bool conv = false;
-  int restarts = 0;
 };
 
 // Schur-decompose H[:m,:m], order the `keep` largest |theta| first and the
-// `nev` largest among them first of all, test residuals of the leading nev
-// Ritz pairs.  Returns true if converged.
-static bool ks_project(KSState &st, int m, int nev, int keep, double tol, std::vector<cplx> *ritzvec) {
-  int ld = m + 1;
+// `nev` largest among them first of all, test the residuals
+// ||B Q y_i|| <= tol |theta_i| of the leading nev Ritz pairs (B = last bq rows
+// of H).  Optionally return W = Q[:, :nev] Y (Ritz vectors in the basis).
+static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol, std::vector<cplx> *ritzvec,
+                       double *maxres) {
+  const int ld = m + bq;
   st.T.assign((size_t)m * m, 0);
   for (int j = 0; j < m; ++j)
     for (int i = 0; i < m; ++i) st.T[i + (size_t)j * m] = st.H[i + (size_t)j * ld];
@@ -402,7 +408,6 @@ static bool ks_project(KSState &st, int m, int nev, int keep, double tol, std::v
   };
   reorder(keep);
   reorder(nev);
-  // residuals of the leading nev Ritz pairs
   std::vector<cplx> Tn((size_t)nev * nev);
   for (int j = 0; j < nev; ++j)
     for (int i = 0; i < nev; ++i) Tn[i + (size_t)j * nev] = st.T[i + (size_t)j * m];
@@ -413,26 +418,30 @@ static bool ks_project(KSState &st, int m, int nev, int keep, double tol, std::v
   g_ztrevc(&side, &howmny, nullptr, &nn, Tn.data(), &ldt, VL.data(), &ldvl, VR.data(), &ldvr, &mmv, &mout,
            wk.data(), rw.data(), &inf);
   if (inf != 0) throw std::runtime_error("ztrevc failed");
-  // b^T Q : last row of H times Q
-  std::vector<cplx> bq(nev, 0);
-  for (int j = 0; j < nev; ++j) {
-    cplx acc = 0;
-    for (int i = 0; i < m; ++i) acc += st.H[m + (size_t)i * ld] * st.Q[i + (size_t)j * m];
-    bq[j] = acc;
-  }
-  bool conv = true;
-  for (int k = 0; k < nev; ++k) {
-    cplx r = 0;
-    double ny = 0;
-    for (int j = 0; j <= k; ++j) {
-      r += bq[j] * VR[j + (size_t)k * nev];
-      ny += std::norm(VR[j + (size_t)k * nev]);
+  // BQ = B Q[:, :nev]  (bq x nev)
+  std::vector<cplx> BQ((size_t)bq * nev, 0);
+  for (int j = 0; j < nev; ++j)
+    for (int i = 0; i < m; ++i) {
+      cplx qv = st.Q[i + (size_t)j * m];
+      if (qv == 0.0) continue;
+      for (int r = 0; r < bq; ++r) BQ[r + (size_t)j * bq] += st.H[(m + r) + (size_t)i * ld] * qv;
     }
-    double res = std::abs(r) / std::sqrt(ny);
-    if (!(res <= tol * std::abs(st.theta[k]))) conv = false;
+  bool conv = true;
+  double worst = 0;
+  for (int k = 0; k < nev; ++k) {
+    double ny = 0, res2 = 0;
+    for (int j = 0; j <= k; ++j) ny += std::norm(VR[j + (size_t)k * nev]);
+    for (int r = 0; r < bq; ++r) {
+      cplx acc = 0;
+      for (int j = 0; j <= k; ++j) acc += BQ[r + (size_t)j * bq] * VR[j + (size_t)k * nev];
+      res2 += std::norm(acc);
+    }
+    double rel = std::sqrt(res2 / ny) / std::max(std::abs(st.theta[k]), 1e-300);
+    worst = std::max(worst, rel);
+    if (!(rel <= tol)) conv = false;
   }
+  if (maxres) *maxres = worst;
   if (ritzvec) {
-    // W = Q[:, :nev] * VR (m x nev)
     ritzvec->assign((size_t)m * nev, 0);
     for (int k = 0; k < nev; ++k)
       for (int j = 0; j <= k; ++j) {
@@ -445,42 +454,57 @@ static bool ks_project(KSState &st, int m, int nev, int keep, double tol, std::v
 
 struct EigResult {
   std::vector<cplx> mu;  // nb x nev
-  int64_t ops = 0, restarts = 0, steps = 0;
+  int64_t ops = 0, restarts = 0, steps = 0, rank_fixes = 0;
 };
 
-// nb instances in lockstep.  Operator: x -> A^{-1} M x in the ND numbering.
-// Output: Ritz vectors X (nb x n x nev, new numbering) in `Xout`.
-static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int itermax, double tol,
+// orthonormalise V[:, j:j+bq] in place (Cholesky-QR2); R (bq x bq) of the
+// factorisation W = Q R is written to Rout[b] (device, bq x bq) when given
+static void block_cholqr2(Ctx *c, int nb, int64_t n, int bq, zd *W, int64_t sV, zd *G, zd *Ri, zd *R1, zd *Rout,
+                          int32_t *bad, zd *atbs, size_t atb_elems, hipStream_t s) {
+  const int64_t sG = (int64_t)bq * bq;
+  launch_atb(nb, n, bq, bq, W, n, sV, W, n, sV, G, bq, sG, true, atbs, atb_elems, s);
+  launch_cholqr(nb, bq, G, Ri, nullptr, R1, bad, s);
+  launch_trmm_right(nb, n, bq, W, n, sV, Ri, W, n, sV, s);
+  launch_atb(nb, n, bq, bq, W, n, sV, W, n, sV, G, bq, sG, true, atbs, atb_elems, s);
+  launch_cholqr(nb, bq, G, Ri, R1, Rout, bad, s);
+  launch_trmm_right(nb, n, bq, W, n, sV, Ri, W, n, sV, s);
+}
+
+static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int bq, int itermax, double tol,
                               const zd *pool, const int32_t *piv, const zd *mvals, zd *Xout) {
   hipStream_t s = c->stream;
   const Symbolic &sy = dm.host.sym;
   const int64_t n = dm.host.neq;
-  const int64_t sV = (int64_t)(m + 1) * n;
+  const int ld = m + bq;                     // rows of H
+  const int64_t sV = (int64_t)(m + bq) * n;  // basis stride per instance
+  const int64_t sH = (int64_t)ld * m;
   zd *V = c->buf<zd>("ks_V", (size_t)nb * sV);
   zd *V2 = c->buf<zd>("ks_V2", (size_t)nb * sV);
-  zd *t1 = c->buf<zd>("ks_t1", (size_t)nb * n);
-  zd *ty = c->buf<zd>("ks_ty", (size_t)nb * n);
-  zd *tw = c->buf<zd>("ks_w", (size_t)nb * std::max<int64_t>(sy.nw, 1));
-  zd *wv = c->buf<zd>("ks_wv", (size_t)nb * n);
-  zd *h1 = c->buf<zd>("ks_h1", (size_t)nb * (m + 1));
-  zd *h2 = c->buf<zd>("ks_h2", (size_t)nb * (m + 1));
-  const int64_t sH = (int64_t)(m + 1) * m;
+  zd *t1 = c->buf<zd>("ks_t1", (size_t)nb * bq * n);
+  zd *ty = c->buf<zd>("ks_ty", (size_t)nb * bq * n);
+  zd *wv = c->buf<zd>("ks_wv", (size_t)nb * bq * n);
+  zd *tw = c->buf<zd>("ks_w", (size_t)nb * bq * std::max<int64_t>(sy.nw, 1));
   zd *Hd = c->buf<zd>("ks_H", (size_t)nb * sH);
-  double *nrm = c->buf<double>("ks_nrm", nb);
-  double *nsc = c->buf<double>("ks_nsc", (size_t)nb * 64);
-  size_t atb_elems = (size_t)nb * (m + 1) * 256;
+  zd *H2 = c->buf<zd>("ks_H2", (size_t)nb * ld * bq);
+  zd *G = c->buf<zd>("ks_G", (size_t)nb * bq * bq);
+  zd *Ri = c->buf<zd>("ks_Ri", (size_t)nb * bq * bq);
+  zd *R1 = c->buf<zd>("ks_R1", (size_t)nb * bq * bq);
+  zd *Rf = c->buf<zd>("ks_Rf", (size_t)nb * bq * bq);
+  int32_t *bad = c->buf<int32_t>("ks_bad", nb);
+  size_t atb_elems = (size_t)nb * (m + bq) * std::max(bq, 32) * 128;
   zd *atbs = c->buf<zd>("ks_atb", atb_elems);
   zd *Qd = c->buf<zd>("ks_Q", (size_t)nb * m * m);
-  // start vector: deterministic pseudo-random
+  EMU_HIP(hipMemsetAsync(bad, 0, sizeof(int32_t) * nb, s));
+  // start block: deterministic pseudo-random, orthonormalised
   {
-    std::vector<zd> v0((size_t)nb * n);
+    std::vector<zd> v0((size_t)nb * bq * n);
     std::mt19937_64 rng(12345);
     std::uniform_real_distribution<double> U(-1.0, 1.0);
-    for (int b = 0; b < nb; ++b)
-      for (int64_t i = 0; i < n; ++i) v0[(size_t)b * n + i] = zmk(U(rng), U(rng));
+    for (auto &z : v0) z = zmk(U(rng), U(rng));
     EMU_HIP(hipMemcpyAsync(t1, v0.data(), sizeof(zd) * v0.size(), hipMemcpyHostToDevice, s));
     EMU_HIP(hipStreamSynchronize(s));
-    launch_norm_scale(nb, n, t1, n, V, sV, nrm, nsc, s);
+    launch_copy(nb, bq * n, t1, bq * n, V, sV, s);
+    block_cholqr2(c, nb, n, bq, V, sV, G, Ri, R1, Rf, bad, atbs, atb_elems, s);
   }
   launch_fill(Hd, (int64_t)nb * sH, zmk(0, 0), s);
   std::vector<KSState> st(nb);
@@ -488,34 +512,38 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int i
   EigResult res;
   res.mu.assign((size_t)nb * nev, 0);
   int k = 0;
-  const int keep = std::min(m - 1, nev + (m - nev) / 2);
+  int keep = std::min(m - bq, nev + (m - nev) / 2);
+  keep = m - ((m - keep + bq - 1) / bq) * bq;  // (m - keep) multiple of bq
+  if (keep < nev) throw std::invalid_argument("Krylov subspace too small for num_BMs");
   bool all_conv = false;
   std::vector<zd> Hh((size_t)nb * sH);
+  Profiler *pf = c->prof.on ? &c->prof : nullptr;
+  const double nbd = nb, nd = (double)n, nzd = (double)dm.host.nnz;
   for (int it = 0; it <= itermax; ++it) {
-    for (int j = k; j < m; ++j) {
-      zd *vj = V + (int64_t)j * n;
-      Profiler *pf = c->prof.on ? &c->prof : nullptr;
-      const double nbd = nb, nd = (double)n, nzd = (double)dm.host.nnz;
-      int e0 = pf ? (pf->start_sweep(), pf->begin(s)) : -1;
-      launch_spmm(nb, n, 1, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, vj, sV, t1, n, s);
-      if (pf) pf->end(s, e0, 4, nbd * (20.0 * nzd + 8.0 * (nd + 1) + 32.0 * nd));
-      lu_solve(dm, nb, 1, pool, piv, t1, ty, wv, tw, s, pf);
-      // CGS with one re-orthogonalisation
+    for (int j = k; j < m; j += bq) {
+      zd *Xj = V + (int64_t)j * n;
+      zd *Wn = V + (int64_t)(j + bq) * n;  // next block slot
+      if (pf) pf->start_sweep();
+      int e0 = pf ? pf->begin(s) : -1;
+      launch_spmm(nb, n, bq, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, t1, (int64_t)bq * n, s);
+      if (pf) pf->end(s, e0, 4, nbd * (20.0 * nzd + 8.0 * (nd + 1) + 32.0 * nd * bq));
+      lu_solve(dm, nb, bq, pool, piv, t1, ty, wv, tw, s, pf);
+      launch_copy(nb, bq * n, wv, (int64_t)bq * n, Wn, sV, s);
+      // block CGS, two passes
       for (int pass = 0; pass < 2; ++pass) {
-        zd *hh = pass == 0 ? h1 : h2;
+        zd *hc = pass == 0 ? Hd + (int64_t)j * ld : H2;
+        int64_t hld = pass == 0 ? ld : ld, hs = pass == 0 ? sH : (int64_t)ld * bq;
         int e1 = pf ? pf->begin(s) : -1;
-        launch_atb(nb, n, j + 1, 1, V, n, sV, wv, n, n, hh, m + 1, m + 1, true, atbs, atb_elems, s);
-        if (pf) pf->end(s, e1, 2, nbd * 16.0 * nd * (j + 2));
+        launch_atb(nb, n, j + bq, bq, V, n, sV, Wn, n, sV, hc, hld, hs, true, atbs, atb_elems, s);
+        if (pf) pf->end(s, e1, 2, nbd * 16.0 * nd * (j + 2 * bq));
         int e2 = pf ? pf->begin(s) : -1;
-        launch_sub_vh(nb, n, j + 1, V, n, sV, hh, m + 1, wv, n, s);
-        if (pf) pf->end(s, e2, 3, nbd * 16.0 * nd * (j + 3));
+        launch_sub_vh_blk(nb, n, j + bq, bq, V, n, sV, hc, hld, hs, Wn, n, sV, s);
+        if (pf) pf->end(s, e2, 3, nbd * 16.0 * nd * (j + 3 * bq));
       }
-      launch_axpy_h(nb, j + 1, h1, h2, m + 1, Hd + (int64_t)j * (m + 1), sH, s);
-      launch_norm_scale(nb, n, wv, n, V + (int64_t)(j + 1) * n, sV, nrm, nsc, s);
-      // H(j+1, j) = ||w||
-      hipLaunchKernelGGL(k_store_norm, dim3((nb + 63) / 64), dim3(64), 0, s, nb, nrm,
-                         Hd + (int64_t)j * (m + 1) + j + 1, sH);
-      res.ops += nb;
+      launch_block_copy(nb, j + bq, bq, H2, ld, (int64_t)ld * bq, Hd + (int64_t)j * ld, ld, sH, true, s);
+      block_cholqr2(c, nb, n, bq, Wn, sV, G, Ri, R1, Rf, bad, atbs, atb_elems, s);
+      launch_block_copy(nb, bq, bq, Rf, bq, (int64_t)bq * bq, Hd + (int64_t)j * ld + j + bq, ld, sH, false, s);
+      res.ops += (int64_t)nb * bq;
       res.steps += 1;
     }
     EMU_HIP(hipMemcpyAsync(Hh.data(), Hd, sizeof(zd) * Hh.size(), hipMemcpyDeviceToHost, s));
@@ -531,7 +559,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int i
         try {
           KSState &S = st[b];
           for (int64_t i = 0; i < sH; ++i) S.H[i] = cplx(Hh[(size_t)b * sH + i].x, Hh[(size_t)b * sH + i].y);
-          convflag[b] = ks_project(S, m, nev, keep, tol, nullptr) ? 1 : 0;
+          convflag[b] = ks_project(S, m, bq, nev, keep, tol, nullptr, nullptr) ? 1 : 0;
         } catch (std::exception &e) {
           errs[b] = e.what();
         }
@@ -540,12 +568,11 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int i
     for (auto &t : th) t.join();
     for (int b = 0; b < nb; ++b) {
       if (!errs[b].empty()) throw std::runtime_error(errs[b]);
-      st[b].conv = convflag[b];
       if (!convflag[b]) all_conv = false;
     }
     res.restarts = it;
     if (all_conv || last) break;
-    // restart: V2[:, :keep] = V[:, :m] Q[:, :keep]; V2[:, keep] = V[:, m]
+    // restart: V2[:, :keep] = V[:, :m] Q[:, :keep]; V2[:, keep:keep+bq] = V[:, m:m+bq]
     std::vector<zd> Qh((size_t)nb * m * keep);
     std::vector<zd> Hn((size_t)nb * sH, zmk(0, 0));
     for (int b = 0; b < nb; ++b) {
@@ -558,27 +585,32 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int i
       for (int j = 0; j < keep; ++j) {
         for (int i = 0; i <= j; ++i) {
           cplx t = S.T[i + (size_t)j * m];
-          Hn[(size_t)b * sH + i + (size_t)j * (m + 1)] = zmk(t.real(), t.imag());
+          Hn[(size_t)b * sH + i + (size_t)j * ld] = zmk(t.real(), t.imag());
         }
-        cplx acc = 0;
-        for (int i = 0; i < m; ++i) acc += S.H[m + (size_t)i * (m + 1)] * S.Q[i + (size_t)j * m];
-        Hn[(size_t)b * sH + keep + (size_t)j * (m + 1)] = zmk(acc.real(), acc.imag());
+        for (int r = 0; r < bq; ++r) {
+          cplx acc = 0;
+          for (int i = 0; i < m; ++i) acc += S.H[(m + r) + (size_t)i * ld] * S.Q[i + (size_t)j * m];
+          Hn[(size_t)b * sH + (keep + r) + (size_t)j * ld] = zmk(acc.real(), acc.imag());
+        }
       }
     }
     EMU_HIP(hipMemcpyAsync(Qd, Qh.data(), sizeof(zd) * Qh.size(), hipMemcpyHostToDevice, s));
     EMU_HIP(hipMemcpyAsync(Hd, Hn.data(), sizeof(zd) * Hn.size(), hipMemcpyHostToDevice, s));
     EMU_HIP(hipStreamSynchronize(s));
     launch_gemm_nn(nb, n, keep, m, zmk(1, 0), V, n, sV, Qd, m, (int64_t)m * keep, zmk(0, 0), V2, n, sV, s);
-    launch_copy(nb, n, V + (int64_t)m * n, sV, V2 + (int64_t)keep * n, sV, s);
+    launch_copy(nb, (int64_t)bq * n, V + (int64_t)m * n, sV, V2 + (int64_t)keep * n, sV, s);
     std::swap(V, V2);
     k = keep;
   }
   if (!all_conv) throw NoConvergence("Krylov-Schur: eigenpairs did not converge within itermax restarts");
+  std::vector<int32_t> badh(nb);
+  EMU_HIP(hipMemcpy(badh.data(), bad, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
+  for (int b = 0; b < nb; ++b) res.rank_fixes += badh[b];
   // Ritz vectors X = V[:, :m] W, W = Q[:, :nev] Y
   std::vector<zd> Wh((size_t)nb * m * nev);
   for (int b = 0; b < nb; ++b) {
     std::vector<cplx> W;
-    ks_project(st[b], m, nev, keep, tol, &W);
+    ks_project(st[b], m, bq, nev, keep, tol, &W, nullptr);
     for (size_t i = 0; i < W.size(); ++i) Wh[(size_t)b * m * nev + i] = zmk(W[i].real(), W[i].imag());
     for (int i = 0; i < nev; ++i) res.mu[(size_t)b * nev + i] = st[b].theta[i];
   }
@@ -688,8 +720,12 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   const int64_t n = mh.neq, nnz = mh.nnz, nel = mh.nel;
   const int nl = (int)a.nl, nb = 2 * nl, ntyp = (int)a.ntyp, nval = (int)a.nval;
   if (nl < 1 || nval < 1 || ntyp < dm.ntyp) throw std::invalid_argument("calc_modes: bad sizes");
-  const int m = (int)std::min<int64_t>(2 * nval + nval / 2 + 3, n - 1);  // py_calc_modes.f:256
-  if (nval >= m) throw std::invalid_argument("calc_modes: num_BMs too large for the mesh");
+  // Krylov dimension: ARPACK's nvect = 2 nval + nval/2 + 3 (py_calc_modes.f:256),
+  // rounded up to a whole number of blocks
+  const int bq = (int)a.block;
+  int m = 2 * nval + nval / 2 + 3;
+  m = ((m + bq - 1) / bq) * bq;
+  if (m + bq >= n || nval >= m) throw std::invalid_argument("calc_modes: num_BMs too large for the mesh");
   // material coefficients (py_calc_modes.f:583-599); prime = +bloch, adjoint = -bloch
   std::vector<cplx> pp((size_t)nb * ntyp), qq((size_t)nb * ntyp), sh(nb);
   std::vector<double> bl(2 * (size_t)nb);
@@ -743,7 +779,7 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   lu_factor(dm, nb, m1, pool, piv, pfl, s);
   // 3. eigensolve
   zd *Xn = c->buf<zd>("Xn", (size_t)nb * n * nval);
-  EigResult er = krylov_schur(c, dm, nb, nval, m, (int)a.itermax, a.tol, pool, piv, m2p, Xn);
+  EigResult er = krylov_schur(c, dm, nb, nval, m, bq, (int)a.itermax, a.tol, pool, piv, m2p, Xn);
   zd *Xo = c->buf<zd>("Xo", (size_t)nb * n * nval);
   launch_permute_vec(dm, nb, nval, Xn, Xo, false, s);
   // 4. beta, sort (z_indexx on -Re(beta^2))
@@ -885,6 +921,7 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
     a.stats[1] = er.restarts;
     a.stats[2] = std::accumulate(pf.begin(), pf.end(), (int64_t)0);
     a.stats[3] = er.steps;
+    a.stats[4] = er.rank_fixes;
   }
 }
 

@@ -46,6 +46,7 @@ struct ModesArgs {
   const double *bloch = nullptr;   // nl x 2
   const double *shift = nullptr;   // nl complex
   double tol = 1e-12;
+  int64_t block = 8;               // Krylov block size (1, 2, 4 or 8)
   double *beta = nullptr;          // nl x nval complex
   double *J = nullptr;             // nl x (2P x nval) complex
   double *Jd = nullptr;            // nl x (nval x 2P) complex

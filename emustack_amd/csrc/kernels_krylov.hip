@@ -283,3 +283,172 @@ void launch_copy(int nb, int64_t n, const zd *src, int64_t ss, zd *dst, int64_t 
 }
 
 }  // namespace emu
+
+namespace emu {
+
+// ---------------------------------------------- block Gram-Schmidt helpers
+// W[b][:, q] -= sum_{c<k} V[b][:, c] H[b][c, q],  q < bq  (thread per row,
+// H staged through LDS in chunks of 64 rows)
+template <int BQ>
+__global__ __launch_bounds__(256) void k_sub_vh_blk(int64_t n, int k, const zd *__restrict__ V, int64_t ldv,
+                                                    int64_t sV, const zd *__restrict__ H, int64_t ldh, int64_t sH,
+                                                    zd *__restrict__ W, int64_t ldw, int64_t sW) {
+  __shared__ zd hs[64][BQ];
+  const int b = blockIdx.y;
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const zd *Vb = V + b * sV;
+  const zd *Hb = H + b * sH;
+  zd acc[BQ];
+#pragma unroll
+  for (int q = 0; q < BQ; ++q) acc[q] = (r < n) ? W[b * sW + q * ldw + r] : zmk(0, 0);
+  for (int c0 = 0; c0 < k; c0 += 64) {
+    const int cn = min(64, k - c0);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 64 * BQ; idx += blockDim.x) {
+      int c = idx / BQ, q = idx - c * BQ;
+      hs[c][q] = c < cn ? Hb[(c0 + c) + (int64_t)q * ldh] : zmk(0, 0);
+    }
+    __syncthreads();
+    if (r < n)
+      for (int c = 0; c < cn; ++c) {
+        const zd v = Vb[r + (int64_t)(c0 + c) * ldv];
+#pragma unroll
+        for (int q = 0; q < BQ; ++q) acc[q] = zfms(v, hs[c][q], acc[q]);
+      }
+  }
+  if (r < n)
+#pragma unroll
+    for (int q = 0; q < BQ; ++q) W[b * sW + q * ldw + r] = acc[q];
+}
+
+void launch_sub_vh_blk(int nb, int64_t n, int k, int bq, const zd *V, int64_t ldv, int64_t sV, const zd *H,
+                       int64_t ldh, int64_t sH, zd *W, int64_t ldw, int64_t sW, hipStream_t s) {
+  dim3 g((unsigned)((n + 255) / 256), nb);
+  switch (bq) {
+    case 1: hipLaunchKernelGGL(k_sub_vh_blk<1>, g, dim3(256), 0, s, n, k, V, ldv, sV, H, ldh, sH, W, ldw, sW); break;
+    case 2: hipLaunchKernelGGL(k_sub_vh_blk<2>, g, dim3(256), 0, s, n, k, V, ldv, sV, H, ldh, sH, W, ldw, sW); break;
+    case 4: hipLaunchKernelGGL(k_sub_vh_blk<4>, g, dim3(256), 0, s, n, k, V, ldv, sV, H, ldh, sH, W, ldw, sW); break;
+    case 8: hipLaunchKernelGGL(k_sub_vh_blk<8>, g, dim3(256), 0, s, n, k, V, ldv, sV, H, ldh, sH, W, ldw, sW); break;
+    default: throw HipError("sub_vh_blk: block size must be 1, 2, 4 or 8");
+  }
+  EMU_LAUNCH_CHECK();
+}
+
+// Cholesky of the bq x bq Gram matrix G = R^H R (R upper), its inverse, and
+// Racc = R * Rprev (Rprev = identity when null).  One thread per instance.
+// A non-positive pivot (rank-deficient block) is replaced by a tiny value
+// and counted in bad[b].
+__global__ void k_cholqr(int nb, int bq, const zd *__restrict__ G, zd *__restrict__ Rinv,
+                         const zd *__restrict__ Rprev, zd *__restrict__ Racc, int32_t *__restrict__ bad) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const zd *g = G + (int64_t)b * bq * bq;
+  zd R[8][8], Ri[8][8];
+  for (int i = 0; i < bq; ++i)
+    for (int j = 0; j < bq; ++j) { R[i][j] = zmk(0, 0); Ri[i][j] = zmk(0, 0); }
+  double gmax = 0;
+  for (int i = 0; i < bq; ++i) gmax = fmax(gmax, g[i + i * bq].x);
+  for (int j = 0; j < bq; ++j) {
+    // R[i][j] for i < j, then R[j][j]
+    for (int i = 0; i < j; ++i) {
+      zd acc = g[i + j * bq];  // G(i, j)
+      for (int k = 0; k < i; ++k) acc = zsub(acc, zmul(zconj(R[k][i]), R[k][j]));
+      R[i][j] = zdiv(acc, R[i][i]);
+    }
+    double d = g[j + j * bq].x;
+    for (int k = 0; k < j; ++k) d -= R[k][j].x * R[k][j].x + R[k][j].y * R[k][j].y;
+    if (!(d > 1e-28 * gmax)) {
+      d = 1e-28 * gmax + 1e-300;
+      bad[b] += 1;
+    }
+    R[j][j] = zmk(sqrt(d), 0);
+  }
+  // inverse of upper triangular R
+  for (int j = 0; j < bq; ++j) {
+    Ri[j][j] = zdiv(zmk(1, 0), R[j][j]);
+    for (int i = j - 1; i >= 0; --i) {
+      zd acc = zmk(0, 0);
+      for (int k = i + 1; k <= j; ++k) acc = zadd(acc, zmul(R[i][k], Ri[k][j]));
+      Ri[i][j] = zsub(zmk(0, 0), zdiv(acc, R[i][i]));
+    }
+  }
+  zd *ri = Rinv + (int64_t)b * bq * bq;
+  zd *ra = Racc + (int64_t)b * bq * bq;
+  const zd *rp = Rprev ? Rprev + (int64_t)b * bq * bq : nullptr;
+  for (int i = 0; i < bq; ++i)
+    for (int j = 0; j < bq; ++j) {
+      ri[i + j * bq] = Ri[i][j];
+      zd acc = zmk(0, 0);
+      if (rp) {
+        for (int k = 0; k < bq; ++k) acc = zadd(acc, zmul(R[i][k], rp[k + j * bq]));
+      } else {
+        acc = R[i][j];
+      }
+      ra[i + j * bq] = acc;
+    }
+}
+
+void launch_cholqr(int nb, int bq, const zd *G, zd *Rinv, const zd *Rprev, zd *Racc, int32_t *bad,
+                   hipStream_t s) {
+  if (bq > 8) throw HipError("cholqr: block size above 8");
+  hipLaunchKernelGGL(k_cholqr, dim3((nb + 63) / 64), dim3(64), 0, s, nb, bq, G, Rinv, Rprev, Racc, bad);
+  EMU_LAUNCH_CHECK();
+}
+
+// Wout[b][:, 0:bq] = W[b][:, 0:bq] * Rinv[b]  (bq x bq upper triangular)
+template <int BQ>
+__global__ __launch_bounds__(256) void k_trmm_right(int64_t n, const zd *W, int64_t ldw, int64_t sW,
+                                                    const zd *__restrict__ Ri, zd *O, int64_t ldo,
+                                                    int64_t sO) {
+  __shared__ zd rs[BQ][BQ];
+  const int b = blockIdx.y;
+  for (int idx = threadIdx.x; idx < BQ * BQ; idx += blockDim.x) rs[idx % BQ][idx / BQ] = Ri[(int64_t)b * BQ * BQ + idx];
+  __syncthreads();
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  zd w[BQ];
+#pragma unroll
+  for (int q = 0; q < BQ; ++q) w[q] = W[b * sW + q * ldw + r];
+#pragma unroll
+  for (int j = 0; j < BQ; ++j) {
+    zd acc = zmk(0, 0);
+#pragma unroll
+    for (int k = 0; k <= j; ++k) acc = zfma(w[k], rs[k][j], acc);
+    O[b * sO + j * ldo + r] = acc;
+  }
+}
+
+void launch_trmm_right(int nb, int64_t n, int bq, const zd *W, int64_t ldw, int64_t sW, const zd *Ri, zd *O,
+                       int64_t ldo, int64_t sO, hipStream_t s) {
+  dim3 g((unsigned)((n + 255) / 256), nb);
+  switch (bq) {
+    case 1: hipLaunchKernelGGL(k_trmm_right<1>, g, dim3(256), 0, s, n, W, ldw, sW, Ri, O, ldo, sO); break;
+    case 2: hipLaunchKernelGGL(k_trmm_right<2>, g, dim3(256), 0, s, n, W, ldw, sW, Ri, O, ldo, sO); break;
+    case 4: hipLaunchKernelGGL(k_trmm_right<4>, g, dim3(256), 0, s, n, W, ldw, sW, Ri, O, ldo, sO); break;
+    case 8: hipLaunchKernelGGL(k_trmm_right<8>, g, dim3(256), 0, s, n, W, ldw, sW, Ri, O, ldo, sO); break;
+    default: throw HipError("trmm_right: block size must be 1, 2, 4 or 8");
+  }
+  EMU_LAUNCH_CHECK();
+}
+
+// Dst[b][i + j*ldd] (+)= Src[b][i + j*lds] for i < rows, j < cols
+__global__ void k_block_copy(int rows, int cols, const zd *__restrict__ S, int64_t lds, int64_t sS,
+                             zd *__restrict__ D, int64_t ldd, int64_t sD, int add) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (idx >= rows * cols) return;
+  const int i = idx % rows, j = idx / rows;
+  zd v = S[b * sS + i + (int64_t)j * lds];
+  zd *d = &D[b * sD + i + (int64_t)j * ldd];
+  *d = add ? zadd(*d, v) : v;
+}
+
+void launch_block_copy(int nb, int rows, int cols, const zd *S, int64_t lds, int64_t sS, zd *D, int64_t ldd,
+                       int64_t sD, bool add, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return;
+  hipLaunchKernelGGL(k_block_copy, dim3((rows * cols + 255) / 256, nb), dim3(256), 0, s, rows, cols, S, lds, sS,
+                     D, ldd, sD, add ? 1 : 0);
+  EMU_LAUNCH_CHECK();
+}
+
+}  // namespace emu

@@ -271,7 +271,21 @@ void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int3
 }
 
 // ---------------------------------------------------------------- solves
+//
+// One workgroup per (front, instance, chunk of RC right-hand sides).  The
+// front's RHS block r[f][RC] lives in LDS.  Triangular parts are processed in
+// SB-column blocks: the SB x SB diagonal block is solved by one wavefront per
+// RHS column (register shuffles, no barrier), then all 256 lanes apply the
+// block to the remaining rows (coalesced column reads of the factor).  Two
+// barriers per SB columns instead of one per column.
 
+constexpr int SB = 32;
+
+__device__ __forceinline__ zd shfl_z(zd v, int lane) {
+  return zmk(__shfl(v.x, lane, 64), __shfl(v.y, lane, 64));
+}
+
+template <int RC>
 __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ fronts, FrontArrays A,
                                              const int32_t *__restrict__ child0,
                                              const int32_t *__restrict__ child1, const zd *__restrict__ pool,
@@ -281,99 +295,220 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ fronts,
   extern __shared__ zd r[];
   const int t = fronts[blockIdx.x];
   const int b = blockIdx.y;
-  const int tid = threadIdx.x;
+  const int q0 = blockIdx.z * RC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int s = A.s[t], u = A.u[t], f = A.f[t], fb = A.b[t];
   const zd *F = pool + b * psz + A.off[t];
   const int32_t *pv = piv + b * npiv + A.pivoff[t];
-  const int ch[2] = {child0[t], child1[t]};
-  for (int q = 0; q < nrhs; ++q) {
-    const zd *bb = bv + ((int64_t)b * nrhs + q) * n;
-    zd *yy = y + ((int64_t)b * nrhs + q) * n;
-    zd *ww = wbuf + ((int64_t)b * nrhs + q) * nw;
-    for (int i = tid; i < f; i += 256) r[i] = i < s ? bb[fb + i] : zmk(0, 0);
-    __syncthreads();
-    for (int c = 0; c < 2; ++c) {
-      int cf = ch[c];
-      if (cf < 0) continue;
-      const int uc = A.u[cf];
-      const int32_t *mp = A.pmap + A.uoff[cf];
-      const zd *wc = ww + A.uoff[cf];
-      for (int k = tid; k < uc; k += 256) r[mp[k]] = zadd(r[mp[k]], wc[k]);
-      __syncthreads();
-    }
-    if (tid == 0)
-      for (int k = 0; k < s; ++k) {
-        int p = pv[k];
-        if (p != k) { zd a = r[k]; r[k] = r[p]; r[p] = a; }
+  const int nq = min(RC, nrhs - q0);
+  const int64_t col0 = (int64_t)b * nrhs + q0;
+  for (int idx = tid; idx < f * RC; idx += 256) {
+    int i = idx / RC, q = idx - i * RC;
+    r[idx] = (i < s && q < nq) ? bv[(col0 + q) * n + fb + i] : zmk(0, 0);
+  }
+  __syncthreads();
+  for (int c = 0; c < 2; ++c) {
+    const int cf = c == 0 ? child0[t] : child1[t];
+    if (cf < 0) continue;
+    const int uc = A.u[cf];
+    const int32_t *mp = A.pmap + A.uoff[cf];
+    for (int idx = tid; idx < uc * RC; idx += 256) {
+      int k = idx / RC, q = idx - k * RC;
+      if (q < nq) {
+        zd *d = &r[mp[k] * RC + q];
+        *d = zadd(*d, wbuf[(col0 + q) * nw + A.uoff[cf] + k]);
       }
-    __syncthreads();
-    for (int k = 0; k < s; ++k) {
-      const zd yk = r[k];
-      const zd *col = F + (int64_t)k * f;
-      for (int i = k + 1 + tid; i < f; i += 256) r[i] = zfms(col[i], yk, r[i]);
-      __syncthreads();
     }
-    for (int i = tid; i < s; i += 256) yy[fb + i] = r[i];
-    for (int i = tid; i < u; i += 256) ww[A.uoff[t] + i] = r[s + i];
     __syncthreads();
+  }
+  if (tid < nq)
+    for (int k = 0; k < s; ++k) {
+      int p = pv[k];
+      if (p != k) {
+        zd a = r[k * RC + tid];
+        r[k * RC + tid] = r[p * RC + tid];
+        r[p * RC + tid] = a;
+      }
+    }
+  __syncthreads();
+  __shared__ zd D[SB][SB + 1];
+  for (int kb = 0; kb < s; kb += SB) {
+    const int w = min(SB, s - kb);
+    for (int idx = tid; idx < w * w; idx += 256) {
+      int i = idx % w, j = idx / w;
+      D[i][j] = F[(kb + i) + (int64_t)(kb + j) * f];
+    }
+    __syncthreads();
+    for (int q = wave; q < nq; q += 4) {
+      zd v = lane < w ? r[(kb + lane) * RC + q] : zmk(0, 0);
+      for (int k = 0; k < w - 1; ++k) {
+        zd yk = shfl_z(v, k);
+        if (lane > k && lane < w) v = zfms(D[lane][k], yk, v);
+      }
+      if (lane < w) r[(kb + lane) * RC + q] = v;
+    }
+    __syncthreads();
+    for (int i = kb + w + tid; i < f; i += 256) {
+      zd acc[RC];
+#pragma unroll
+      for (int q = 0; q < RC; ++q) acc[q] = r[i * RC + q];
+      for (int kk = 0; kk < w; ++kk) {
+        const zd l = F[i + (int64_t)(kb + kk) * f];
+#pragma unroll
+        for (int q = 0; q < RC; ++q) acc[q] = zfms(l, r[(kb + kk) * RC + q], acc[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < RC; ++q) r[i * RC + q] = acc[q];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < f * RC; idx += 256) {
+    int i = idx / RC, q = idx - i * RC;
+    if (q >= nq) continue;
+    if (i < s) y[(col0 + q) * n + fb + i] = r[idx];
+    else wbuf[(col0 + q) * nw + A.uoff[t] + (i - s)] = r[idx];
   }
 }
 
+template <int RC>
 __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ fronts, FrontArrays A,
                                              const zd *__restrict__ pool, int64_t psz, int nrhs, int64_t n,
                                              const zd *__restrict__ y, zd *__restrict__ x) {
   extern __shared__ zd r[];
   const int t = fronts[blockIdx.x];
   const int b = blockIdx.y;
-  const int tid = threadIdx.x;
+  const int q0 = blockIdx.z * RC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int s = A.s[t], u = A.u[t], f = A.f[t], fb = A.b[t];
   const zd *F = pool + b * psz + A.off[t];
   const int32_t *ui = A.uidx + A.uoff[t];
-  for (int q = 0; q < nrhs; ++q) {
-    const zd *yy = y + ((int64_t)b * nrhs + q) * n;
-    zd *xx = x + ((int64_t)b * nrhs + q) * n;
-    for (int i = tid; i < f; i += 256) r[i] = i < s ? yy[fb + i] : xx[ui[i - s]];
-    __syncthreads();
-    for (int i = tid; i < s; i += 256) {
-      zd acc = r[i];
-      for (int c = 0; c < u; ++c) acc = zfms(F[i + (int64_t)(s + c) * f], r[s + c], acc);
-      r[i] = acc;
+  const int nq = min(RC, nrhs - q0);
+  const int64_t col0 = (int64_t)b * nrhs + q0;
+  for (int idx = tid; idx < f * RC; idx += 256) {
+    int i = idx / RC, q = idx - i * RC;
+    zd v = zmk(0, 0);
+    if (q < nq) v = i < s ? y[(col0 + q) * n + fb + i] : x[(col0 + q) * n + ui[i - s]];
+    r[idx] = v;
+  }
+  __syncthreads();
+  for (int i = tid; i < s; i += 256) {
+    zd acc[RC];
+#pragma unroll
+    for (int q = 0; q < RC; ++q) acc[q] = r[i * RC + q];
+    for (int c = 0; c < u; ++c) {
+      const zd uv = F[i + (int64_t)(s + c) * f];
+#pragma unroll
+      for (int q = 0; q < RC; ++q) acc[q] = zfms(uv, r[(s + c) * RC + q], acc[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < RC; ++q) r[i * RC + q] = acc[q];
+  }
+  __syncthreads();
+  __shared__ zd D[SB][SB + 1];
+  for (int bi = (s + SB - 1) / SB - 1; bi >= 0; --bi) {
+    const int kb = bi * SB, w = min(SB, s - kb);
+    for (int idx = tid; idx < w * w; idx += 256) {
+      int i = idx % w, j = idx / w;
+      D[i][j] = F[(kb + i) + (int64_t)(kb + j) * f];
     }
     __syncthreads();
-    for (int k = s - 1; k >= 0; --k) {
-      const zd xk = zdiv(r[k], F[k + (int64_t)k * f]);
-      const zd *col = F + (int64_t)k * f;
-      for (int i = tid; i < k; i += 256) r[i] = zfms(col[i], xk, r[i]);
-      __syncthreads();
-      if (tid == 0) r[k] = xk;
+    for (int q = wave; q < nq; q += 4) {
+      zd v = lane < w ? r[(kb + lane) * RC + q] : zmk(0, 0);
+      for (int k = w - 1; k >= 0; --k) {
+        const zd xk = zdiv(shfl_z(v, k), D[k][k]);
+        if (lane == k) v = xk;
+        else if (lane < k) v = zfms(D[lane][k], xk, v);
+      }
+      if (lane < w) r[(kb + lane) * RC + q] = v;
     }
     __syncthreads();
-    for (int i = tid; i < s; i += 256) xx[fb + i] = r[i];
+    for (int i = tid; i < kb; i += 256) {
+      zd acc[RC];
+#pragma unroll
+      for (int q = 0; q < RC; ++q) acc[q] = r[i * RC + q];
+      for (int kk = 0; kk < w; ++kk) {
+        const zd uv = F[i + (int64_t)(kb + kk) * f];
+#pragma unroll
+        for (int q = 0; q < RC; ++q) acc[q] = zfms(uv, r[(kb + kk) * RC + q], acc[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < RC; ++q) r[i * RC + q] = acc[q];
+    }
     __syncthreads();
   }
+  for (int idx = tid; idx < s * RC; idx += 256) {
+    int i = idx / RC, q = idx - i * RC;
+    if (q < nq) x[(col0 + q) * n + fb + i] = r[idx];
+  }
+}
+
+static const size_t kMaxLds = 160 * 1024 - 16 * 1024;  // dynamic part (static D block + margin)
+
+template <int RC>
+static void set_attr_once() {
+  static bool attr = false;
+  if (!attr) {
+    EMU_HIP(hipFuncSetAttribute((const void *)k_fwd<RC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds));
+    EMU_HIP(hipFuncSetAttribute((const void *)k_bwd<RC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds));
+    attr = true;
+  }
+}
+
+// largest RHS chunk (8, 4, 2, 1) not above nrhs whose LDS block fits
+static int pick_rc(int nrhs, int max_f) {
+  const size_t per = (size_t)max_f * sizeof(zd);
+  for (int rc : {8, 4, 2}) if (nrhs >= rc && rc * per <= kMaxLds) return rc;
+  return 1;
+}
+
+template <int RC>
+static void launch_fwd(const LevelSched &L, DevMesh &dm, FrontArrays A, const int32_t *c0, const int32_t *c1, int nb,
+                       int nrhs, const zd *pool, const int32_t *piv, const zd *bnew, zd *y, zd *w, hipStream_t s) {
+  set_attr_once<RC>();
+  const Symbolic &sy = dm.host.sym;
+  const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
+  size_t lds = (size_t)L.max_f * RC * sizeof(zd);
+  hipLaunchKernelGGL(k_fwd<RC>, dim3((unsigned)L.fronts_cnt, nb, nz), dim3(256), lds, s, dm.tasks.p + L.fronts_off,
+                     A, c0, c1, pool, sy.pool, piv, sy.npiv, nrhs, sy.n, sy.nw, bnew, y, w);
+}
+
+template <int RC>
+static void launch_bwd(const LevelSched &L, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *pool,
+                       const zd *y, zd *x, hipStream_t s) {
+  set_attr_once<RC>();
+  const Symbolic &sy = dm.host.sym;
+  const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
+  size_t lds = (size_t)L.max_f * RC * sizeof(zd);
+  hipLaunchKernelGGL(k_bwd<RC>, dim3((unsigned)L.fronts_cnt, nb, nz), dim3(256), lds, s, dm.tasks.p + L.fronts_off,
+                     A, pool, sy.pool, nrhs, sy.n, y, x);
 }
 
 void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *pool, const int32_t *piv, const zd *bnew,
               zd *y, zd *x, zd *w, hipStream_t s, Profiler *prof) {
-  const Symbolic &sy = dm.host.sym;
+  if ((size_t)dm.max_f * sizeof(zd) > kMaxLds) throw HipError("front too large for the LDS-resident solve");
   FrontArrays A = fa(dm);
-  size_t lds = (size_t)dm.max_f * sizeof(zd);
   const int32_t *c0 = dm.tasks.p + dm.child_off[0];
   const int32_t *c1 = dm.tasks.p + dm.child_off[1];
-  if (prof) prof->start_sweep();
   const double scale = (double)nb * nrhs;
   for (const LevelSched &L : dm.sched) {
     int ev = prof ? prof->begin(s) : -1;
-    hipLaunchKernelGGL(k_fwd, dim3((unsigned)L.fronts_cnt, nb), dim3(256), lds, s,
-                       dm.tasks.p + L.fronts_off, A, c0, c1, pool, sy.pool, piv, sy.npiv, nrhs,
-                       sy.n, sy.nw, bnew, y, w);
+    switch (pick_rc(nrhs, L.max_f)) {
+      case 8: launch_fwd<8>(L, dm, A, c0, c1, nb, nrhs, pool, piv, bnew, y, w, s); break;
+      case 4: launch_fwd<4>(L, dm, A, c0, c1, nb, nrhs, pool, piv, bnew, y, w, s); break;
+      case 2: launch_fwd<2>(L, dm, A, c0, c1, nb, nrhs, pool, piv, bnew, y, w, s); break;
+      default: launch_fwd<1>(L, dm, A, c0, c1, nb, nrhs, pool, piv, bnew, y, w, s); break;
+    }
     if (prof) prof->end(s, ev, 0, L.fwd_bytes * scale);
   }
   EMU_LAUNCH_CHECK();
   for (auto it = dm.sched.rbegin(); it != dm.sched.rend(); ++it) {
     int ev = prof ? prof->begin(s) : -1;
-    hipLaunchKernelGGL(k_bwd, dim3((unsigned)it->fronts_cnt, nb), dim3(256), lds, s,
-                       dm.tasks.p + it->fronts_off, A, pool, sy.pool, nrhs, sy.n, y, x);
+    switch (pick_rc(nrhs, it->max_f)) {
+      case 8: launch_bwd<8>(*it, dm, A, nb, nrhs, pool, y, x, s); break;
+      case 4: launch_bwd<4>(*it, dm, A, nb, nrhs, pool, y, x, s); break;
+      case 2: launch_bwd<2>(*it, dm, A, nb, nrhs, pool, y, x, s); break;
+      default: launch_bwd<1>(*it, dm, A, nb, nrhs, pool, y, x, s); break;
+    }
     if (prof) prof->end(s, ev, 1, it->bwd_bytes * scale);
   }
   EMU_LAUNCH_CHECK();

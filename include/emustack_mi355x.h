@@ -106,8 +106,9 @@ int emu_calc_modes_2d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval,
  * instance-major: lambda[nbatch], n_eff[nbatch*2*nb_typ_el],
  * bloch_vec[nbatch*2], shift[nbatch*2]; outputs beta1[nbatch*2*nval],
  * overlap_J[nbatch*2*2neq_PW*nval], overlap_J_dagger[nbatch*2*nval*2neq_PW].
- * opts[0] = eigen tolerance (<= 0: default 1e-12); opts[1] = precision
- * (0 = complex128 everywhere).  stats[0..7] (may be NULL): OP applications,
+ * opts (may be NULL; 4 entries): opts[0] = eigen tolerance (<= 0: default
+ * 1e-12); opts[1] = precision (0 = complex128 everywhere); opts[2] = Krylov
+ * block size (1, 2, 4 or 8; <= 0: default 8); opts[3] reserved (0).  stats[0..7] (may be NULL): OP applications,
  * restarts (max over instances), pivot perturbations, Arnoldi steps,
  * 0, 0, 0, 0. */
 int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch,
@@ -123,7 +124,7 @@ int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch,
  * class k: launches timed, total milliseconds, total algorithmic bytes of
  * those launches.  Classes: 0 forward-solve level sweep (k_fwd), 1 backward
  * level sweep (k_bwd), 2 Gram-Schmidt projections (k_atb_partial),
- * 3 Gram-Schmidt updates (k_sub_vh), 4 mass-matrix SpMV (k_spmm),
+ * 3 Gram-Schmidt updates (k_sub_vh_blk), 4 mass-matrix SpMV (k_spmm),
  * 5..7 reserved. */
 int emu_profile(emu_ctx *ctx, int enable, int every, double *out);
 
