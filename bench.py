@@ -156,7 +156,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU; the workload is independent wavelength points and '
-              'would shard by wavelength across GPUs (see DESIGN.md), but only --gpus 1 is measured.',
+              'would shard by wavelength across GPUs, but only --gpus 1 is measured.',
               file=sys.stderr)
         sys.exit(2)
     os.makedirs(args.workdir, exist_ok=True)

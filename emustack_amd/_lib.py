@@ -4,6 +4,7 @@ The library is built in-tree by emustack_amd.build; importing this module
 never builds and never falls back to anything else: if the library is
 missing the error says how to build it.
 """
+import contextlib
 import ctypes
 import os
 import threading
@@ -47,6 +48,20 @@ class EmuError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__('%s (status %d)' % (msg, code))
         self.code = code
+
+
+def _host_lapack_threads(nl):
+    """The library projects the Krylov-Schur problem of each batch instance
+    on its own host thread; the registered LAPACK (scipy's OpenBLAS) then
+    runs single-threaded so that concurrent calls do not contend for one
+    BLAS thread pool."""
+    if nl <= 1:
+        return contextlib.nullcontext()
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:
+        return contextlib.nullcontext()
+    return threadpool_limits(limits=1, user_api='blas')
 
 
 def _lapack_pointers():
@@ -171,11 +186,12 @@ class Context:
         Jd = np.zeros((nl, 2 * neq_pw, nval), np.complex128)   # C-order view of F-order (nval x 2P)
         opts = np.array([tol, 0.0, float(block), 0.0])
         st = np.zeros(8, np.int64)
-        check(lib().emu_calc_modes_2d_batch(self.h, mesh.h, nl, ptr(lam), nval, ordre, ntyp,
-                                            ptr(n_eff.view(np.float64)), ptr(bloch),
-                                            ptr(shift.view(np.float64)), E_H_field, itermax, neq_pw, ptr(opts),
-                                            ptr(beta.view(np.float64)), ptr(J.view(np.float64)),
-                                            ptr(Jd.view(np.float64)), ptr(st, c_i64p)))
+        with _host_lapack_threads(nl):
+            check(lib().emu_calc_modes_2d_batch(self.h, mesh.h, nl, ptr(lam), nval, ordre, ntyp,
+                                                ptr(n_eff.view(np.float64)), ptr(bloch),
+                                                ptr(shift.view(np.float64)), E_H_field, itermax, neq_pw, ptr(opts),
+                                                ptr(beta.view(np.float64)), ptr(J.view(np.float64)),
+                                                ptr(Jd.view(np.float64)), ptr(st, c_i64p)))
         return beta, J.transpose(0, 2, 1), Jd.transpose(0, 2, 1), st
 
     PROFILE_CLASSES = ('solve_fwd', 'solve_bwd', 'gs_project', 'gs_update', 'spmv_M')

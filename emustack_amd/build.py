@@ -16,8 +16,8 @@ SOURCES = ['mesh.cpp', 'symbolic.cpp', 'capi.cpp', 'kernels_fem.hip', 'kernels_l
            'kernels_krylov.hip', 'kernels_post.hip', 'engine.hip']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('EMU_OFFLOAD_ARCH', 'gfx950')
-# packed-fp32 ops are disabled for all device code (known
-# hazards of this platform)
+# packed-fp32 instructions (v_pk_*_f32) are disabled for all device code:
+# some of their forms have produced silently wrong results on this hardware
 DEVFLAGS = ['-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
 CXXFLAGS = ['-O3', '-fPIC', '-std=c++17', '-Wall', '-Wno-unused-function', '-Wno-unused-variable',
             '-Wno-unused-but-set-variable', '-Wno-unused-result']

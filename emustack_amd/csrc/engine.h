@@ -110,8 +110,12 @@ struct LevelSched {
   std::vector<StepPlan> steps;
   int64_t fronts_off, fronts_cnt;     // fronts of the level
   int64_t ea_off[2], ea_cnt[2];       // extend-add tasks (child front, col0) by slot
+  int64_t sf_off, sf_cnt;             // forward-solve row tiles (front, row0)
+  int64_t sb_off, sb_cnt;             // backward-solve row tiles (front, row0)
   int32_t max_f;
-  double fwd_bytes = 0, bwd_bytes = 0;  // algorithmic bytes per instance and RHS
+  // algorithmic bytes of one solve sweep of the level, per instance: factor
+  // bytes (once per RHS chunk) and vector bytes (per RHS)
+  double fwd_mat = 0, fwd_vec = 0, bwd_mat = 0, bwd_vec = 0;
 };
 
 struct DevMesh {
@@ -139,8 +143,12 @@ struct DevMesh {
   DBuf<int64_t> foff, fuoff, fpivoff;
   DBuf<int32_t> uidx, pmap;
   DBuf<int64_t> nnz2front;
-  DBuf<int32_t> tasks;               // all task lists (+ child arrays)
-  int64_t child_off[2] = {0, 0};     // child0[t], child1[t] inside tasks
+  DBuf<int64_t> fioff, ffvoff;       // inverted-form offset (prefix of 2 f s), prefix of f
+  DBuf<int32_t> cmap;                // per front row: update-vector index of child slot 0 / 1, or -1
+  int64_t isz = 0;                   // inverted-form entries per instance
+  DBuf<int32_t> tasks;               // all task lists
+  int64_t tri_off = 0, tri_cnt = 0;  // triangular inversion tasks (front, J, kind)
+  int64_t wz_off = 0, wz_cnt = 0;    // off-diagonal inverted blocks (front, kind, row0, col0)
   std::vector<LevelSched> sched;     // sched[0] = deepest level
   int32_t max_f = 0;
   // geometry for the plane-wave integrals
@@ -160,11 +168,16 @@ void launch_gather(DevMesh &dm, int nb, int ntyp, const zd *pp, const zd *qq, co
 void launch_permute_vals(DevMesh &dm, int nb, const zd *src, zd *dst, hipStream_t s);
 
 // ---------------- LU kernels (kernels_lu.hip) ----------------
-void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int32_t *pflag,
-               hipStream_t s);
-// x = A^{-1} b, both in the new (nested-dissection) numbering, [nb][nrhs][n]
-void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *pool, const int32_t *piv, const zd *b,
-              zd *y, zd *x, zd *w, hipStream_t s, Profiler *prof = nullptr);
+// numeric factorisation into pool (piv: LAPACK-style row swaps, pflag: zero
+// pivots perturbed, sig: pivot-order gather), then the inverted-factor form
+// inv [nb][isz] that the solves read
+void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int32_t *pflag, int32_t *sig,
+               zd *inv, hipStream_t s);
+void lu_invert(DevMesh &dm, int nb, const zd *pool, zd *inv, hipStream_t s);
+// x = A^{-1} b, both in the new (nested-dissection) numbering, [nb][nrhs][n];
+// y (same shape) and w ([nb][nrhs][nw]) are work vectors
+void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *sig, const zd *b, zd *y, zd *x,
+              zd *w, hipStream_t s, Profiler *prof = nullptr);
 // permute vectors old <-> new numbering: dst[new] = src[perm[new]] or inverse
 void launch_permute_vec(DevMesh &dm, int nb, int ncol, const zd *src, zd *dst, bool to_new,
                         hipStream_t s);

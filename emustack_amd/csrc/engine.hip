@@ -186,26 +186,76 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
       }
       L.ea_cnt[sl] = ((int64_t)tasks.size() - L.ea_off[sl]) / 2;
     }
-    // algorithmic bytes of one solve sweep of this level (per instance, per RHS)
+    // solve row tiles
+    L.sf_off = (int64_t)tasks.size();
+    for (int32_t t : LP.fronts)
+      for (int32_t r0 = 0; r0 < sy.fronts[t].f; r0 += 64) { tasks.push_back(t); tasks.push_back(r0); }
+    L.sf_cnt = ((int64_t)tasks.size() - L.sf_off) / 2;
+    L.sb_off = (int64_t)tasks.size();
+    for (int32_t t : LP.fronts)
+      for (int32_t r0 = 0; r0 < sy.fronts[t].s; r0 += 64) { tasks.push_back(t); tasks.push_back(r0); }
+    L.sb_cnt = ((int64_t)tasks.size() - L.sb_off) / 2;
+    // algorithmic bytes of one solve sweep of this level (per instance):
+    // triangle incl. diagonal + off-diagonal block of the inverted form once
+    // per RHS chunk; b/y/x/update vectors per RHS
     for (int32_t t : LP.fronts) {
       const FrontInfo &F = sy.fronts[t];
       double ss = F.s, uu = F.u, wc = 0;
       for (int c = 0; c < F.nchild; ++c) wc += sy.fronts[F.child[c]].u;
-      L.fwd_bytes += 16.0 * (ss * (ss - 1) / 2 + uu * ss + 2 * ss + uu + wc) + 4.0 * ss;
-      L.bwd_bytes += 16.0 * (ss * (ss + 1) / 2 + ss * uu + 2 * ss + uu) + 4.0 * uu;
+      L.fwd_mat += 16.0 * (ss * (ss + 1) / 2 + uu * ss) + 4.0 * ss + 8.0 * (ss + uu);
+      L.fwd_vec += 16.0 * (2 * ss + uu + wc);
+      L.bwd_mat += 16.0 * (ss * (ss + 1) / 2 + ss * uu) + 4.0 * uu;
+      L.bwd_vec += 16.0 * (2 * ss + uu);
     }
     dm.sched.push_back(L);
   }
-  for (int c = 0; c < 2; ++c) {
-    dm.child_off[c] = (int64_t)tasks.size();
-    for (int32_t i = 0; i < nf; ++i) tasks.push_back(sy.fronts[i].child[c]);
+  // inverted-factor form: offsets, child update-vector map, task lists
+  {
+    std::vector<int64_t> io(nf), fv(nf);
+    int64_t acc = 0, accf = 0;
+    for (int32_t i = 0; i < nf; ++i) {
+      const FrontInfo &F = sy.fronts[i];
+      io[i] = acc;
+      fv[i] = accf;
+      acc += 2 * (int64_t)F.f * F.s;
+      accf += F.f;
+    }
+    dm.isz = std::max<int64_t>(acc, 1);
+    std::vector<int32_t> cm(2 * std::max<int64_t>(accf, 1), -1);
+    for (int32_t i = 0; i < nf; ++i) {
+      const FrontInfo &F = sy.fronts[i];
+      if (F.parent < 0) continue;
+      for (int32_t k = 0; k < F.u; ++k) {
+        int32_t prow = sy.pmap[F.uoff + k];
+        cm[2 * (fv[F.parent] + prow) + F.slot] = (int32_t)(F.uoff + k);
+      }
+    }
+    dm.fioff.upload(io, s);
+    dm.ffvoff.upload(fv, s);
+    dm.cmap.upload(cm, s);
+    dm.tri_off = (int64_t)tasks.size();
+    for (int32_t i = 0; i < nf; ++i)
+      for (int kind = 0; kind < 2; ++kind)
+        for (int32_t J = 0; J < (sy.fronts[i].s + 31) / 32; ++J) {
+          tasks.push_back(i); tasks.push_back(J); tasks.push_back(kind);
+        }
+    dm.tri_cnt = ((int64_t)tasks.size() - dm.tri_off) / 3;
+    dm.wz_off = (int64_t)tasks.size();
+    for (int32_t i = 0; i < nf; ++i) {
+      const FrontInfo &F = sy.fronts[i];
+      for (int kind = 0; kind < 2; ++kind) {
+        const int32_t M = kind ? F.s : F.u, N = kind ? F.u : F.s;
+        for (int32_t r0 = 0; r0 < M; r0 += TL)
+          for (int32_t c0 = 0; c0 < N; c0 += TL) {
+            tasks.push_back(i); tasks.push_back(kind); tasks.push_back(r0); tasks.push_back(c0);
+          }
+      }
+    }
+    dm.wz_cnt = ((int64_t)tasks.size() - dm.wz_off) / 4;
   }
   dm.tasks.upload(tasks, s);
   launch_elem_integrals(dm, s);
   EMU_HIP(hipStreamSynchronize(s));
-  // the forward/backward solve kernels keep one front column in LDS
-  size_t lds = (size_t)dm.max_f * sizeof(zd);
-  if (lds > 160 * 1024) throw std::runtime_error("front order too large for the LDS-resident solve");
 }
 
 // ------------------------------------------------------------- workspace
@@ -332,7 +382,9 @@ void lu_solve_host(Ctx *c, DevMesh &dm, int nb, const double *mat1, int nrhs, co
   zd *pool = c->buf<zd>("pool", (size_t)nb * sy.pool);
   int32_t *piv = c->buf<int32_t>("piv", (size_t)nb * sy.npiv);
   int32_t *pfl = c->buf<int32_t>("pflag", (size_t)nb * sy.npiv);
-  lu_factor(dm, nb, m1, pool, piv, pfl, c->stream);
+  int32_t *sig = c->buf<int32_t>("sig", (size_t)nb * sy.npiv);
+  zd *inv = c->buf<zd>("inv", (size_t)nb * dm.isz);
+  lu_factor(dm, nb, m1, pool, piv, pfl, sig, inv, c->stream);
   zd *bo = c->buf<zd>("lu_b", (size_t)nb * nrhs * n);
   zd *bn = c->buf<zd>("lu_bn", (size_t)nb * nrhs * n);
   zd *yn = c->buf<zd>("lu_y", (size_t)nb * nrhs * n);
@@ -341,7 +393,7 @@ void lu_solve_host(Ctx *c, DevMesh &dm, int nb, const double *mat1, int nrhs, co
   EMU_HIP(hipMemcpyAsync(bo, b, sizeof(zd) * nb * nrhs * n, hipMemcpyHostToDevice, c->stream));
   EMU_HIP(hipStreamSynchronize(c->stream));
   launch_permute_vec(dm, nb, nrhs, bo, bn, true, c->stream);
-  lu_solve(dm, nb, nrhs, pool, piv, bn, yn, xn, w, c->stream);
+  lu_solve(dm, nb, nrhs, inv, sig, bn, yn, xn, w, c->stream);
   launch_permute_vec(dm, nb, nrhs, xn, bo, false, c->stream);
   EMU_HIP(hipMemcpyAsync(x, bo, sizeof(zd) * nb * nrhs * n, hipMemcpyDeviceToHost, c->stream));
   std::vector<int32_t> pf((size_t)nb * sy.npiv);
@@ -368,9 +420,10 @@ struct KSState {
 // Schur-decompose H[:m,:m], order the `keep` largest |theta| first and the
 // `nev` largest among them first of all, test the residuals
 // ||B Q y_i|| <= tol |theta_i| of the leading nev Ritz pairs (B = last bq rows
-// of H).  Optionally return W = Q[:, :nev] Y (Ritz vectors in the basis).
+// of H).  Optionally return W = Q[:, :nev] Y (Ritz vectors in the basis),
+// always or only when converged.
 static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol, std::vector<cplx> *ritzvec,
-                       double *maxres) {
+                       double *maxres, bool ritz_always = true) {
   const int ld = m + bq;
   st.T.assign((size_t)m * m, 0);
   for (int j = 0; j < m; ++j)
@@ -441,7 +494,7 @@ static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol
     if (!(rel <= tol)) conv = false;
   }
   if (maxres) *maxres = worst;
-  if (ritzvec) {
+  if (ritzvec && (conv || ritz_always)) {
     ritzvec->assign((size_t)m * nev, 0);
     for (int k = 0; k < nev; ++k)
       for (int j = 0; j <= k; ++j) {
@@ -455,6 +508,39 @@ static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol
 struct EigResult {
   std::vector<cplx> mu;  // nb x nev
   int64_t ops = 0, restarts = 0, steps = 0, rank_fixes = 0;
+};
+
+// Opt-in stage timer (environment EMU_TIMING=1): synchronises the stream at
+// stage boundaries and prints wall times to stderr.  Off by default because
+// the synchronisations serialise host and device.
+struct StageTimer {
+  bool on = false;
+  hipStream_t s = nullptr;
+  std::chrono::steady_clock::time_point t0;
+  std::vector<std::pair<std::string, double>> acc;
+  explicit StageTimer(hipStream_t st) : s(st) {
+    const char *e = getenv("EMU_TIMING");
+    on = e && e[0] == '1';
+    t0 = std::chrono::steady_clock::now();
+  }
+  void mark(const char *name) {
+    if (!on) return;
+    (void)hipStreamSynchronize(s);
+    auto t = std::chrono::steady_clock::now();
+    double dt = std::chrono::duration<double>(t - t0).count();
+    t0 = t;
+    for (auto &p : acc)
+      if (p.first == name) { p.second += dt; return; }
+    acc.push_back({name, dt});
+  }
+  ~StageTimer() {
+    if (!on) return;
+    double tot = 0;
+    for (auto &p : acc) tot += p.second;
+    fprintf(stderr, "[emu timing]");
+    for (auto &p : acc) fprintf(stderr, " %s=%.3fs", p.first.c_str(), p.second);
+    fprintf(stderr, " total=%.3fs\n", tot);
+  }
 };
 
 // orthonormalise V[:, j:j+bq] in place (Cholesky-QR2); R (bq x bq) of the
@@ -471,7 +557,7 @@ static void block_cholqr2(Ctx *c, int nb, int64_t n, int bq, zd *W, int64_t sV, 
 }
 
 static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int bq, int itermax, double tol,
-                              const zd *pool, const int32_t *piv, const zd *mvals, zd *Xout) {
+                              const zd *inv, const int32_t *sig, const zd *mvals, zd *Xout, StageTimer &tm) {
   hipStream_t s = c->stream;
   const Symbolic &sy = dm.host.sym;
   const int64_t n = dm.host.neq;
@@ -517,6 +603,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   if (keep < nev) throw std::invalid_argument("Krylov subspace too small for num_BMs");
   bool all_conv = false;
   std::vector<zd> Hh((size_t)nb * sH);
+  std::vector<std::vector<cplx>> ritz(nb);
   Profiler *pf = c->prof.on ? &c->prof : nullptr;
   const double nbd = nb, nd = (double)n, nzd = (double)dm.host.nnz;
   for (int it = 0; it <= itermax; ++it) {
@@ -527,7 +614,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
       int e0 = pf ? pf->begin(s) : -1;
       launch_spmm(nb, n, bq, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, t1, (int64_t)bq * n, s);
       if (pf) pf->end(s, e0, 4, nbd * (20.0 * nzd + 8.0 * (nd + 1) + 32.0 * nd * bq));
-      lu_solve(dm, nb, bq, pool, piv, t1, ty, wv, tw, s, pf);
+      lu_solve(dm, nb, bq, inv, sig, t1, ty, wv, tw, s, pf);
       launch_copy(nb, bq * n, wv, (int64_t)bq * n, Wn, sV, s);
       // block CGS, two passes
       for (int pass = 0; pass < 2; ++pass) {
@@ -546,6 +633,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
       res.ops += (int64_t)nb * bq;
       res.steps += 1;
     }
+    tm.mark("krylov_expand");
     EMU_HIP(hipMemcpyAsync(Hh.data(), Hd, sizeof(zd) * Hh.size(), hipMemcpyDeviceToHost, s));
     EMU_HIP(hipStreamSynchronize(s));
     if (c->prof.on) c->prof.collect();
@@ -559,7 +647,9 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
         try {
           KSState &S = st[b];
           for (int64_t i = 0; i < sH; ++i) S.H[i] = cplx(Hh[(size_t)b * sH + i].x, Hh[(size_t)b * sH + i].y);
-          convflag[b] = ks_project(S, m, bq, nev, keep, tol, nullptr, nullptr) ? 1 : 0;
+          // Ritz vectors of the basis are formed once the pairs pass the test
+          // (kept for the final combination) or on the last allowed restart
+          convflag[b] = ks_project(S, m, bq, nev, keep, tol, &ritz[b], nullptr, last) ? 1 : 0;
         } catch (std::exception &e) {
           errs[b] = e.what();
         }
@@ -571,6 +661,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
       if (!convflag[b]) all_conv = false;
     }
     res.restarts = it;
+    tm.mark("host_schur");
     if (all_conv || last) break;
     // restart: V2[:, :keep] = V[:, :m] Q[:, :keep]; V2[:, keep:keep+bq] = V[:, m:m+bq]
     std::vector<zd> Qh((size_t)nb * m * keep);
@@ -601,6 +692,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
     launch_copy(nb, (int64_t)bq * n, V + (int64_t)m * n, sV, V2 + (int64_t)keep * n, sV, s);
     std::swap(V, V2);
     k = keep;
+    tm.mark("restart");
   }
   if (!all_conv) throw NoConvergence("Krylov-Schur: eigenpairs did not converge within itermax restarts");
   std::vector<int32_t> badh(nb);
@@ -609,8 +701,8 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   // Ritz vectors X = V[:, :m] W, W = Q[:, :nev] Y
   std::vector<zd> Wh((size_t)nb * m * nev);
   for (int b = 0; b < nb; ++b) {
-    std::vector<cplx> W;
-    ks_project(st[b], m, bq, nev, keep, tol, &W, nullptr);
+    const std::vector<cplx> &W = ritz[b];
+    if (W.size() != (size_t)m * nev) throw std::logic_error("Ritz vectors missing after convergence");
     for (size_t i = 0; i < W.size(); ++i) Wh[(size_t)b * m * nev + i] = zmk(W[i].real(), W[i].imag());
     for (int i = 0; i < nev; ++i) res.mu[(size_t)b * nev + i] = st[b].theta[i];
   }
@@ -619,6 +711,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   EMU_HIP(hipStreamSynchronize(s));
   launch_gemm_nn(nb, n, nev, m, zmk(1, 0), V, n, sV, Wd, m, (int64_t)m * nev, zmk(0, 0), Xout, n, (int64_t)n * nev, s);
   EMU_HIP(hipStreamSynchronize(s));
+  tm.mark("ritz_vectors");
   return res;
 }
 
@@ -763,6 +856,8 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   upload_params(c, dm, nb, ntyp, reinterpret_cast<const double *>(pp.data()),
                 reinterpret_cast<const double *>(qq.data()), reinterpret_cast<const double *>(sh.data()), bl.data(),
                 &dpp, &dqq, &dsh, &dbl);
+  StageTimer tm(s);
+  tm.mark("setup");
   // 1. assembly
   zd *ve = c->buf<zd>("ve", (size_t)nb * nel * 14);
   zd *m1 = c->buf<zd>("mat1", (size_t)nb * nnz);
@@ -771,15 +866,19 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   launch_phase(dm, nb, dbl, ve, s);
   launch_gather(dm, nb, ntyp, dpp, dqq, dsh, ve, m1, m2, s);
   launch_permute_vals(dm, nb, m2, m2p, s);
+  tm.mark("assembly");
   // 2. LU of K - shift M
   const Symbolic &sy = mh.sym;
   zd *pool = c->buf<zd>("pool", (size_t)nb * sy.pool);
   int32_t *piv = c->buf<int32_t>("piv", (size_t)nb * sy.npiv);
   int32_t *pfl = c->buf<int32_t>("pflag", (size_t)nb * sy.npiv);
-  lu_factor(dm, nb, m1, pool, piv, pfl, s);
+  int32_t *sig = c->buf<int32_t>("sig", (size_t)nb * sy.npiv);
+  zd *inv = c->buf<zd>("inv", (size_t)nb * dm.isz);
+  lu_factor(dm, nb, m1, pool, piv, pfl, sig, inv, s);
+  tm.mark("lu_factor");
   // 3. eigensolve
   zd *Xn = c->buf<zd>("Xn", (size_t)nb * n * nval);
-  EigResult er = krylov_schur(c, dm, nb, nval, m, bq, (int)a.itermax, a.tol, pool, piv, m2p, Xn);
+  EigResult er = krylov_schur(c, dm, nb, nval, m, bq, (int)a.itermax, a.tol, inv, sig, m2p, Xn, tm);
   zd *Xo = c->buf<zd>("Xo", (size_t)nb * n * nval);
   launch_permute_vec(dm, nb, nval, Xn, Xo, false, s);
   // 4. beta, sort (z_indexx on -Re(beta^2))
@@ -846,6 +945,7 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   EMU_HIP(hipStreamSynchronize(s));
   launch_permute_cols(nb, n, nval, Xn, n * nval, dord, Xo, n * nval, s);
   launch_scale_cols(nb, n, nval, Xo, n, n * nval, zmax, nval, false, s);
+  tm.mark("normalise_orthogonality");
   // 7. plane-wave ordering (pw_ordering.f:48-78) and J / J^dagger
   const double d = mh.lat[0], dy = mh.lat[3];
   std::vector<std::pair<int, int>> pws;
@@ -916,6 +1016,7 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
         Jd[2 * (k + (size_t)q * nval) + 1] = Ra[q + (size_t)k * P2].y;
       }
   }
+  tm.mark("J_overlaps");
   if (a.stats) {
     a.stats[0] = er.ops;
     a.stats[1] = er.restarts;

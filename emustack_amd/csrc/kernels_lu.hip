@@ -11,9 +11,10 @@
 //   gemm   : trailing update F22 -= L21 U12 (64 x 64 tiles)
 //   extend : child Schur complement added into its parent (slot 0, then 1:
 //            no two tasks of one launch touch the same parent -> no atomics)
-// The solve uses the same tree: forward elimination bottom-up with
-// update vectors passed to the parent (deterministic, atomic-free), back
-// substitution top-down.
+// The factors are then rewritten into an inverted-factor form and the solves
+// walk the same tree as dense matrix-vector products: forward elimination
+// bottom-up with update vectors passed to the parent (deterministic,
+// atomic-free), back substitution top-down.
 #include "engine.h"
 
 namespace emu {
@@ -24,15 +25,16 @@ constexpr int EA_COLS = 8; // columns per extend-add task
 
 struct FrontArrays {
   const int32_t *s, *u, *f, *b, *parent;
-  const int64_t *off, *uoff, *pivoff;
-  const int32_t *uidx, *pmap;
+  const int64_t *off, *uoff, *pivoff, *ioff, *fvoff;
+  const int32_t *uidx, *pmap, *cmap;
 };
 
 static FrontArrays fa(DevMesh &dm) {
   FrontArrays a;
   a.s = dm.fs.p; a.u = dm.fu.p; a.f = dm.ff.p; a.b = dm.fb.p; a.parent = dm.fparent.p;
   a.off = dm.foff.p; a.uoff = dm.fuoff.p; a.pivoff = dm.fpivoff.p;
-  a.uidx = dm.uidx.p; a.pmap = dm.pmap.p;
+  a.ioff = dm.fioff.p; a.fvoff = dm.ffvoff.p;
+  a.uidx = dm.uidx.p; a.pmap = dm.pmap.p; a.cmap = dm.cmap.p;
   return a;
 }
 
@@ -61,7 +63,8 @@ __global__ void k_scatter_fronts(int64_t nnz, int64_t pool, const int64_t *__res
 __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ tasks, int k0,
                                                FrontArrays A, zd *__restrict__ pool, int64_t psz,
                                                int32_t *__restrict__ piv, int64_t npiv,
-                                               int32_t *__restrict__ pflag, double pert) {
+                                               int32_t *__restrict__ pflag, int32_t *__restrict__ sig,
+                                               double pert) {
   const int t = tasks[blockIdx.x];
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
@@ -69,7 +72,12 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ tasks
   zd *F = pool + b * psz + A.off[t];
   int32_t *pv = piv + b * npiv + A.pivoff[t];
   int32_t *pf = pflag + b * npiv + A.pivoff[t];
+  int32_t *sg = sig + b * npiv + A.pivoff[t];
   const int w = min(NB, s - k0);
+  if (k0 == 0) {
+    for (int i = tid; i < s; i += 256) sg[i] = i;
+    __syncthreads();
+  }
   __shared__ double redv[256];
   __shared__ int redi[256];
   __shared__ zd urow[NB];
@@ -109,6 +117,11 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ tasks
     if (tid == 0) {
       pv[k] = p;
       pf[k] = zero ? 1 : 0;
+      if (p != k) {
+        const int32_t a = sg[k];
+        sg[k] = sg[p];
+        sg[p] = a;
+      }
       if (zero) F[k + (int64_t)k * f] = d;
     }
     const zd dinv = zdiv(zmk(1.0, 0.0), d);
@@ -237,8 +250,8 @@ __global__ __launch_bounds__(256) void k_extend_add(const int32_t *__restrict__ 
   }
 }
 
-void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int32_t *pflag,
-               hipStream_t s) {
+void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int32_t *pflag, int32_t *sig,
+               zd *inv, hipStream_t s) {
   const Symbolic &sy = dm.host.sym;
   const int64_t psz = sy.pool;
   FrontArrays A = fa(dm);
@@ -254,7 +267,7 @@ void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int3
     for (const StepPlan &st : L.steps) {
       if (st.panel_cnt)
         hipLaunchKernelGGL(k_panel, dim3((unsigned)st.panel_cnt, nb), dim3(256), 0, s,
-                           dm.tasks.p + st.panel_off, st.k0, A, pool, psz, piv, sy.npiv, pflag, pert);
+                           dm.tasks.p + st.panel_off, st.k0, A, pool, psz, piv, sy.npiv, pflag, sig, pert);
       if (st.trsm_cnt)
         hipLaunchKernelGGL(k_trsm, dim3((unsigned)st.trsm_cnt, nb), dim3(64), 0, s,
                            dm.tasks.p + st.trsm_off, st.k0, A, pool, psz);
@@ -268,248 +281,372 @@ void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int3
                            dm.tasks.p + L.ea_off[sl], A, pool, psz);
     EMU_LAUNCH_CHECK();
   }
+  lu_invert(dm, nb, pool, inv, s);
 }
 
-// ---------------------------------------------------------------- solves
+// ------------------------------------------------- inverted-factor form
 //
-// One workgroup per (front, instance, chunk of RC right-hand sides).  The
-// front's RHS block r[f][RC] lives in LDS.  Triangular parts are processed in
-// SB-column blocks: the SB x SB diagonal block is solved by one wavefront per
-// RHS column (register shuffles, no barrier), then all 256 lanes apply the
-// block to the remaining rows (coalesced column reads of the factor).  Two
-// barriers per SB columns instead of one per column.
+// After the numeric factorisation every front is rewritten, out of place,
+// into the form the solves use:
+//   forward block  (f x s, ld f):  [ L11^{-1} ; -L21 L11^{-1} ]
+//   backward block (s x f, ld s):  [ U11^{-1} , -U11^{-1} U12 ]
+// so that one front's forward step is y = Fwd * (P r_S) (+ r_U on the update
+// rows) and its backward step is x_S = Bwd * [y_S ; x_U]: dense matrix-vector
+// products that split over row tiles across the whole GPU instead of a
+// column-sequential substitution inside one workgroup.  P is the front's
+// row-pivot permutation, applied as a gather (sig).
 
-constexpr int SB = 32;
+constexpr int TB = 32;  // triangular block of the inversion
 
 __device__ __forceinline__ zd shfl_z(zd v, int lane) {
   return zmk(__shfl(v.x, lane, 64), __shfl(v.y, lane, 64));
 }
 
+// One workgroup per (front, column block J, kind, instance): column block J
+// of L11^{-1} (kind 0, unit lower) or U11^{-1} (kind 1, upper) by blocked
+// substitution against identity columns.  Column blocks are independent.
+__global__ __launch_bounds__(256) void k_triinv(const int32_t *__restrict__ tasks, FrontArrays A,
+                                                const zd *__restrict__ pool, int64_t psz, zd *__restrict__ inv,
+                                                int64_t isz) {
+  const int t = tasks[3 * blockIdx.x], J = tasks[3 * blockIdx.x + 1], kind = tasks[3 * blockIdx.x + 2];
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int s = A.s[t], f = A.f[t];
+  const zd *F = pool + b * psz + A.off[t];
+  zd *O = inv + b * isz + A.ioff[t] + (kind ? (int64_t)f * s : 0);
+  const int64_t ldo = kind ? s : f;
+  const int nbk = (s + TB - 1) / TB;
+  const int j0 = J * TB, wj = min(TB, s - j0);
+  __shared__ zd Ts[TB][TB + 1];
+  __shared__ zd Xs[TB][TB + 1];
+  const int r = tid & 31, cg = tid >> 5;  // rows r, columns cg + 8m
+  // structural zeros of this column block
+  {
+    const int z0 = kind ? j0 + wj : 0, z1 = kind ? s : j0;
+    for (int idx = tid; idx < (z1 - z0) * wj; idx += 256) {
+      int i = z0 + idx % (z1 - z0), c = idx / (z1 - z0);
+      O[i + (int64_t)(j0 + c) * ldo] = zmk(0, 0);
+    }
+  }
+  const int Ib = J, Ie = kind ? -1 : nbk, dI = kind ? -1 : 1;
+  for (int I = Ib; I != Ie; I += dI) {
+    const int i0 = I * TB, wi = min(TB, s - i0);
+    zd acc[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int c = cg + 8 * m;
+      acc[m] = (I == J && r == c) ? zmk(1, 0) : zmk(0, 0);
+    }
+    const int Kb = kind ? I + 1 : J, Ke = kind ? J + 1 : I;
+    for (int K = Kb; K < Ke; ++K) {
+      const int k0 = K * TB, wk = min(TB, s - k0);
+      for (int idx = tid; idx < TB * TB; idx += 256) {
+        int i = idx % TB, kk = idx / TB;
+        Ts[i][kk] = (i < wi && kk < wk) ? F[(i0 + i) + (int64_t)(k0 + kk) * f] : zmk(0, 0);
+        Xs[i][kk] = (i < wk && kk < wj) ? O[(k0 + i) + (int64_t)(j0 + kk) * ldo] : zmk(0, 0);
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int kk = 0; kk < TB; ++kk) {
+        const zd a = Ts[r][kk];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[m] = zfms(a, Xs[kk][cg + 8 * m], acc[m]);
+      }
+      __syncthreads();
+    }
+    for (int idx = tid; idx < TB * TB; idx += 256) {
+      int i = idx % TB, j = idx / TB;
+      Ts[i][j] = (i < wi && j < wi) ? F[(i0 + i) + (int64_t)(i0 + j) * f] : zmk(0, 0);
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) Xs[r][cg + 8 * m] = acc[m];
+    __syncthreads();
+    for (int c = wave; c < wj; c += 4) {
+      zd v = lane < wi ? Xs[lane][c] : zmk(0, 0);
+      if (kind == 0) {
+        for (int k = 0; k < wi - 1; ++k) {
+          const zd xk = shfl_z(v, k);
+          if (lane > k && lane < wi) v = zfms(Ts[lane][k], xk, v);
+        }
+      } else {
+        for (int k = wi - 1; k >= 0; --k) {
+          const zd xk = zdiv(shfl_z(v, k), Ts[k][k]);
+          if (lane == k) v = xk;
+          else if (lane < k) v = zfms(Ts[lane][k], xk, v);
+        }
+      }
+      if (lane < wi) Xs[lane][c] = v;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < wi * wj; idx += 256) {
+      int i = idx % wi, c = idx / wi;
+      O[(i0 + i) + (int64_t)(j0 + c) * ldo] = Xs[i][c];
+    }
+    __syncthreads();
+  }
+}
+
+// Off-diagonal blocks of the inverted form (64 x 64 output tiles):
+//   kind 0:  Fwd[s+i, j] = -sum_{k>=j} L21[i,k] Linv[k,j]
+//   kind 1:  Bwd[i, s+c] = -sum_{k>=i} Uinv[i,k] U12[k,c]
+__global__ __launch_bounds__(256) void k_wz(const int32_t *__restrict__ tasks, FrontArrays A,
+                                            const zd *__restrict__ pool, int64_t psz, zd *__restrict__ inv,
+                                            int64_t isz) {
+  const int t = tasks[4 * blockIdx.x], kind = tasks[4 * blockIdx.x + 1];
+  const int r0 = tasks[4 * blockIdx.x + 2], c0 = tasks[4 * blockIdx.x + 3];
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int s = A.s[t], u = A.u[t], f = A.f[t];
+  const zd *F = pool + b * psz + A.off[t];
+  zd *Ofw = inv + b * isz + A.ioff[t];
+  zd *Obw = Ofw + (int64_t)f * s;
+  const int M = kind ? s : u, N = kind ? u : s;
+  const int kbeg = kind ? r0 : c0;
+  __shared__ zd As[NB][TILE];
+  __shared__ zd Bs[NB][TILE];
+  const int tx = tid % 16, ty = tid / 16;
+  zd acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = zmk(0, 0);
+  for (int k0 = kbeg; k0 < s; k0 += NB) {
+    for (int idx = tid; idx < NB * TILE; idx += 256) {
+      int i = idx % TILE, kk = idx / TILE;
+      int rr = r0 + i, k = k0 + kk;
+      zd a = zmk(0, 0);
+      if (rr < M && k < s) a = kind ? Obw[rr + (int64_t)k * s] : F[(s + rr) + (int64_t)k * f];
+      As[kk][i] = a;
+    }
+    for (int idx = tid; idx < NB * TILE; idx += 256) {
+      int kk = idx % NB, j = idx / NB;
+      int cc = c0 + j, k = k0 + kk;
+      zd v = zmk(0, 0);
+      if (cc < N && k < s) v = kind ? F[k + (int64_t)(s + cc) * f] : Ofw[k + (int64_t)cc * f];
+      Bs[kk][j] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < NB; ++kk) {
+      zd a[4], bb[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bb[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = zfma(a[i], bb[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int cc = c0 + tx + 16 * j;
+    if (cc >= N) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int rr = r0 + ty + 16 * i;
+      if (rr >= M) continue;
+      zd v = zmk(-acc[i][j].x, -acc[i][j].y);
+      if (kind) Obw[rr + (int64_t)(s + cc) * s] = v;
+      else Ofw[(s + rr) + (int64_t)cc * f] = v;
+    }
+  }
+}
+
+void lu_invert(DevMesh &dm, int nb, const zd *pool, zd *inv, hipStream_t s) {
+  const Symbolic &sy = dm.host.sym;
+  FrontArrays A = fa(dm);
+  if (dm.tri_cnt)
+    hipLaunchKernelGGL(k_triinv, dim3((unsigned)dm.tri_cnt, nb), dim3(256), 0, s, dm.tasks.p + dm.tri_off, A, pool,
+                       sy.pool, inv, dm.isz);
+  EMU_LAUNCH_CHECK();
+  if (dm.wz_cnt)
+    hipLaunchKernelGGL(k_wz, dim3((unsigned)dm.wz_cnt, nb), dim3(256), 0, s, dm.tasks.p + dm.wz_off, A, pool,
+                       sy.pool, inv, dm.isz);
+  EMU_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------------------- solves
+//
+// One workgroup per (front row tile of RT rows, instance, chunk of RC
+// right-hand sides).  The front's input vector is staged through LDS in
+// chunks of KC entries, gathered on the fly (forward: b at the pivot rows in
+// pivot order plus the children's update vectors; backward: y at the own
+// rows, x at the ancestors' rows); the four waves split each chunk and
+// reduce at the end.  Row tiles of one level never write what another
+// reads, so a level is one launch.
+
+constexpr int RT = 64;
+constexpr int KC = 64;
+
 template <int RC>
-__global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ fronts, FrontArrays A,
-                                             const int32_t *__restrict__ child0,
-                                             const int32_t *__restrict__ child1, const zd *__restrict__ pool,
-                                             int64_t psz, const int32_t *__restrict__ piv, int64_t npiv,
-                                             int nrhs, int64_t n, int64_t nw, const zd *__restrict__ bv,
-                                             zd *__restrict__ y, zd *__restrict__ wbuf) {
-  extern __shared__ zd r[];
-  const int t = fronts[blockIdx.x];
+__global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, FrontArrays A,
+                                             const zd *__restrict__ inv, int64_t isz, const int32_t *__restrict__ sig,
+                                             int64_t npiv, int nrhs, int64_t n, int64_t nw,
+                                             const zd *__restrict__ bv, zd *__restrict__ y, zd *__restrict__ wbuf) {
+  const int t = tasks[2 * blockIdx.x], r0 = tasks[2 * blockIdx.x + 1];
   const int b = blockIdx.y;
   const int q0 = blockIdx.z * RC;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int s = A.s[t], u = A.u[t], f = A.f[t], fb = A.b[t];
-  const zd *F = pool + b * psz + A.off[t];
-  const int32_t *pv = piv + b * npiv + A.pivoff[t];
+  const int tid = threadIdx.x, lr = tid & 63, kg = tid >> 6;
+  const int s = A.s[t], f = A.f[t], fb = A.b[t];
+  const zd *Fw = inv + b * isz + A.ioff[t];
+  const int32_t *sg = sig + b * npiv + A.pivoff[t];
+  const int32_t *cm = A.cmap + 2 * A.fvoff[t];
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
-  for (int idx = tid; idx < f * RC; idx += 256) {
-    int i = idx / RC, q = idx - i * RC;
-    r[idx] = (i < s && q < nq) ? bv[(col0 + q) * n + fb + i] : zmk(0, 0);
+  const int row = r0 + lr;
+  const int kend = min(s, r0 + RT);
+  __shared__ zd xin[KC][RC];
+  __shared__ zd red[3][RT][RC];
+  zd acc[RC];
+#pragma unroll
+  for (int q = 0; q < RC; ++q) acc[q] = zmk(0, 0);
+  for (int kc = 0; kc < kend; kc += KC) {
+    for (int idx = tid; idx < KC * RC; idx += 256) {
+      const int k = idx % KC, q = idx / KC;
+      const int kk = kc + k;
+      zd v = zmk(0, 0);
+      if (kk < kend && q < nq) {
+        const int j = sg[kk];
+        v = bv[(col0 + q) * n + fb + j];
+        const int c0i = cm[2 * j], c1i = cm[2 * j + 1];
+        if (c0i >= 0) v = zadd(v, wbuf[(col0 + q) * nw + c0i]);
+        if (c1i >= 0) v = zadd(v, wbuf[(col0 + q) * nw + c1i]);
+      }
+      xin[k][q] = v;
+    }
+    __syncthreads();
+    const int kl = min(KC, kend - kc);
+    if (row < f) {
+      const zd *col = Fw + row + (int64_t)kc * f;
+#pragma unroll 4
+      for (int k = kg; k < kl; k += 4) {
+        const zd a = col[(int64_t)k * f];
+#pragma unroll
+        for (int q = 0; q < RC; ++q) acc[q] = zfma(a, xin[k][q], acc[q]);
+      }
+    }
+    __syncthreads();
   }
+  if (kg > 0)
+#pragma unroll
+    for (int q = 0; q < RC; ++q) red[kg - 1][lr][q] = acc[q];
   __syncthreads();
-  for (int c = 0; c < 2; ++c) {
-    const int cf = c == 0 ? child0[t] : child1[t];
-    if (cf < 0) continue;
-    const int uc = A.u[cf];
-    const int32_t *mp = A.pmap + A.uoff[cf];
-    for (int idx = tid; idx < uc * RC; idx += 256) {
-      int k = idx / RC, q = idx - k * RC;
-      if (q < nq) {
-        zd *d = &r[mp[k] * RC + q];
-        *d = zadd(*d, wbuf[(col0 + q) * nw + A.uoff[cf] + k]);
+  if (kg == 0 && row < f) {
+    for (int q = 0; q < nq; ++q) {
+      zd v = zadd(zadd(acc[q], red[0][lr][q]), zadd(red[1][lr][q], red[2][lr][q]));
+      if (row < s) {
+        y[(col0 + q) * n + fb + row] = v;
+      } else {
+        const int c0i = cm[2 * row], c1i = cm[2 * row + 1];
+        if (c0i >= 0) v = zadd(v, wbuf[(col0 + q) * nw + c0i]);
+        if (c1i >= 0) v = zadd(v, wbuf[(col0 + q) * nw + c1i]);
+        wbuf[(col0 + q) * nw + A.uoff[t] + (row - s)] = v;
       }
     }
-    __syncthreads();
-  }
-  if (tid < nq)
-    for (int k = 0; k < s; ++k) {
-      int p = pv[k];
-      if (p != k) {
-        zd a = r[k * RC + tid];
-        r[k * RC + tid] = r[p * RC + tid];
-        r[p * RC + tid] = a;
-      }
-    }
-  __syncthreads();
-  __shared__ zd D[SB][SB + 1];
-  for (int kb = 0; kb < s; kb += SB) {
-    const int w = min(SB, s - kb);
-    for (int idx = tid; idx < w * w; idx += 256) {
-      int i = idx % w, j = idx / w;
-      D[i][j] = F[(kb + i) + (int64_t)(kb + j) * f];
-    }
-    __syncthreads();
-    for (int q = wave; q < nq; q += 4) {
-      zd v = lane < w ? r[(kb + lane) * RC + q] : zmk(0, 0);
-      for (int k = 0; k < w - 1; ++k) {
-        zd yk = shfl_z(v, k);
-        if (lane > k && lane < w) v = zfms(D[lane][k], yk, v);
-      }
-      if (lane < w) r[(kb + lane) * RC + q] = v;
-    }
-    __syncthreads();
-    for (int i = kb + w + tid; i < f; i += 256) {
-      zd acc[RC];
-#pragma unroll
-      for (int q = 0; q < RC; ++q) acc[q] = r[i * RC + q];
-      for (int kk = 0; kk < w; ++kk) {
-        const zd l = F[i + (int64_t)(kb + kk) * f];
-#pragma unroll
-        for (int q = 0; q < RC; ++q) acc[q] = zfms(l, r[(kb + kk) * RC + q], acc[q]);
-      }
-#pragma unroll
-      for (int q = 0; q < RC; ++q) r[i * RC + q] = acc[q];
-    }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < f * RC; idx += 256) {
-    int i = idx / RC, q = idx - i * RC;
-    if (q >= nq) continue;
-    if (i < s) y[(col0 + q) * n + fb + i] = r[idx];
-    else wbuf[(col0 + q) * nw + A.uoff[t] + (i - s)] = r[idx];
   }
 }
 
 template <int RC>
-__global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ fronts, FrontArrays A,
-                                             const zd *__restrict__ pool, int64_t psz, int nrhs, int64_t n,
+__global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, FrontArrays A,
+                                             const zd *__restrict__ inv, int64_t isz, int nrhs, int64_t n,
                                              const zd *__restrict__ y, zd *__restrict__ x) {
-  extern __shared__ zd r[];
-  const int t = fronts[blockIdx.x];
+  const int t = tasks[2 * blockIdx.x], r0 = tasks[2 * blockIdx.x + 1];
   const int b = blockIdx.y;
   const int q0 = blockIdx.z * RC;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int s = A.s[t], u = A.u[t], f = A.f[t], fb = A.b[t];
-  const zd *F = pool + b * psz + A.off[t];
+  const int tid = threadIdx.x, lr = tid & 63, kg = tid >> 6;
+  const int s = A.s[t], f = A.f[t], fb = A.b[t];
+  const zd *Bw = inv + b * isz + A.ioff[t] + (int64_t)f * s;
   const int32_t *ui = A.uidx + A.uoff[t];
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
-  for (int idx = tid; idx < f * RC; idx += 256) {
-    int i = idx / RC, q = idx - i * RC;
-    zd v = zmk(0, 0);
-    if (q < nq) v = i < s ? y[(col0 + q) * n + fb + i] : x[(col0 + q) * n + ui[i - s]];
-    r[idx] = v;
-  }
-  __syncthreads();
-  for (int i = tid; i < s; i += 256) {
-    zd acc[RC];
+  const int row = r0 + lr;
+  __shared__ zd xin[KC][RC];
+  __shared__ zd red[3][RT][RC];
+  zd acc[RC];
 #pragma unroll
-    for (int q = 0; q < RC; ++q) acc[q] = r[i * RC + q];
-    for (int c = 0; c < u; ++c) {
-      const zd uv = F[i + (int64_t)(s + c) * f];
-#pragma unroll
-      for (int q = 0; q < RC; ++q) acc[q] = zfms(uv, r[(s + c) * RC + q], acc[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < RC; ++q) r[i * RC + q] = acc[q];
-  }
-  __syncthreads();
-  __shared__ zd D[SB][SB + 1];
-  for (int bi = (s + SB - 1) / SB - 1; bi >= 0; --bi) {
-    const int kb = bi * SB, w = min(SB, s - kb);
-    for (int idx = tid; idx < w * w; idx += 256) {
-      int i = idx % w, j = idx / w;
-      D[i][j] = F[(kb + i) + (int64_t)(kb + j) * f];
+  for (int q = 0; q < RC; ++q) acc[q] = zmk(0, 0);
+  for (int kc = r0; kc < f; kc += KC) {
+    for (int idx = tid; idx < KC * RC; idx += 256) {
+      const int k = idx % KC, q = idx / KC;
+      const int kk = kc + k;
+      zd v = zmk(0, 0);
+      if (kk < f && q < nq) v = kk < s ? y[(col0 + q) * n + fb + kk] : x[(col0 + q) * n + ui[kk - s]];
+      xin[k][q] = v;
     }
     __syncthreads();
-    for (int q = wave; q < nq; q += 4) {
-      zd v = lane < w ? r[(kb + lane) * RC + q] : zmk(0, 0);
-      for (int k = w - 1; k >= 0; --k) {
-        const zd xk = zdiv(shfl_z(v, k), D[k][k]);
-        if (lane == k) v = xk;
-        else if (lane < k) v = zfms(D[lane][k], xk, v);
+    const int kl = min(KC, f - kc);
+    if (row < s) {
+      const zd *col = Bw + row + (int64_t)kc * s;
+#pragma unroll 4
+      for (int k = kg; k < kl; k += 4) {
+        const zd a = col[(int64_t)k * s];
+#pragma unroll
+        for (int q = 0; q < RC; ++q) acc[q] = zfma(a, xin[k][q], acc[q]);
       }
-      if (lane < w) r[(kb + lane) * RC + q] = v;
-    }
-    __syncthreads();
-    for (int i = tid; i < kb; i += 256) {
-      zd acc[RC];
-#pragma unroll
-      for (int q = 0; q < RC; ++q) acc[q] = r[i * RC + q];
-      for (int kk = 0; kk < w; ++kk) {
-        const zd uv = F[i + (int64_t)(kb + kk) * f];
-#pragma unroll
-        for (int q = 0; q < RC; ++q) acc[q] = zfms(uv, r[(kb + kk) * RC + q], acc[q]);
-      }
-#pragma unroll
-      for (int q = 0; q < RC; ++q) r[i * RC + q] = acc[q];
     }
     __syncthreads();
   }
-  for (int idx = tid; idx < s * RC; idx += 256) {
-    int i = idx / RC, q = idx - i * RC;
-    if (q < nq) x[(col0 + q) * n + fb + i] = r[idx];
-  }
+  if (kg > 0)
+#pragma unroll
+    for (int q = 0; q < RC; ++q) red[kg - 1][lr][q] = acc[q];
+  __syncthreads();
+  if (kg == 0 && row < s)
+    for (int q = 0; q < nq; ++q)
+      x[(col0 + q) * n + fb + row] = zadd(zadd(acc[q], red[0][lr][q]), zadd(red[1][lr][q], red[2][lr][q]));
 }
 
-static const size_t kMaxLds = 160 * 1024 - 16 * 1024;  // dynamic part (static D block + margin)
-
-template <int RC>
-static void set_attr_once() {
-  static bool attr = false;
-  if (!attr) {
-    EMU_HIP(hipFuncSetAttribute((const void *)k_fwd<RC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds));
-    EMU_HIP(hipFuncSetAttribute((const void *)k_bwd<RC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds));
-    attr = true;
-  }
-}
-
-// largest RHS chunk (8, 4, 2, 1) not above nrhs whose LDS block fits
-static int pick_rc(int nrhs, int max_f) {
-  const size_t per = (size_t)max_f * sizeof(zd);
-  for (int rc : {8, 4, 2}) if (nrhs >= rc && rc * per <= kMaxLds) return rc;
+static int pick_rc(int nrhs) {
+  for (int rc : {8, 4, 2})
+    if (nrhs >= rc) return rc;
   return 1;
 }
 
 template <int RC>
-static void launch_fwd(const LevelSched &L, DevMesh &dm, FrontArrays A, const int32_t *c0, const int32_t *c1, int nb,
-                       int nrhs, const zd *pool, const int32_t *piv, const zd *bnew, zd *y, zd *w, hipStream_t s) {
-  set_attr_once<RC>();
+static void launch_fwd(const LevelSched &L, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *inv,
+                       const int32_t *sig, const zd *bnew, zd *y, zd *w, hipStream_t s) {
   const Symbolic &sy = dm.host.sym;
   const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
-  size_t lds = (size_t)L.max_f * RC * sizeof(zd);
-  hipLaunchKernelGGL(k_fwd<RC>, dim3((unsigned)L.fronts_cnt, nb, nz), dim3(256), lds, s, dm.tasks.p + L.fronts_off,
-                     A, c0, c1, pool, sy.pool, piv, sy.npiv, nrhs, sy.n, sy.nw, bnew, y, w);
+  hipLaunchKernelGGL(k_fwd<RC>, dim3((unsigned)L.sf_cnt, nb, nz), dim3(256), 0, s, dm.tasks.p + L.sf_off, A, inv,
+                     dm.isz, sig, sy.npiv, nrhs, sy.n, std::max<int64_t>(sy.nw, 1), bnew, y, w);
 }
 
 template <int RC>
-static void launch_bwd(const LevelSched &L, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *pool,
+static void launch_bwd(const LevelSched &L, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *inv,
                        const zd *y, zd *x, hipStream_t s) {
-  set_attr_once<RC>();
   const Symbolic &sy = dm.host.sym;
   const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
-  size_t lds = (size_t)L.max_f * RC * sizeof(zd);
-  hipLaunchKernelGGL(k_bwd<RC>, dim3((unsigned)L.fronts_cnt, nb, nz), dim3(256), lds, s, dm.tasks.p + L.fronts_off,
-                     A, pool, sy.pool, nrhs, sy.n, y, x);
+  hipLaunchKernelGGL(k_bwd<RC>, dim3((unsigned)L.sb_cnt, nb, nz), dim3(256), 0, s, dm.tasks.p + L.sb_off, A, inv,
+                     dm.isz, nrhs, sy.n, y, x);
 }
 
-void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *pool, const int32_t *piv, const zd *bnew,
-              zd *y, zd *x, zd *w, hipStream_t s, Profiler *prof) {
-  if ((size_t)dm.max_f * sizeof(zd) > kMaxLds) throw HipError("front too large for the LDS-resident solve");
+void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *sig, const zd *bnew, zd *y, zd *x,
+              zd *w, hipStream_t s, Profiler *prof) {
   FrontArrays A = fa(dm);
-  const int32_t *c0 = dm.tasks.p + dm.child_off[0];
-  const int32_t *c1 = dm.tasks.p + dm.child_off[1];
-  const double scale = (double)nb * nrhs;
+  const int rc = pick_rc(nrhs);
+  const double chunks = (double)nb * ((nrhs + rc - 1) / rc), vecs = (double)nb * nrhs;
   for (const LevelSched &L : dm.sched) {
     int ev = prof ? prof->begin(s) : -1;
-    switch (pick_rc(nrhs, L.max_f)) {
-      case 8: launch_fwd<8>(L, dm, A, c0, c1, nb, nrhs, pool, piv, bnew, y, w, s); break;
-      case 4: launch_fwd<4>(L, dm, A, c0, c1, nb, nrhs, pool, piv, bnew, y, w, s); break;
-      case 2: launch_fwd<2>(L, dm, A, c0, c1, nb, nrhs, pool, piv, bnew, y, w, s); break;
-      default: launch_fwd<1>(L, dm, A, c0, c1, nb, nrhs, pool, piv, bnew, y, w, s); break;
+    switch (rc) {
+      case 8: launch_fwd<8>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, s); break;
+      case 4: launch_fwd<4>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, s); break;
+      case 2: launch_fwd<2>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, s); break;
+      default: launch_fwd<1>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, s); break;
     }
-    if (prof) prof->end(s, ev, 0, L.fwd_bytes * scale);
+    if (prof) prof->end(s, ev, 0, L.fwd_mat * chunks + L.fwd_vec * vecs);
   }
   EMU_LAUNCH_CHECK();
   for (auto it = dm.sched.rbegin(); it != dm.sched.rend(); ++it) {
     int ev = prof ? prof->begin(s) : -1;
-    switch (pick_rc(nrhs, it->max_f)) {
-      case 8: launch_bwd<8>(*it, dm, A, nb, nrhs, pool, y, x, s); break;
-      case 4: launch_bwd<4>(*it, dm, A, nb, nrhs, pool, y, x, s); break;
-      case 2: launch_bwd<2>(*it, dm, A, nb, nrhs, pool, y, x, s); break;
-      default: launch_bwd<1>(*it, dm, A, nb, nrhs, pool, y, x, s); break;
+    switch (rc) {
+      case 8: launch_bwd<8>(*it, dm, A, nb, nrhs, inv, y, x, s); break;
+      case 4: launch_bwd<4>(*it, dm, A, nb, nrhs, inv, y, x, s); break;
+      case 2: launch_bwd<2>(*it, dm, A, nb, nrhs, inv, y, x, s); break;
+      default: launch_bwd<1>(*it, dm, A, nb, nrhs, inv, y, x, s); break;
     }
-    if (prof) prof->end(s, ev, 1, it->bwd_bytes * scale);
+    if (prof) prof->end(s, ev, 1, it->bwd_mat * chunks + it->bwd_vec * vecs);
   }
   EMU_LAUNCH_CHECK();
 }

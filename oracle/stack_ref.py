@@ -27,9 +27,17 @@ def grating_orders_2d(max_order, d=1.0, dy=1.0):
     return pxs_mesh[low], pys_mesh[low]
 
 
-def air_sort(k_pll_norm, wl_norm, max_order, d_ratio=1.0):
+def grating_orders(max_order, d_ratio=1.0, world_1d=False):
+    """1D orders (mode_calcs.py:55-59, no y orders) or the 2D set."""
+    if world_1d:
+        pxs = np.arange(-max_order, max_order + 1)
+        return pxs, np.zeros_like(pxs)
+    return grating_orders_2d(max_order, 1.0, d_ratio)
+
+
+def air_sort(k_pll_norm, wl_norm, max_order, d_ratio=1.0, world_1d=False):
     """Sort order of the air reference (mode_calcs.py:164-170)."""
-    pxs, pys = grating_orders_2d(max_order, 1.0, d_ratio)
+    pxs, pys = grating_orders(max_order, d_ratio, world_1d)
     a0, b0 = k_pll_norm
     al = a0 + pxs * 2 * np.pi
     be = b0 + pys * 2 * np.pi / d_ratio
@@ -44,9 +52,9 @@ def air_sort(k_pll_norm, wl_norm, max_order, d_ratio=1.0):
     return s, pxs, pys
 
 
-def film_kz(n, k_pll_norm, wl_norm, max_order, sort_order, d_ratio=1.0):
+def film_kz(n, k_pll_norm, wl_norm, max_order, sort_order, d_ratio=1.0, world_1d=False):
     """Anallo.calc_kz (mode_calcs.py:116-189) with the air-reference sort."""
-    pxs, pys = grating_orders_2d(max_order, 1.0, d_ratio)
+    pxs, pys = grating_orders(max_order, d_ratio, world_1d)
     a0, b0 = k_pll_norm
     al = a0 + pxs * 2 * np.pi
     be = b0 + pys * 2 * np.pi / d_ratio
@@ -92,18 +100,19 @@ def r_t_tf_ns(Z1, J, Jd):
     return R12, T12, R21, T21
 
 
-def calc_scat(layers, wl_norm, k_pll_norm, max_order, pol='TE', d_ratio=1.0):
+def calc_scat(layers, wl_norm, k_pll_norm, max_order, pol='TE', d_ratio=1.0, world_1d=False):
     """Stack.calc_scat (stack.py:162-427) for shears=None, calc_fluxes=True.
+    world_1d: thin films expand in the 1D grating orders (1D stacks).
     Returns dict with R_net, T_net, a_list, r_list, t_list."""
-    s, _, _ = air_sort(k_pll_norm, wl_norm, max_order, d_ratio)
-    kz_air, spec_air, nprop_air, k_air = film_kz(1.0, k_pll_norm, wl_norm, max_order, s, d_ratio)
+    s, _, _ = air_sort(k_pll_norm, wl_norm, max_order, d_ratio, world_1d)
+    kz_air, spec_air, nprop_air, k_air = film_kz(1.0, k_pll_norm, wl_norm, max_order, s, d_ratio, world_1d)
     Zair = film_Z(1.0, kz_air, k_air)
     r12, r21, t12, t21, kzs = [], [], [], [], []
     nprop = []
     spec = []
     for lay in layers:
         if lay['kind'] == 'tf':
-            kz, sp_, npw, k = film_kz(lay['n'], k_pll_norm, wl_norm, max_order, s, d_ratio)
+            kz, sp_, npw, k = film_kz(lay['n'], k_pll_norm, wl_norm, max_order, s, d_ratio, world_1d)
             R12, T12, R21, T21 = r_t_anallo(Zair, film_Z(lay['n'], kz, k))
             kzs.append(np.append(kz, kz))
             nprop.append(npw)

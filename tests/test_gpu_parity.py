@@ -1,9 +1,9 @@
 """GPU parity tests: every stage of the HIP path against the oracle on the
 same inputs, and the full drop-in API against the reference's golden R/T/A.
 Tolerances: assembly 1e-13 relative (same FP64 arithmetic, different
-summation order); LU residual 1e-8; beta 1e-9 relative (north star);
-R/T/A 1e-6 against the reference files (their own tolerance) and 1e-8
-against the oracle."""
+summation order); LU residual 1e-8; beta 1e-9 relative; R/T/A 1e-6 against
+the reference files (their own tolerance) and 1e-7 against the oracle (two
+FP64 eigensolvers agree to the conditioning of clustered modes)."""
 import os
 
 import numpy as np

@@ -40,9 +40,12 @@ def test_mesh_preprocessing_matches_oracle(path):
     np.testing.assert_array_equal(te, m.type_el)
 
 
-def _walk_multifrontal(mh, A, b):
+def _walk_multifrontal(mh, A, b, inverted=False):
     """Factor and solve A x = b in numpy with the library's symbolic data
-    (fronts, update index sets, extend-add maps, entry scatter map)."""
+    (fronts, update index sets, extend-add maps, entry scatter map).  With
+    inverted=True the solves follow the device scheme: each front rewritten
+    as Fwd = [L11^-1; -L21 L11^-1], Bwd = [U11^-1, -U11^-1 U12], the pivot
+    order applied as a gather, children's update vectors pulled per row."""
     S = mh.symbolic()
     fr = S['fronts']
     arr = mh.arrays()
@@ -74,6 +77,8 @@ def _walk_multifrontal(mh, A, b):
                 P[np.ix_(mp, mp)] += F[s:, s:]
                 pool[S['foff'][par]:S['foff'][par] + fp * fp] = P.reshape(-1, order='F')
     bn = b[perm]
+    if inverted:
+        return _solve_inverted(mh, S, fr, front, pivs, bn, perm)
     y = np.zeros(mh.neq, complex)
     wv = {}
     for lvl in range(fr[:, 5].max(), -1, -1):
@@ -105,6 +110,59 @@ def _walk_multifrontal(mh, A, b):
     return xo
 
 
+def _solve_inverted(mh, S, fr, front, pivs, bn, perm):
+    nf = len(fr)
+    fvoff = np.concatenate([[0], np.cumsum(fr[:, 2])])
+    cmap = -np.ones((fvoff[-1], 2), np.int64)
+    for t in range(nf):
+        s, u, f, b0, par, _, c0, c1 = fr[t]
+        for slot, c in enumerate((c0, c1)):
+            if c >= 0:
+                uc = fr[c][1]
+                mp = S['pmap'][S['uoff'][c]:S['uoff'][c] + uc]
+                cmap[fvoff[t] + mp, slot] = S['uoff'][c] + np.arange(uc)
+    w = np.zeros(max(mh.nw, 1), complex)
+    y = np.zeros(mh.neq, complex)
+    eye = np.eye
+    fwd, bwd, sig = {}, {}, {}
+    for t in range(nf):
+        s, u, f = fr[t][:3]
+        F = front(t)
+        L11 = np.tril(F[:s, :s], -1) + eye(s)
+        Li = sla.solve_triangular(L11, eye(s), lower=True, unit_diagonal=True)
+        Ui = sla.solve_triangular(np.triu(F[:s, :s]), eye(s), lower=False)
+        fwd[t] = np.vstack([Li, -F[s:, :s] @ Li])
+        bwd[t] = np.hstack([Ui, -Ui @ F[:s, s:]])
+        g = np.arange(s)
+        for k in range(s):
+            p = pivs[t][k]
+            g[[k, p]] = g[[p, k]]
+        sig[t] = g
+
+    def pulled(t, rows):
+        v = np.zeros(len(rows), complex)
+        for slot in range(2):
+            ci = cmap[fvoff[t] + rows, slot]
+            v[ci >= 0] += w[ci[ci >= 0]]
+        return v
+    for lvl in range(fr[:, 5].max(), -1, -1):
+        for t in np.nonzero(fr[:, 5] == lvl)[0]:
+            s, u, f, b0 = fr[t][:4]
+            xin = bn[b0 + sig[t]] + pulled(t, sig[t])
+            out = fwd[t] @ xin
+            y[b0:b0 + s] = out[:s]
+            w[S['uoff'][t]:S['uoff'][t] + u] = out[s:] + pulled(t, np.arange(s, f))
+    x = np.zeros(mh.neq, complex)
+    for lvl in range(0, fr[:, 5].max() + 1):
+        for t in np.nonzero(fr[:, 5] == lvl)[0]:
+            s, u, f, b0 = fr[t][:4]
+            ui = S['uidx'][S['uoff'][t]:S['uoff'][t] + u]
+            x[b0:b0 + s] = bwd[t] @ np.concatenate([y[b0:b0 + s], x[ui]])
+    xo = np.zeros_like(x)
+    xo[perm] = x
+    return xo
+
+
 def test_symbolic_multifrontal_solves_pencil():
     mh = _lib.MeshHandle(MESH_600)
     m = fem2d.build_mesh(MESH_600)
@@ -118,6 +176,8 @@ def test_symbolic_multifrontal_solves_pencil():
     b = rng.standard_normal(m.neq) + 1j * rng.standard_normal(m.neq)
     x = _walk_multifrontal(mh, A, b)
     assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-9
+    xi = _walk_multifrontal(mh, A, b, inverted=True)
+    assert np.linalg.norm(A @ xi - b) / np.linalg.norm(b) < 1e-9
     # every front variable is owned exactly once, update sets are above the front
     fr = mh.symbolic()['fronts']
     assert fr[:, 0].sum() == mh.neq

@@ -1,0 +1,203 @@
+"""Pins of the 1D-grating oracle (oracle/fem1d.py) against what the
+mathematics and independent methods fix:
+  * element integrals against the rational tables of the reference's
+    matrix_{mxx,kxy,kyy,myy}_1d.f (golden constants);
+  * exponential integrals against their alpha -> 0 limits and adaptive
+    quadrature;
+  * propagation constants against the exact lamellar-grating (Rytov)
+    dispersion relations for TE and TM;
+  * the near-homogeneous grating against the Airy thin-film result;
+  * the stacked-grating structure of the reference's case 4 against an
+    independent TE rigorous coupled-wave (Fourier modal) calculation, and
+    energy conservation.
+The case-4 golden R/T files are NOT reproduced (2e-2 apart, while the
+restatement and the independent Fourier-modal method agree to 5e-4); that
+comparison is kept as an expected failure documenting the gap."""
+import numpy as np
+import pytest
+import scipy.integrate as si
+
+from conftest import load_txt
+from oracle import fem1d, stack_ref
+
+WL, PERIOD, ORDER = 800.0, 760.0, 6
+
+
+def _kp(wl=WL, period=PERIOD):
+    th = ph = 1e-5 * np.pi / 180
+    k = 2 * np.pi / wl
+    return k * np.sin(th) * np.array([np.cos(ph), np.sin(ph)]) * period
+
+
+def _modes(diam, neff, lc=0.005, nval=46, wl=WL, period=PERIOD, order=ORDER):
+    x, tn, te = fem1d.grating_mesh_1d(period, diam, lc)
+    m = fem1d.build_mesh_1d(x, tn, te)
+    neff = np.asarray(neff, complex)
+    wln = wl / period
+    k0 = 2 * np.pi / wln
+    kp = _kp(wl, period)
+    shift = 1.1 * np.real(neff).max() ** 2 * k0 ** 2 - kp @ kp
+    b, J, Jd = fem1d.calc_modes_1d(wln, nval, order, m, neff, kp[0], kp[1], shift)
+    return dict(kind='ns', kz=b, J=J, Jd=Jd), te
+
+
+def test_element_integrals_match_reference_tables():
+    h = 0.0125
+    Mxx, Kxy, Kyx, Kyy, Myy = fem1d.element_mats_1d(h)
+    mxx = np.array([[4, -1, 2], [-1, 4, 2], [2, 2, 16]])
+    kxy = np.array([[-83, -7, 99, -9], [7, 83, 9, -99], [-44, 44, -108, 108]])
+    kyy = np.array([[148, -13, -189, 54], [-13, 148, 54, -189], [-189, 54, 432, -297], [54, -189, -297, 432]])
+    myy = np.array([[128, 19, 99, -36], [19, 128, -36, 99], [99, -36, 648, -81], [-36, 99, -81, 648]])
+    np.testing.assert_allclose(Mxx, mxx * h / 30, rtol=1e-13, atol=1e-16)
+    np.testing.assert_allclose(Kxy, kxy / 120, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(Kyx, kxy.T / 120, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(Kyy, kyy / (40 * h), rtol=1e-13)
+    np.testing.assert_allclose(Myy, myy * h / 1680, rtol=1e-13, atol=1e-17)
+
+
+@pytest.mark.parametrize('alpha', [1e-9, 7.3, -25.0, 180.0])
+def test_exponential_integrals(alpha):
+    xmin, xmax = 0.31, 0.41
+    p2, p3 = fem1d.exp_integrals_1d(xmin, xmax, alpha)
+    h = xmax - xmin
+    if abs(alpha) < 1e-6:
+        np.testing.assert_allclose(p2, np.array([1, 1, 4]) * h / 6, rtol=1e-9)
+        np.testing.assert_allclose(p3, np.array([1, 1, 3, 3]) * h / 8, rtol=1e-9)
+        return
+    t = lambda x: (x - xmin) / h
+    for vals, nodes in ((p2, fem1d._P2_NODES), (p3, fem1d._P3_NODES)):
+        for i, v in enumerate(vals):
+            f = lambda x, part: getattr(fem1d.lagrange(nodes, t(x))[0][i, 0] * np.exp(1j * alpha * x), part)
+            re = si.quad(f, xmin, xmax, args=('real',), epsabs=1e-15, epsrel=1e-13, limit=200)[0]
+            im = si.quad(f, xmin, xmax, args=('imag',), epsabs=1e-15, epsrel=1e-13, limit=200)[0]
+            assert abs(v - (re + 1j * im)) <= 1e-12 * max(h, abs(v))
+
+
+def test_dof_numbering_is_periodic():
+    x, tn, te = fem1d.grating_mesh_1d(PERIOD, [190], 0.01)
+    m = fem1d.build_mesh_1d(x, tn, te)
+    assert m.neq == 9 * m.nel
+    assert sorted(set(m.eq.ravel())) == list(range(m.neq))
+    # continuous P3 components: right end of element e is the left end of e+1
+    np.testing.assert_array_equal(m.eq[:-1, 4], m.eq[1:, 3])
+    np.testing.assert_array_equal(m.eq[:-1, 8], m.eq[1:, 7])
+    assert m.eq[-1, 4] == m.eq[0, 3] and m.eq[-1, 8] == m.eq[0, 7]
+    assert abs(m.period_x - 1.0) < 1e-14
+    # 0.25-wide centred inclusion: elements 37..61 of 100 (x in [0.37, 0.62])
+    np.testing.assert_array_equal(np.nonzero(te == 2)[0], np.arange(37, 62))
+
+
+@pytest.fixture(scope='module')
+def grating2():
+    return _modes([190], [3.61, 1.46])
+
+
+def test_beta_satisfies_rytov_dispersion(grating2):
+    """TE (E_y) and TM (E_x, E_z) modes of a lamellar grating satisfy
+    cos(k1 a) cos(k2 b) - (r + 1/r)/2 sin(k1 a) sin(k2 b) = cos(kx) with
+    r = k1/k2 (TE) or eps2 k1 / (eps1 k2) (TM)."""
+    lay, te = grating2
+    k0 = 2 * np.pi / (WL / PERIOD)
+    kx = _kp()[0]
+    a = (te == 2).sum() * 0.005
+    n1, n2 = 1.46, 3.61
+
+    def f(beta, tm):
+        k1 = np.sqrt(complex(n1 ** 2 * k0 ** 2 - beta ** 2))
+        k2 = np.sqrt(complex(n2 ** 2 * k0 ** 2 - beta ** 2))
+        r = n2 ** 2 * k1 / (n1 ** 2 * k2) if tm else k1 / k2
+        return (np.cos(k1 * a) * np.cos(k2 * (1 - a)) - 0.5 * (r + 1 / r) * np.sin(k1 * a) * np.sin(k2 * (1 - a))
+                - np.cos(kx))
+    good = 0
+    for b in lay['kz'][:10]:
+        assert min(abs(f(b, False)), abs(f(b, True))) < 1e-7
+        good += 1
+    assert good == 10
+
+
+def test_near_homogeneous_grating_matches_airy():
+    lay, _ = _modes([190], [2.0, 2.0 + 1e-7], lc=0.01, nval=30)
+    lay['h'] = 300 / PERIOD
+    air = dict(kind='tf', n=1.0, h=None)
+    film = dict(kind='tf', n=2.0, h=300 / PERIOD)
+    for pol in ('TE', 'TM'):
+        o = stack_ref.calc_scat([air, lay, air], WL / PERIOD, _kp(), ORDER, pol=pol, world_1d=True)
+        f = stack_ref.calc_scat([air, film, air], WL / PERIOD, _kp(), ORDER, pol=pol, world_1d=True)
+        assert abs(o['r_list'][-1] - f['r_list'][-1]) < 5e-5
+
+
+def _rcwa_te(profiles, N, wl=WL, period=PERIOD):
+    """Independent TE Fourier-modal (RCWA) reflectance at normal incidence of
+    lamellar layers listed top to bottom as (eps per uniform cell, height_nm)
+    between air half-spaces."""
+    p = np.arange(-N, N + 1)
+    k0 = 2 * np.pi / wl * period
+    kx = 2 * np.pi * p
+    m = np.arange(-2 * N, 2 * N + 1)
+
+    def conv(eps):
+        nel = len(eps)
+        c = np.zeros(len(m), complex)
+        for j, ev in enumerate(eps):
+            lo, hi = j / nel, (j + 1) / nel
+            mm = np.where(m == 0, 1, m)
+            c += ev * np.where(m == 0, hi - lo,
+                               (np.exp(-2j * np.pi * m * lo) - np.exp(-2j * np.pi * m * hi)) / (2j * np.pi * mm))
+        return c[p[:, None] - p[None, :] + 2 * N]
+
+    def modes(eps):
+        w, W = np.linalg.eig(np.diag(kx ** 2).astype(complex) - k0 ** 2 * conv(eps))
+        q = np.sqrt(w.astype(complex))
+        return W, np.where(q.real < 0, -q, q)
+    kz0 = np.sqrt((k0 ** 2 - kx ** 2).astype(complex))
+    kz0 = np.where(kz0.imag < 0, -kz0, kz0)
+    air = (np.eye(len(p)), -1j * kz0)
+    n = len(p)
+    I = np.eye(n)
+
+    def interface(A, B):
+        (Wa, Qa), (Wb, Qb) = A, B
+        Va, Vb = Wa * Qa[None, :], Wb * Qb[None, :]
+        S = np.linalg.solve(np.block([[Wb, -Wa], [-Vb, -Va]]), np.block([[Wa, -Wb], [-Va, -Vb]]))
+        return dict(T=S[:n, :n], Rb=S[:n, n:], R=S[n:, :n], Tb=S[n:, n:])
+
+    def star(A, B):
+        X = np.linalg.inv(I - B['R'] @ A['Rb'])
+        Y = np.linalg.inv(I - A['Rb'] @ B['R'])
+        return dict(T=B['T'] @ Y @ A['T'], Rb=B['Rb'] + B['T'] @ Y @ A['Rb'] @ B['Tb'],
+                    R=A['R'] + A['Tb'] @ X @ B['R'] @ A['T'], Tb=A['Tb'] @ X @ B['Tb'])
+    regs = [(air, 0.0)] + [(modes(e), h / period) for e, h in profiles] + [(air, 0.0)]
+    S = None
+    for i in range(len(regs) - 1):
+        Si = interface(regs[i][0], regs[i + 1][0])
+        S = Si if S is None else star(S, Si)
+        (Wb, Qb), hb = regs[i + 1]
+        if hb:
+            P = np.diag(np.exp(-Qb * hb))
+            S = star(S, dict(T=P, Rb=0 * P, R=0 * P, Tb=P))
+    r = S['R'][:, N]
+    prop = kz0.imag == 0
+    return float(np.sum(np.abs(r[prop]) ** 2 * kz0[prop].real) / kz0[N].real)
+
+
+def test_case4_structure_matches_fourier_modal_method():
+    g1, t1 = _modes([190, 190], [3.61, 1.46, 1.46])
+    g2, t2 = _modes([190], [3.61, 1.46])
+    g1['h'], g2['h'] = 150 / PERIOD, 900 / PERIOD
+    air = dict(kind='tf', n=1.0, h=None)
+    o = stack_ref.calc_scat([air, g1, g2, air], WL / PERIOD, _kp(), ORDER, pol='TE', world_1d=True)
+    R = o['r_list'][-1]
+    assert abs(R + o['t_list'][-1] - 1) < 1e-10 and o['a_list'][-1] < 1e-10
+    eps = lambda te, n: np.array([n[t - 1] ** 2 for t in te])
+    R_fmm = _rcwa_te([(eps(t2, [3.61, 1.46]), 900.0), (eps(t1, [3.61, 1.46, 1.46]), 150.0)], 20)
+    assert abs(R - R_fmm) < 1.5e-3
+    pytest.case4_R = R
+
+
+@pytest.mark.xfail(reason='reference case-4 golden R/T (0.6067/0.3933) are 2e-2 from both the restatement '
+                          'and an independent Fourier-modal calculation (0.5867)', strict=True)
+def test_case4_golden_rta():
+    R = getattr(pytest, 'case4_R', None)
+    if R is None:
+        pytest.skip('needs the case-4 structure test')
+    assert abs(R - load_txt('case_4', 'Reflectance_stack0001.txt')[1]) < 1e-6
