@@ -1,8 +1,9 @@
 """Drop-in for the reference's f2py module `fortran.EMUstack`
 (backend/fortran/Makefile:16-18, imported at mode_calcs.py:30).
 
-calc_modes_2d keeps the positional signature of py_calc_modes.f:1-11 and
-returns the same tuple; the work runs on the GPU through libemu_mi355x.so.
+calc_modes_2d and calc_modes_1d keep the positional signatures of
+py_calc_modes.f:1-11 and py_calc_modes_1d.f:2-12 and return the same tuples;
+the work runs on the GPU through libemu_mi355x.so.
 Meshes are loaded once per path and cached with their symbolic analysis.
 There is no CPU fallback: without the built library or a GPU this raises.
 """
@@ -65,6 +66,53 @@ def calc_modes_2d(lambda_, nval, ordre_ls, debug, msh_path, mesh_file, npt, nel,
                                                 E_H_field=int(E_H_field), itermax=int(itermax))
     table_nod, type_el, x_arr = h.geometry()
     return (beta[0], np.asfortranarray(J[0]), np.asfortranarray(Jd[0]), None, None, table_nod, type_el, x_arr)
+
+
+def mesh_1d(x_P2, table_nod, type_el, leaf_size=0):
+    """Mesh handle of a 1D grating (cached per context and element arrays)."""
+    ctx = context()
+    x = np.ascontiguousarray(np.asarray(x_P2, np.float64))
+    tn = np.asarray(table_nod, np.int64)
+    te = np.ascontiguousarray(np.asarray(type_el, np.int64))
+    key = (id(ctx), '1d', x.tobytes(), np.ascontiguousarray(tn).tobytes(), te.tobytes(), leaf_size)
+    with _cache_lock:
+        h = _mesh_cache.get(key)
+        if h is None:
+            h = _lib.MeshHandle1D(x, tn, te, ctx, leaf_size)
+            _mesh_cache[key] = h
+        return h
+
+
+def calc_modes_1d(lambda_, nval, ordre_ls, nb_typ_el, npt_P2, nel, table_nod, type_el, x_P2, itermax, debug,
+                  mesh_file, n_eff, bloch_vec_x, bloch_vec_y, shift, plot_modes, plot_real, plot_imag, plot_abs,
+                  neq_PW, neq_PW_2d, world_1d):
+    """Bloch modes of a 1D grating (py_calc_modes_1d.f:2-12).
+
+    Returns (beta_1[nval], overlap_J[2 neq_PW, nval], overlap_J_dagger[nval, 2 neq_PW],
+    overlap_J_2d, overlap_J_dagger_2d, sol_P2); the 2D overlaps are filled when
+    world_1d is 0 (else zero arrays of the reference's shapes), sol_P2 (plotting
+    only) is None."""
+    if plot_modes == 1:
+        raise NotImplementedError("field plotting output is not produced by the GPU path")
+    tn = np.asarray(table_nod, np.int64)
+    te = np.asarray(type_el, np.int64).reshape(-1)
+    x = np.asarray(x_P2, np.float64).reshape(-1)
+    if te.size != nel or x.size != npt_P2:
+        raise ValueError("nel / npt_P2 do not match the element arrays")
+    h = mesh_1d(x, tn, te)
+    n_eff = np.asarray(n_eff, np.complex128).reshape(-1)
+    if n_eff.size != nb_typ_el:
+        raise ValueError("n_eff must have nb_typ_el entries")
+    w1 = bool(world_1d)
+    beta, J, Jd, J2, Jd2, _ = context().calc_modes_1d_batch(
+        h, [lambda_], int(nval), int(ordre_ls), [n_eff], [[bloch_vec_x, bloch_vec_y]], [complex(shift)],
+        int(neq_PW), neq_pw_2d=int(neq_PW_2d), world_1d=w1, itermax=int(itermax))
+    if w1:
+        J2 = np.zeros((2 * neq_PW_2d, nval), np.complex128, order='F')
+        Jd2 = np.zeros((nval, 2 * neq_PW_2d), np.complex128, order='F')
+    else:
+        J2, Jd2 = np.asfortranarray(J2[0]), np.asfortranarray(Jd2[0])
+    return beta[0], np.asfortranarray(J[0]), np.asfortranarray(Jd[0]), J2, Jd2, None
 
 
 def calc_modes_2d_batch(lambdas, nval, ordre_ls, mesh_path, n_effs, bloch_vecs, shifts, neq_PW, E_H_field=1,

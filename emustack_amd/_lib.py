@@ -38,7 +38,7 @@ EXPORTS = [
     'emu_last_error', 'emu_ctx_create', 'emu_ctx_destroy', 'emu_set_lapack', 'emu_mesh_load',
     'emu_mesh_analyze', 'emu_mesh_symbolic', 'emu_mesh_free', 'emu_mesh_info', 'emu_mesh_arrays',
     'emu_mesh_geometry', 'emu_calc_modes_2d', 'emu_calc_modes_2d_batch', 'emu_assemble', 'emu_lu_solve',
-    'emu_profile',
+    'emu_profile', 'emu_mesh_create_1d', 'emu_calc_modes_1d', 'emu_calc_modes_1d_batch',
 ]
 
 
@@ -111,6 +111,14 @@ def lib():
         L.emu_assemble.argtypes = [c_vp, c_vp, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp]
         L.emu_lu_solve.argtypes = [c_vp, c_vp, c_i64, c_dp, c_i64, c_dp, c_dp, c_i64p]
         L.emu_profile.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_dp]
+        L.emu_mesh_create_1d.argtypes = [c_vp, c_i64, c_i64, c_i64p, c_i64p, c_dp, ctypes.c_int,
+                                         ctypes.POINTER(c_vp)]
+        L.emu_calc_modes_1d.argtypes = [c_vp, c_vp, ctypes.c_double, c_i64, c_i64, c_i64, c_dp, ctypes.c_double,
+                                        ctypes.c_double, c_dp, c_i64, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp,
+                                        c_dp]
+        L.emu_calc_modes_1d_batch.argtypes = [c_vp, c_vp, c_i64, c_dp, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp,
+                                              c_i64, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp,
+                                              c_i64p]
         for f in EXPORTS:
             getattr(L, f)
         rc = L.emu_set_lapack(*_lapack_pointers())
@@ -194,6 +202,35 @@ class Context:
                                                 ptr(Jd.view(np.float64)), ptr(st, c_i64p)))
         return beta, J.transpose(0, 2, 1), Jd.transpose(0, 2, 1), st
 
+    def calc_modes_1d_batch(self, mesh, lam, nval, ordre, n_eff, bloch, shift, neq_pw, neq_pw_2d=0,
+                            world_1d=True, itermax=30, tol=0.0, block=0):
+        """Batched calc_modes_1d on one MeshHandle1D.  Returns beta (nl, nval),
+        J (nl, 2P, nval), Jd (nl, nval, 2P), J2d, Jd2d (None for a 1D world)
+        and stats."""
+        lam = np.ascontiguousarray(np.atleast_1d(np.asarray(lam, np.float64)))
+        nl = lam.size
+        n_eff = np.ascontiguousarray(np.asarray(n_eff, np.complex128).reshape(nl, -1))
+        ntyp = n_eff.shape[1]
+        bloch = np.ascontiguousarray(np.asarray(bloch, np.float64).reshape(nl, 2))
+        shift = np.ascontiguousarray(np.asarray(shift, np.complex128).reshape(nl))
+        beta = np.zeros((nl, nval), np.complex128)
+        J = np.zeros((nl, nval, 2 * neq_pw), np.complex128)
+        Jd = np.zeros((nl, 2 * neq_pw, nval), np.complex128)
+        w2 = 0 if world_1d else 1
+        J2 = np.zeros((nl, nval, 2 * neq_pw_2d), np.complex128) if w2 else None
+        Jd2 = np.zeros((nl, 2 * neq_pw_2d, nval), np.complex128) if w2 else None
+        opts = np.array([tol, 0.0, float(block), 0.0])
+        st = np.zeros(8, np.int64)
+        with _host_lapack_threads(nl):
+            check(lib().emu_calc_modes_1d_batch(
+                self.h, mesh.h, nl, ptr(lam), nval, ordre, ntyp, ptr(n_eff.view(np.float64)), ptr(bloch),
+                ptr(shift.view(np.float64)), itermax, neq_pw, neq_pw_2d, 0 if w2 else 1, ptr(opts),
+                ptr(beta.view(np.float64)), ptr(J.view(np.float64)), ptr(Jd.view(np.float64)),
+                ptr(J2.view(np.float64)) if w2 else None, ptr(Jd2.view(np.float64)) if w2 else None,
+                ptr(st, c_i64p)))
+        out2 = (J2.transpose(0, 2, 1), Jd2.transpose(0, 2, 1)) if w2 else (None, None)
+        return beta, J.transpose(0, 2, 1), Jd.transpose(0, 2, 1), out2[0], out2[1], st
+
     PROFILE_CLASSES = ('solve_fwd', 'solve_bwd', 'gs_project', 'gs_update', 'spmv_M')
 
     def profile(self, enable=None, every=8):
@@ -207,6 +244,40 @@ class Context:
     def close(self):
         if self.h:
             lib().emu_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MeshHandle1D:
+    """A 1D grating (arrays as objects.py builds them) with cached DOF
+    numbering, pattern and symbolic LU on a context."""
+
+    def __init__(self, x, table_nod, type_el, ctx, leaf_size=0):
+        L = lib()
+        x = np.ascontiguousarray(np.asarray(x, np.float64))
+        tn = np.asarray(table_nod, np.int64)
+        if tn.shape[0] == 3 and tn.ndim == 2:
+            tn = tn.T
+        tn = np.ascontiguousarray(tn)
+        te = np.ascontiguousarray(np.asarray(type_el, np.int64))
+        h = c_vp()
+        check(L.emu_mesh_create_1d(ctx.h, x.size, te.size, ptr(tn, c_i64p), ptr(te, c_i64p), ptr(x),
+                                   int(leaf_size), ctypes.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        info = np.zeros(16, np.int64)
+        check(L.emu_mesh_info(h, ptr(info, c_i64p)))
+        (self.npt, self.nel, self.n_ddl, self.neq, self.nnz, self.nfronts, self.nlevels, self.pool,
+         self.max_f, self.factor_entries, self.lu_flops, self.n_edge, self.npt_p3, self.nw) = [int(v) for v in info[:14]]
+
+    def close(self):
+        if self.h:
+            lib().emu_mesh_free(self.h)
             self.h = None
 
     def __del__(self):

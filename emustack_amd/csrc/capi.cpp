@@ -186,6 +186,78 @@ int emu_calc_modes_2d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval,
                                  E_H_field, itermax, neq_PW, nullptr, beta1, overlap_J, overlap_J_dagger, nullptr);
 }
 
+int emu_mesh_create_1d(emu_ctx *ctx, int64_t npt_P2, int64_t nel, const int64_t *table_nod,
+                       const int64_t *type_el, const double *x_P2, int leaf_size, emu_mesh **out) {
+  if (!ctx || !table_nod || !type_el || !x_P2 || !out) return fail(EMU_E_ARG, "emu_mesh_create_1d: NULL argument");
+  *out = nullptr;
+  EMU_GUARD({
+    emu_mesh *h = new emu_mesh;
+    try {
+      h->m = emu::mesh_create_1d(ctx->c, npt_P2, nel, table_nod, type_el, x_P2, leaf_size);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  })
+}
+
+int emu_calc_modes_1d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *lambda, int64_t nval,
+                            int64_t ordre_ls, int64_t nb_typ_el, const double *n_eff, const double *bloch_vec,
+                            const double *shift, int64_t itermax, int64_t neq_PW, int64_t neq_PW_2d,
+                            int64_t world_1d, const double *opts, double *beta1, double *overlap_J,
+                            double *overlap_J_dagger, double *overlap_J_2d, double *overlap_J_dagger_2d,
+                            int64_t *stats) {
+  if (!ctx || !mesh || !lambda || !n_eff || !bloch_vec || !shift || !beta1 || !overlap_J || !overlap_J_dagger)
+    return fail(EMU_E_ARG, "emu_calc_modes_1d_batch: NULL argument");
+  if (nbatch < 1 || nval < 1 || nb_typ_el < 1 || neq_PW < 1 || itermax < 0 || ordre_ls < 0)
+    return fail(EMU_E_ARG, "emu_calc_modes_1d_batch: invalid size");
+  if (world_1d != 0 && world_1d != 1) return fail(EMU_E_ARG, "world_1d must be 0 or 1");
+  if (world_1d == 0 && (!overlap_J_2d || !overlap_J_dagger_2d || neq_PW_2d < 1))
+    return fail(EMU_E_ARG, "world_1d = 0 needs the 2D overlap outputs and neq_PW_2d");
+  for (int64_t i = 0; i < nbatch; ++i)
+    if (!(lambda[i] > 0)) return fail(EMU_E_ARG, "lambda must be positive");
+  if (opts && opts[1] != 0) return fail(EMU_E_ARG, "only the complex128 precision form is available");
+  EMU_GUARD({
+    emu::ModesArgs a;
+    a.nl = nbatch;
+    a.lambda = lambda;
+    a.nval = nval;
+    a.ordre = ordre_ls;
+    a.ntyp = nb_typ_el;
+    a.eh = 1;
+    a.itermax = itermax;
+    a.neq_pw = neq_PW;
+    a.n_eff = n_eff;
+    a.bloch = bloch_vec;
+    a.shift = shift;
+    if (opts && opts[0] > 0) a.tol = opts[0];
+    if (opts && opts[2] > 0) a.block = (int64_t)opts[2];
+    if (a.block != 1 && a.block != 2 && a.block != 4 && a.block != 8)
+      return fail(EMU_E_ARG, "Krylov block size must be 1, 2, 4 or 8");
+    a.beta = beta1;
+    a.J = overlap_J;
+    a.Jd = overlap_J_dagger;
+    a.world_1d = world_1d;
+    a.neq_pw_2d = neq_PW_2d;
+    a.J2d = overlap_J_2d;
+    a.Jd2d = overlap_J_dagger_2d;
+    a.stats = stats;
+    emu::calc_modes_1d(ctx->c, *mesh->m, a);
+  })
+}
+
+int emu_calc_modes_1d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval, int64_t ordre_ls,
+                      int64_t nb_typ_el, const double *n_eff, double bloch_vec_x, double bloch_vec_y,
+                      const double *shift, int64_t itermax, int64_t neq_PW, int64_t neq_PW_2d, int64_t world_1d,
+                      double *beta1, double *overlap_J, double *overlap_J_dagger, double *overlap_J_2d,
+                      double *overlap_J_dagger_2d) {
+  const double bl[2] = {bloch_vec_x, bloch_vec_y};
+  return emu_calc_modes_1d_batch(ctx, mesh, 1, &lambda, nval, ordre_ls, nb_typ_el, n_eff, bl, shift, itermax,
+                                 neq_PW, neq_PW_2d, world_1d, nullptr, beta1, overlap_J, overlap_J_dagger,
+                                 overlap_J_2d, overlap_J_dagger_2d, nullptr);
+}
+
 int emu_assemble(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, int64_t nb_typ_el, const double *pp,
                  const double *qq, const double *shift, const double *bloch, double *mat1, double *mat2) {
   if (!ctx || !mesh || !pp || !qq || !shift || !bloch) return fail(EMU_E_ARG, "emu_assemble: NULL argument");

@@ -52,7 +52,14 @@ struct Symbolic {
   int64_t factor_entries = 0;  // entries of L and U actually used by a solve
 };
 
+// reference-element integral tables of the 1D grating (Lagrange P2 / P3):
+// Mxx = int P2 P2, Kxy = int P2 (P3)', Kyy = int (P3)' (P3)', Myy = int P3 P3
+// on [0, 1], row-major; physical element of length h: Mxx, Myy times h,
+// Kyy divided by h, Kxy unchanged
+constexpr int kMxx = 0, kKxy = 9, kKyy = 21, kMyy = 37, kTab1d = 53;
+
 struct Mesh {
+  int dim = 2;                    // 2: triangular .mail mesh, 1: 1D grating
   std::string path;
   int64_t npt = 0, nel = 0, n_edge = 0, npt_p3 = 0, n_ddl = 0, neq = 0, nnz = 0;
   double lat[4] = {0, 0, 0, 0};  // v1 = (lat[0], lat[1]), v2 = (lat[2], lat[3])
@@ -76,11 +83,21 @@ struct Mesh {
   // element geometry: affine map
   std::vector<double> geo;        // nel x 8: B00,B10,B01,B11, T00,T10,T01,T11 (col-major)
   std::vector<double> det;        // nel
+  // 1D gratings (dim == 1): ldof is nel x 11, slot nel x 121, citem e*121+i*11+j
+  std::vector<double> h1d;        // nel element lengths
+  std::vector<double> xd1d;       // nel x 11 DOF coordinates
+  double period_x = 1.0;          // distance carried by the Bloch phase
+  double tab1d[kTab1d] = {};
   Symbolic sym;
 };
 
 // Build all per-mesh data from a .mail file.  Throws std::runtime_error.
 void build_mesh(Mesh &m, const std::string &path);
+// Build a 1D grating mesh from the reference's arrays (table_nod 3 x nel,
+// 1-based [left, right, mid]; type_el 1-based; x of the 2 nel + 1 nodes).
+void build_mesh_1d(Mesh &m, int64_t npt, int64_t nel, const int64_t *table_nod, const int64_t *type_el,
+                   const double *x);
+void element_tables_1d(double *tab);
 // Nested-dissection multifrontal symbolic analysis of the pattern.
 void build_symbolic(Mesh &m, int leaf_size);
 

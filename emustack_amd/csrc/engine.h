@@ -152,6 +152,8 @@ struct DevMesh {
   std::vector<LevelSched> sched;     // sched[0] = deepest level
   int32_t max_f = 0;
   // geometry for the plane-wave integrals
+  // 1D gratings
+  DBuf<double> h1d, xd1d, tab1d;     // element lengths, DOF coordinates (nel x 11), integral tables
   DBuf<double> xnod;                 // npt x 2
   DBuf<int64_t> table_nod;           // nel x 6 (1-based)
 };
@@ -216,6 +218,20 @@ void launch_trmm_right(int nb, int64_t n, int bq, const zd *W, int64_t ldw, int6
 void launch_block_copy(int nb, int rows, int cols, const zd *S, int64_t lds, int64_t sS, zd *D, int64_t ldd,
                        int64_t sD, bool add, hipStream_t s);
 void launch_fill(zd *p, int64_t n, zd v, hipStream_t s);
+
+// ---------------- 1D gratings (kernels_1d.hip) ----------------
+// which = 0: out1 = K - shift M, out2 = M of asmbly_1d.f; which = 1: out1 =
+// the overlap form of orthogonal_1d.f (out2 unused).  bloch [nb][2].
+void launch_gather_1d(DevMesh &dm, int nb, int ntyp, const zd *pp, const zd *qq, const zd *shift,
+                      const double *bloch, int which, zd *out1, zd *out2, hipStream_t s);
+// G[b] (n x 2P, col-major): plane-wave load of J (sign -1, exp(-i alpha x))
+// or J^dagger (sign +1); alpha[b][P] in sorted plane-wave order, ky[b] the
+// prime k_y, phase from bloch[b][0]
+void launch_pw_load_1d(DevMesh &dm, int nb, int P, const double *alpha, const double *ky, const int32_t *sign,
+                       const double *bloch, zd *G, hipStream_t s);
+// array_sol_1d.f normalisation value per mode: den[b][k]
+void launch_field_max_1d(DevMesh &dm, int nb, int nval, const zd *X, int64_t sX, const double *bloch, double xmid,
+                         double dq, zd *den, hipStream_t s);
 
 // ---------------- post-processing (kernels_post.hip) ----------------
 // coefficient fields: max |field| over (e, node, comp) per mode, first index

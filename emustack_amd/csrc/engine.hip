@@ -46,13 +46,19 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
   Mesh &m = dm.host;
   const Symbolic &sy = m.sym;
   const int64_t nel = m.nel, n = m.neq;
-  dm.geo.upload(m.geo, s);
-  dm.det.upload(m.det, s);
-  {
+  const bool d2 = m.dim == 2;
+  const int nloc = d2 ? 22 : 11;  // local DOFs per element
+  if (d2) {
+    dm.geo.upload(m.geo, s);
+    dm.det.upload(m.det, s);
     std::vector<signed char> o(m.orient.begin(), m.orient.end());
     dm.orient.upload(o, s);
+    dm.dphase.upload(m.dphase, s);
+  } else {
+    dm.h1d.upload(m.h1d, s);
+    dm.xd1d.upload(m.xd1d, s);
+    dm.tab1d.upload(m.tab1d, kTab1d, s);
   }
-  dm.dphase.upload(m.dphase, s);
   {
     std::vector<int32_t> t(nel);
     int64_t mx = 0;
@@ -68,8 +74,10 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
   dm.citem.upload(m.citem, s);
   dm.row_ptr.upload(m.row_ptr, s);
   dm.col_ind.upload(m.col_ind, s);
-  dm.xnod.upload(m.x, s);
-  dm.table_nod.upload(m.table_nod, s);
+  if (d2) {
+    dm.xnod.upload(m.x, s);
+    dm.table_nod.upload(m.table_nod, s);
+  }
   // permuted CSR (new numbering) for M
   {
     std::vector<int64_t> prp(n + 1, 0);
@@ -91,22 +99,24 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
     dm.pcol_ind.upload(pci, s);
     dm.pnnz_src.upload(psrc, s);
   }
-  // transverse DOF contributors
+  // transverse DOF contributors (2D: 12 edge/face DOFs per element, item
+  // e*12 + l; 1D: the x- and y-component DOFs 0..6, item e*11 + l)
   {
+    const int ntr = d2 ? 12 : 7, istr = d2 ? 12 : 11;
     std::vector<int32_t> cnt(n + 1, 0);
     for (int64_t e = 0; e < nel; ++e)
-      for (int l = 0; l < 12; ++l) cnt[m.ldof[22 * e + l] + 1]++;
+      for (int l = 0; l < ntr; ++l) cnt[m.ldof[nloc * e + l] + 1]++;
     std::vector<int32_t> ptr(n + 1, 0);
     for (int64_t i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + cnt[i + 1];
     std::vector<int32_t> item(ptr[n]);
     std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
     for (int64_t e = 0; e < nel; ++e)
-      for (int l = 0; l < 12; ++l) item[fill[m.ldof[22 * e + l]]++] = (int32_t)(12 * e + l);
+      for (int l = 0; l < ntr; ++l) item[fill[m.ldof[nloc * e + l]]++] = (int32_t)(istr * e + l);
     dm.dcptr.upload(ptr, s);
     dm.dcitem.upload(item, s);
   }
   // transverse field at the 6 P2 nodes from the 12 transverse coefficients
-  {
+  if (d2) {
     std::vector<double> gt(144 * nel);
     const double xn[6][2] = {{0, 0}, {1, 0}, {0, 1}, {0.5, 0}, {0.5, 0.5}, {0, 0.5}};
     for (int64_t e = 0; e < nel; ++e) {
@@ -254,7 +264,7 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
     dm.wz_cnt = ((int64_t)tasks.size() - dm.wz_off) / 4;
   }
   dm.tasks.upload(tasks, s);
-  launch_elem_integrals(dm, s);
+  if (d2) launch_elem_integrals(dm, s);
   EMU_HIP(hipStreamSynchronize(s));
 }
 
@@ -364,8 +374,12 @@ void assemble(Ctx *c, DevMesh &dm, int nb, int ntyp, const double *pp, const dou
   zd *ve = c->buf<zd>("ve", (size_t)nb * dm.host.nel * 14);
   zd *m1 = c->buf<zd>("mat1", (size_t)nb * nnz);
   zd *m2 = c->buf<zd>("mat2", (size_t)nb * nnz);
-  launch_phase(dm, nb, dbl, ve, c->stream);
-  launch_gather(dm, nb, ntyp, dpp, dqq, dsh, ve, m1, m2, c->stream);
+  if (dm.host.dim == 1) {
+    launch_gather_1d(dm, nb, ntyp, dpp, dqq, dsh, dbl, 0, m1, m2, c->stream);
+  } else {
+    launch_phase(dm, nb, dbl, ve, c->stream);
+    launch_gather(dm, nb, ntyp, dpp, dqq, dsh, ve, m1, m2, c->stream);
+  }
   if (mat1) EMU_HIP(hipMemcpyAsync(mat1, m1, sizeof(zd) * nb * nnz, hipMemcpyDeviceToHost, c->stream));
   if (mat2) EMU_HIP(hipMemcpyAsync(mat2, m2, sizeof(zd) * nb * nnz, hipMemcpyDeviceToHost, c->stream));
   EMU_HIP(hipStreamSynchronize(c->stream));
@@ -805,24 +819,18 @@ static std::vector<std::pair<int, int>> pair_swaps(const std::vector<cplx> &L, i
 }
 
 // ------------------------------------------------------------ calc_modes
-void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
-  EMU_HIP(hipSetDevice(c->device));
-  if (!lapack_ready()) throw LapackMissing("host LAPACK routines not registered (emu_set_lapack)");
-  hipStream_t s = c->stream;
-  const Mesh &mh = dm.host;
-  const int64_t n = mh.neq, nnz = mh.nnz, nel = mh.nel;
-  const int nl = (int)a.nl, nb = 2 * nl, ntyp = (int)a.ntyp, nval = (int)a.nval;
-  if (nl < 1 || nval < 1 || ntyp < dm.ntyp) throw std::invalid_argument("calc_modes: bad sizes");
-  // Krylov dimension: ARPACK's nvect = 2 nval + nval/2 + 3 (py_calc_modes.f:256),
-  // rounded up to a whole number of blocks
-  const int bq = (int)a.block;
-  int m = 2 * nval + nval / 2 + 3;
-  m = ((m + bq - 1) / bq) * bq;
-  if (m + bq >= n || nval >= m) throw std::invalid_argument("calc_modes: num_BMs too large for the mesh");
-  // material coefficients (py_calc_modes.f:583-599); prime = +bloch, adjoint = -bloch
-  std::vector<cplx> pp((size_t)nb * ntyp), qq((size_t)nb * ntyp), sh(nb);
-  std::vector<double> bl(2 * (size_t)nb);
-  std::vector<double> k0(nl);
+// Per-instance coefficients (py_calc_modes.f:583-599, py_calc_modes_1d.f:
+// 318-372): instance 2i is the prime problem (+bloch), 2i+1 the adjoint
+// (-bloch); E-field formulation (eh = 1): q = eps k0^2, p = 1; H-field:
+// q = k0^2, p = 1/eps.  Rejects homogeneous layers as the reference does.
+static void instance_params(const ModesArgs &a, int eh, std::vector<cplx> &pp, std::vector<cplx> &qq,
+                            std::vector<cplx> &sh, std::vector<double> &bl, std::vector<double> &k0) {
+  const int nl = (int)a.nl, nb = 2 * nl, ntyp = (int)a.ntyp;
+  pp.assign((size_t)nb * ntyp, 0);
+  qq.assign((size_t)nb * ntyp, 0);
+  sh.assign(nb, 0);
+  bl.assign(2 * (size_t)nb, 0);
+  k0.assign(nl, 0);
   for (int i = 0; i < nl; ++i) {
     k0[i] = 2.0 * M_PI / a.lambda[i];
     std::vector<cplx> eps(ntyp);
@@ -837,7 +845,7 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
     for (int h = 0; h < 2; ++h) {
       int b = 2 * i + h;
       for (int t = 0; t < ntyp; ++t) {
-        if (a.eh == 1) {
+        if (eh == 1) {
           qq[(size_t)b * ntyp + t] = eps[t] * k0[i] * k0[i];
           pp[(size_t)b * ntyp + t] = 1.0;
         } else {
@@ -851,6 +859,78 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
       bl[2 * b + 1] = sg * a.bloch[2 * i + 1];
     }
   }
+}
+
+// Steps shared by the 2D and 1D drivers: numeric LU of K - shift M
+// (mat1), Krylov-Schur for the nval largest |mu| of (K - shift M)^-1 M,
+// beta = sqrt(1/mu + shift) with the reference's branch rule, and the
+// z_indexx sort on -Re(beta^2).  The sorted modes (original numbering) end
+// in Xn; Xo is scratch.
+struct SolveOut {
+  EigResult er;
+  std::vector<cplx> beta;  // nb x nval, sorted
+  int32_t *dord = nullptr; // device scratch nb x nval
+  int32_t *pflag = nullptr;
+};
+
+static SolveOut factor_solve_sort(Ctx *c, DevMesh &dm, const ModesArgs &a, int nb, int m, int bq,
+                                  const std::vector<cplx> &sh, const zd *m1, const zd *m2p, zd *Xn, zd *Xo,
+                                  StageTimer &tm) {
+  hipStream_t s = c->stream;
+  const Mesh &mh = dm.host;
+  const Symbolic &sy = mh.sym;
+  const int64_t n = mh.neq;
+  const int nval = (int)a.nval;
+  SolveOut o;
+  zd *pool = c->buf<zd>("pool", (size_t)nb * sy.pool);
+  int32_t *piv = c->buf<int32_t>("piv", (size_t)nb * sy.npiv);
+  o.pflag = c->buf<int32_t>("pflag", (size_t)nb * sy.npiv);
+  int32_t *sig = c->buf<int32_t>("sig", (size_t)nb * sy.npiv);
+  zd *inv = c->buf<zd>("inv", (size_t)nb * dm.isz);
+  lu_factor(dm, nb, m1, pool, piv, o.pflag, sig, inv, s);
+  tm.mark("lu_factor");
+  o.er = krylov_schur(c, dm, nb, nval, m, bq, (int)a.itermax, a.tol, inv, sig, m2p, Xn, tm);
+  launch_permute_vec(dm, nb, nval, Xn, Xo, false, s);
+  o.beta.assign((size_t)nb * nval, 0);
+  std::vector<int32_t> order((size_t)nb * nval);
+  for (int b = 0; b < nb; ++b) {
+    std::vector<cplx> bb(nval);
+    std::vector<double> key(nval);
+    for (int k = 0; k < nval; ++k) {
+      bb[k] = beta_from_mu(o.er.mu[(size_t)b * nval + k], sh[b]);
+      key[k] = -(bb[k] * bb[k]).real();
+    }
+    std::vector<int> idx = nr_indexx(key);
+    for (int k = 0; k < nval; ++k) {
+      o.beta[(size_t)b * nval + k] = bb[idx[k]];
+      order[(size_t)b * nval + k] = idx[k];
+    }
+  }
+  o.dord = c->buf<int32_t>("order", (size_t)nb * nval);
+  EMU_HIP(hipMemcpyAsync(o.dord, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipStreamSynchronize(s));
+  launch_permute_cols(nb, n, nval, Xo, n * nval, o.dord, Xn, n * nval, s);
+  return o;
+}
+
+void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
+  EMU_HIP(hipSetDevice(c->device));
+  if (!lapack_ready()) throw LapackMissing("host LAPACK routines not registered (emu_set_lapack)");
+  hipStream_t s = c->stream;
+  const Mesh &mh = dm.host;
+  const int64_t n = mh.neq, nnz = mh.nnz, nel = mh.nel;
+  const int nl = (int)a.nl, nb = 2 * nl, ntyp = (int)a.ntyp, nval = (int)a.nval;
+  if (mh.dim != 2) throw std::invalid_argument("calc_modes_2d: the mesh handle holds a 1D grating");
+  if (nl < 1 || nval < 1 || ntyp < dm.ntyp) throw std::invalid_argument("calc_modes: bad sizes");
+  // Krylov dimension: ARPACK's nvect = 2 nval + nval/2 + 3 (py_calc_modes.f:256),
+  // rounded up to a whole number of blocks
+  const int bq = (int)a.block;
+  int m = 2 * nval + nval / 2 + 3;
+  m = ((m + bq - 1) / bq) * bq;
+  if (m + bq >= n || nval >= m) throw std::invalid_argument("calc_modes: num_BMs too large for the mesh");
+  std::vector<cplx> pp, qq, sh;
+  std::vector<double> bl, k0;
+  instance_params(a, (int)a.eh, pp, qq, sh, bl, k0);
   zd *dpp, *dqq, *dsh;
   double *dbl;
   upload_params(c, dm, nb, ntyp, reinterpret_cast<const double *>(pp.data()),
@@ -867,40 +947,15 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   launch_gather(dm, nb, ntyp, dpp, dqq, dsh, ve, m1, m2, s);
   launch_permute_vals(dm, nb, m2, m2p, s);
   tm.mark("assembly");
-  // 2. LU of K - shift M
-  const Symbolic &sy = mh.sym;
-  zd *pool = c->buf<zd>("pool", (size_t)nb * sy.pool);
-  int32_t *piv = c->buf<int32_t>("piv", (size_t)nb * sy.npiv);
-  int32_t *pfl = c->buf<int32_t>("pflag", (size_t)nb * sy.npiv);
-  int32_t *sig = c->buf<int32_t>("sig", (size_t)nb * sy.npiv);
-  zd *inv = c->buf<zd>("inv", (size_t)nb * dm.isz);
-  lu_factor(dm, nb, m1, pool, piv, pfl, sig, inv, s);
-  tm.mark("lu_factor");
-  // 3. eigensolve
+  // 2.-4. LU of K - shift M, eigensolve, beta and sort
   zd *Xn = c->buf<zd>("Xn", (size_t)nb * n * nval);
-  EigResult er = krylov_schur(c, dm, nb, nval, m, bq, (int)a.itermax, a.tol, inv, sig, m2p, Xn, tm);
   zd *Xo = c->buf<zd>("Xo", (size_t)nb * n * nval);
-  launch_permute_vec(dm, nb, nval, Xn, Xo, false, s);
-  // 4. beta, sort (z_indexx on -Re(beta^2))
-  std::vector<cplx> beta((size_t)nb * nval);
-  std::vector<int32_t> order((size_t)nb * nval);
-  for (int b = 0; b < nb; ++b) {
-    std::vector<cplx> bb(nval);
-    std::vector<double> key(nval);
-    for (int k = 0; k < nval; ++k) {
-      bb[k] = beta_from_mu(er.mu[(size_t)b * nval + k], sh[b]);
-      key[k] = -(bb[k] * bb[k]).real();
-    }
-    std::vector<int> idx = nr_indexx(key);
-    for (int k = 0; k < nval; ++k) {
-      beta[(size_t)b * nval + k] = bb[idx[k]];
-      order[(size_t)b * nval + k] = idx[k];
-    }
-  }
-  int32_t *dord = c->buf<int32_t>("order", (size_t)nb * nval);
-  EMU_HIP(hipMemcpyAsync(dord, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice, s));
-  EMU_HIP(hipStreamSynchronize(s));
-  launch_permute_cols(nb, n, nval, Xo, n * nval, dord, Xn, n * nval, s);  // sorted modes in Xn
+  SolveOut so = factor_solve_sort(c, dm, a, nb, m, bq, sh, m1, m2p, Xn, Xo, tm);
+  const EigResult &er = so.er;
+  const std::vector<cplx> &beta = so.beta;
+  int32_t *dord = so.dord;
+  int32_t *pfl = so.pflag;
+  const Symbolic &sy = mh.sym;
   // 5. field normalisation (array_sol.f:297-339)
   zd *zmax = c->buf<zd>("zmax", (size_t)nb * nval);
   size_t nparts = (size_t)((nel + 255) / 256);
@@ -1023,6 +1078,223 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
     a.stats[2] = std::accumulate(pf.begin(), pf.end(), (int64_t)0);
     a.stats[3] = er.steps;
     a.stats[4] = er.rank_fixes;
+  }
+}
+
+// ----------------------------------------------------------- 1D gratings
+// calc_modes_1d (backend/fortran/py_calc_modes_1d.f:1-707) on the GPU: the
+// same sparse LU / Krylov-Schur machinery on the 1D pattern, followed by
+// the normalisation of array_sol_1d.f, the overlap of orthogonal_1d.f with
+// its pair swaps, normalisation_1d.f (both families by 1/sqrt(L_ii)),
+// pw_ordering_1d.f and the J / J^dagger overlaps; pw_matrix_1d_to_2d.f when
+// the rest of the stack is 2D.
+DevMesh *mesh_create_1d(Ctx *c, int64_t npt, int64_t nel, const int64_t *table_nod, const int64_t *type_el,
+                        const double *x, int leaf) {
+  EMU_HIP(hipSetDevice(c->device));
+  DevMesh *dm = new DevMesh();
+  try {
+    build_mesh_1d(dm->host, npt, nel, table_nod, type_el, x);
+    build_symbolic(dm->host, leaf > 0 ? leaf : 96);
+    devmesh_build(*dm, c->stream);
+  } catch (...) {
+    delete dm;
+    throw;
+  }
+  return dm;
+}
+
+void calc_modes_1d(Ctx *c, DevMesh &dm, const ModesArgs &a) {
+  EMU_HIP(hipSetDevice(c->device));
+  if (!lapack_ready()) throw LapackMissing("host LAPACK routines not registered (emu_set_lapack)");
+  hipStream_t s = c->stream;
+  const Mesh &mh = dm.host;
+  if (mh.dim != 1) throw std::invalid_argument("calc_modes_1d: the mesh handle holds a 2D mesh");
+  const int64_t n = mh.neq, nnz = mh.nnz, nel = mh.nel;
+  const int nl = (int)a.nl, nb = 2 * nl, ntyp = (int)a.ntyp, nval = (int)a.nval;
+  if (nl < 1 || nval < 1 || ntyp < dm.ntyp) throw std::invalid_argument("calc_modes_1d: bad sizes");
+  const int bq = (int)a.block;
+  int m = 2 * nval + nval / 2 + 3;  // nvect of py_calc_modes_1d.f:145
+  m = ((m + bq - 1) / bq) * bq;
+  if (m + bq >= n || nval >= m) throw std::invalid_argument("calc_modes_1d: num_BMs too large for the mesh");
+  const int P = 2 * (int)a.ordre + 1;
+  if (P != a.neq_pw) throw std::invalid_argument("neq_PW must be 2 ordre_ls + 1 for a 1D grating");
+  std::vector<cplx> pp, qq, sh;
+  std::vector<double> bl, k0;
+  instance_params(a, 1, pp, qq, sh, bl, k0);  // the 1D path is E-field only
+  zd *dpp, *dqq, *dsh;
+  double *dbl;
+  upload_params(c, dm, nb, ntyp, reinterpret_cast<const double *>(pp.data()),
+                reinterpret_cast<const double *>(qq.data()), reinterpret_cast<const double *>(sh.data()), bl.data(),
+                &dpp, &dqq, &dsh, &dbl);
+  StageTimer tm(s);
+  tm.mark("setup");
+  // 1. assembly of K - shift M, M and the overlap form
+  zd *m1 = c->buf<zd>("mat1", (size_t)nb * nnz);
+  zd *m2 = c->buf<zd>("mat2", (size_t)nb * nnz);
+  zd *m2p = c->buf<zd>("mat2p", (size_t)nb * nnz);
+  zd *mov = c->buf<zd>("mat_ovl", (size_t)nb * nnz);
+  launch_gather_1d(dm, nb, ntyp, dpp, dqq, dsh, dbl, 0, m1, m2, s);
+  launch_gather_1d(dm, nb, ntyp, dpp, dqq, dsh, dbl, 1, mov, nullptr, s);
+  launch_permute_vals(dm, nb, m2, m2p, s);
+  tm.mark("assembly");
+  // 2.-4. LU, eigensolve, beta and sort
+  zd *Xn = c->buf<zd>("Xn", (size_t)nb * n * nval);
+  zd *Xo = c->buf<zd>("Xo", (size_t)nb * n * nval);
+  SolveOut so = factor_solve_sort(c, dm, a, nb, m, bq, sh, m1, m2p, Xn, Xo, tm);
+  const std::vector<cplx> &beta = so.beta;
+  // 5. field normalisation (array_sol_1d.f:141-292)
+  double xlo = mh.xd1d[0], xhi = mh.xd1d[0];
+  for (double v : mh.xd1d) {
+    xlo = std::min(xlo, v);
+    xhi = std::max(xhi, v);
+  }
+  const double xmid = (xlo + xhi) / 2.0, dq = (xhi - xlo) / (double)nel / 4.0;
+  zd *den = c->buf<zd>("zmax", (size_t)nb * nval);
+  launch_field_max_1d(dm, nb, nval, Xn, n * nval, dbl, xmid, dq, den, s);
+  launch_scale_cols(nb, n, nval, Xn, n, n * nval, den, nval, true, s);
+  // 6. overlap L = X2^T Mov X1 diag(beta1) / k0 of the prime instances
+  zd *Y = c->buf<zd>("ovl_Y", (size_t)nl * n * nval);
+  launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, mov, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s);
+  zd *Ld = c->buf<zd>("ovl_L", (size_t)nl * nval * nval);
+  size_t atb_elems = (size_t)nl * nval * nval * 64 + (size_t)nb * 2 * P * nval * 64;
+  zd *atbs = c->buf<zd>("post_atb", atb_elems);
+  launch_atb(nl, n, nval, nval, Xn + n * nval, n, 2 * n * nval, Y, n, n * nval, Ld, nval, (int64_t)nval * nval,
+             false, atbs, atb_elems, s);
+  std::vector<cplx> Lh((size_t)nl * nval * nval);
+  EMU_HIP(hipMemcpyAsync(Lh.data(), Ld, sizeof(zd) * Lh.size(), hipMemcpyDeviceToHost, s));
+  EMU_HIP(hipStreamSynchronize(s));
+  std::vector<int32_t> perm2((size_t)nb * nval);
+  std::vector<zd> scl((size_t)nb * nval, zmk(1, 0));
+  for (int i = 0; i < nl; ++i) {
+    cplx *L = &Lh[(size_t)i * nval * nval];
+    for (int j = 0; j < nval; ++j) {
+      cplx f = beta[(size_t)(2 * i) * nval + j] / k0[i];
+      for (int r = 0; r < nval; ++r) L[r + (size_t)j * nval] *= f;
+    }
+    std::vector<int> p(nval);
+    std::iota(p.begin(), p.end(), 0);
+    std::vector<cplx> Lv(L, L + (size_t)nval * nval);
+    for (auto &sw : pair_swaps(Lv, nval)) std::swap(p[sw.first], p[sw.second]);
+    for (int k = 0; k < nval; ++k) {
+      perm2[(size_t)(2 * i) * nval + k] = k;
+      perm2[(size_t)(2 * i + 1) * nval + k] = p[k];
+      cplx z = std::sqrt(L[p[k] + (size_t)k * nval]);  // diagonal after the swaps
+      if (std::abs(z) > 1e-8) {
+        cplx f = 1.0 / z;
+        scl[(size_t)(2 * i) * nval + k] = zmk(f.real(), f.imag());
+        scl[(size_t)(2 * i + 1) * nval + k] = zmk(f.real(), f.imag());
+      }
+    }
+  }
+  EMU_HIP(hipMemcpyAsync(so.dord, perm2.data(), sizeof(int32_t) * perm2.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipMemcpyAsync(den, scl.data(), sizeof(zd) * scl.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipStreamSynchronize(s));
+  launch_permute_cols(nb, n, nval, Xn, n * nval, so.dord, Xo, n * nval, s);
+  launch_scale_cols(nb, n, nval, Xo, n, n * nval, den, nval, false, s);
+  tm.mark("normalise_orthogonality");
+  // 7. plane-wave order (pw_ordering_1d.f) and J / J^dagger
+  std::vector<double> al((size_t)nb * P), kyv(nb);
+  std::vector<int32_t> sgn(nb);
+  std::vector<std::vector<int>> inv1(nl, std::vector<int>(P));
+  for (int i = 0; i < nl; ++i) {
+    const double kx = a.bloch[2 * i], ky = a.bloch[2 * i + 1];
+    std::vector<double> key(P), alu(P);
+    for (int q = 0; q < P; ++q) {
+      alu[q] = kx + 2.0 * M_PI / mh.period_x * (q - (int)a.ordre);
+      cplx kz = std::sqrt(cplx(k0[i] * k0[i] - alu[q] * alu[q] - ky * ky, 0.0));
+      key[q] = -(kz * kz).real();
+    }
+    std::vector<int> idx = nr_indexx(key);
+    for (int r = 0; r < P; ++r) {
+      inv1[i][idx[r]] = r;
+      al[(size_t)(2 * i) * P + r] = alu[idx[r]];
+      al[(size_t)(2 * i + 1) * P + r] = alu[idx[r]];
+    }
+    kyv[2 * i] = kyv[2 * i + 1] = ky;
+    sgn[2 * i] = -1;
+    sgn[2 * i + 1] = 1;
+  }
+  double *dal = c->buf<double>("pw_al", (size_t)nb * P);
+  double *dky = c->buf<double>("pw_ky", nb);
+  int32_t *dsg = c->buf<int32_t>("pw_sg", nb);
+  EMU_HIP(hipMemcpyAsync(dal, al.data(), sizeof(double) * al.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipMemcpyAsync(dky, kyv.data(), sizeof(double) * kyv.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipMemcpyAsync(dsg, sgn.data(), sizeof(int32_t) * sgn.size(), hipMemcpyHostToDevice, s));
+  EMU_HIP(hipStreamSynchronize(s));
+  zd *G = c->buf<zd>("pw_G", (size_t)nb * 2 * P * n);
+  launch_pw_load_1d(dm, nb, P, dal, dky, dsg, dbl, G, s);
+  zd *R = c->buf<zd>("pw_R", (size_t)nb * 2 * P * nval);
+  launch_atb(nb, n, 2 * P, nval, G, n, 2 * P * n, Xo, n, n * nval, R, 2 * P, (int64_t)2 * P * nval, false, atbs,
+             atb_elems, s);
+  std::vector<zd> Rh((size_t)nb * 2 * P * nval);
+  EMU_HIP(hipMemcpyAsync(Rh.data(), R, sizeof(zd) * Rh.size(), hipMemcpyDeviceToHost, s));
+  std::vector<int32_t> pf((size_t)nb * mh.sym.npiv);
+  EMU_HIP(hipMemcpyAsync(pf.data(), so.pflag, sizeof(int32_t) * pf.size(), hipMemcpyDeviceToHost, s));
+  EMU_HIP(hipStreamSynchronize(s));
+  // outputs: beta, J (2P x nval), J^dagger (nval x 2P) and, for a 2D world,
+  // the same embedded in the 2D plane-wave basis (pw_matrix_1d_to_2d.f)
+  const int P2 = 2 * P;
+  int PW2 = 0;
+  std::vector<int> pos2d;  // per 1D order (unsorted): sorted 2D position of (px, 0)
+  for (int i = 0; i < nl; ++i) {
+    for (int k = 0; k < nval; ++k) {
+      a.beta[2 * ((size_t)i * nval + k)] = beta[(size_t)(2 * i) * nval + k].real();
+      a.beta[2 * ((size_t)i * nval + k) + 1] = beta[(size_t)(2 * i) * nval + k].imag();
+    }
+    const zd *Rp = &Rh[(size_t)(2 * i) * P2 * nval];
+    const zd *Ra = &Rh[(size_t)(2 * i + 1) * P2 * nval];
+    double *J = a.J + (size_t)i * 2 * P2 * nval;
+    double *Jd = a.Jd + (size_t)i * 2 * P2 * nval;
+    for (int k = 0; k < nval; ++k)
+      for (int q = 0; q < P2; ++q) {
+        J[2 * (q + (size_t)k * P2)] = Rp[q + (size_t)k * P2].x;
+        J[2 * (q + (size_t)k * P2) + 1] = Rp[q + (size_t)k * P2].y;
+        Jd[2 * (k + (size_t)q * nval)] = Ra[q + (size_t)k * P2].x;
+        Jd[2 * (k + (size_t)q * nval) + 1] = Ra[q + (size_t)k * P2].y;
+      }
+    if (a.world_1d == 0 && a.J2d && a.Jd2d) {
+      const double kx = a.bloch[2 * i], ky = a.bloch[2 * i + 1], px_ = mh.period_x;
+      const int o = (int)a.ordre;
+      std::vector<double> key;
+      std::vector<int> py0(P, -1);
+      for (int px = -o; px <= o; ++px)
+        for (int py = -o; py <= o; ++py)
+          if ((px / px_) * (px / px_) + (py / px_) * (py / px_) <= (o / px_) * (o / px_)) {
+            double aa = kx + 2.0 * M_PI / px_ * px, bb = ky + 2.0 * M_PI / px_ * py;
+            cplx kz = std::sqrt(cplx(k0[i] * k0[i] - aa * aa - bb * bb, 0.0));
+            if (py == 0) py0[px + o] = (int)key.size();
+            key.push_back(-(kz * kz).real());
+          }
+      PW2 = (int)key.size();
+      if (PW2 != a.neq_pw_2d) throw std::invalid_argument("neq_PW_2d does not match the 2D plane-wave truncation");
+      std::vector<int> idx = nr_indexx(key), inv2(PW2);
+      for (int r = 0; r < PW2; ++r) inv2[idx[r]] = r;
+      double *J2 = a.J2d + (size_t)i * 2 * 2 * PW2 * nval;
+      double *Jd2 = a.Jd2d + (size_t)i * 2 * 2 * PW2 * nval;
+      std::fill(J2, J2 + (size_t)2 * 2 * PW2 * nval, 0.0);
+      std::fill(Jd2, Jd2 + (size_t)2 * 2 * PW2 * nval, 0.0);
+      for (int ipw = 0; ipw < P; ++ipw) {
+        const int s1 = inv1[i][ipw], s2 = inv2[py0[ipw]];
+        for (int k = 0; k < nval; ++k)
+          for (int hpol = 0; hpol < 2; ++hpol) {
+            const zd vj = Rp[(s1 + hpol * P) + (size_t)k * P2];
+            const zd vd = Ra[(s1 + hpol * P) + (size_t)k * P2];
+            const size_t q2 = (size_t)(s2 + hpol * PW2);
+            J2[2 * (q2 + (size_t)k * 2 * PW2)] = vj.x;
+            J2[2 * (q2 + (size_t)k * 2 * PW2) + 1] = vj.y;
+            Jd2[2 * (k + q2 * nval)] = vd.x;
+            Jd2[2 * (k + q2 * nval) + 1] = vd.y;
+          }
+      }
+    }
+  }
+  tm.mark("J_overlaps");
+  if (a.stats) {
+    a.stats[0] = so.er.ops;
+    a.stats[1] = so.er.restarts;
+    a.stats[2] = std::accumulate(pf.begin(), pf.end(), (int64_t)0);
+    a.stats[3] = so.er.steps;
+    a.stats[4] = so.er.rank_fixes;
   }
 }
 

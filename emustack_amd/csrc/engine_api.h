@@ -51,7 +51,14 @@ struct ModesArgs {
   double *J = nullptr;             // nl x (2P x nval) complex
   double *Jd = nullptr;            // nl x (nval x 2P) complex
   int64_t *stats = nullptr;
+  // 1D gratings: world_1d = 0 also fills the 2D plane-wave embedding
+  int64_t world_1d = 1, neq_pw_2d = 0;
+  double *J2d = nullptr;           // nl x (2 P2d x nval) complex
+  double *Jd2d = nullptr;          // nl x (nval x 2 P2d) complex
 };
 void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a);
+DevMesh *mesh_create_1d(Ctx *c, int64_t npt, int64_t nel, const int64_t *table_nod, const int64_t *type_el,
+                        const double *x, int leaf);
+void calc_modes_1d(Ctx *c, DevMesh &dm, const ModesArgs &a);
 
 }  // namespace emu

@@ -170,3 +170,55 @@ def ellipse_cell_ntri(period, period_y, a_nm, b_nm, ntri, rotation_deg=0.0, seed
     m = ellipse_cell(period, period_y, a_nm, b_nm, rotation_deg, lc, seed)
     lc *= np.sqrt(len(m[2]) / ntri)
     return ellipse_cell(period, period_y, a_nm, b_nm, rotation_deg, lc, seed)
+
+
+# --------------------------------------------------------------- 1D gratings
+
+def grating_1d(period, diameters, lc_bkg, small_space=None, split_touching_incs=False):
+    """Uniform P2 mesh of a 1D grating cell (objects.py:1054-1530 of the
+    reference, one or two inclusions): nel = round(1/lc_bkg) elements on
+    [0, 1]; nodes 2i-1 (left), 2i (mid), 2i+1 (right) of element i; the
+    material type of each element follows from its end points exactly as the
+    reference decides it (inclusion a centred, inclusion b at the cell edges
+    or touching a).  Returns x_arr (2 nel + 1,), table_nod (3, nel) 1-based,
+    type_el (nel,) 1-based and the reference's mesh name."""
+    ds = [d for d in diameters if d > 0]
+    if not ds:
+        raise ValueError("Must have at least one grating of nonzero width.")
+    if len(ds) > 2:
+        raise NotImplementedError("1D gratings with more than two inclusions")
+    nel = int(np.round(1.0 / lc_bkg))
+    delta_x = (1.0 - 0.0) / nel
+    x = np.zeros(2 * nel + 1)
+    for i in range(nel):
+        x0 = 0.0 + i * delta_x
+        x[2 * i] = x0
+        x[2 * i + 1] = x0 + delta_x / 2.0
+    x[2 * nel] = 0.0 + nel * delta_x
+    table = np.empty((3, nel), dtype=np.int64, order='F')
+    table[0] = 2 * np.arange(1, nel + 1) - 1
+    table[1] = 2 * np.arange(1, nel + 1) + 1
+    table[2] = 2 * np.arange(1, nel + 1)
+    r1 = ds[0] / (2.0 * period)
+    r2 = ds[1] / (2.0 * period) if len(ds) > 1 else 0.0
+    if len(ds) > 1:
+        small = 0.5 - r1 - r2 if small_space is None else small_space
+        large = 0.5 - r1 - r2 if small_space is None else 1.0 - small_space - 2 * r1 - 2 * r2
+    type_el = np.ones(nel, dtype=np.int64)
+    for i in range(nel):
+        x1, x2 = x[2 * i], x[2 * i + 2]
+        dx = x2 - x1
+        if (x1 <= 0.5 + r1 and 0.5 - r1 - dx <= x1 and x2 <= 0.5 + r1 and 0.5 - r1 - dx <= x2):
+            type_el[i] = 2
+        elif len(ds) == 2 and split_touching_incs:
+            if ((x1 <= 0.5 + r1 + r2 and 0.5 + r1 - dx <= x1 and x2 <= 0.5 + r1 + r2 and 0.5 + r1 - dx <= x2)
+                    or (x1 <= 0.5 - r1 and 0.5 - r1 - r2 - dx <= x1 and x2 <= 0.5 - r1
+                        and 0.5 - r1 - r2 - dx <= x2)):
+                type_el[i] = 3
+        elif len(ds) == 2:
+            if (x1 <= 0.5 - r1 - small and x2 <= 0.5 - r1 - small) or (x1 >= 0.5 + large + r1
+                                                                        and x2 >= 0.5 + large + r1):
+                type_el[i] = 3
+    fmt = lambda v: ('%.6f' % v).rstrip('0').rstrip('.') if v != int(v) else str(int(v))
+    name = '1D_' + '_'.join(fmt(v) for v in [period] + ds)
+    return x, table, type_el, name

@@ -197,7 +197,26 @@ class Simmo(Modes):
         FEM_debug = 0
         st = self.structure
         if st.periodicity == '1D_array':
-            raise NotImplementedError("1D_array NanoStructs are not on this build's GPU path yet")
+            if st.world_1d is True:
+                world_1d, num_pw_per_pol_2d = 1, 1
+            else:
+                world_1d = 0
+                num_pw_per_pol_2d = self.calc_2d_grating_orders(self.max_order_PWs)[0].size
+            resm = EMUstack.calc_modes_1d(
+                self.wl_norm(), self.num_BMs, self.max_order_PWs, st.nb_typ_el, st.n_msh_pts, st.n_msh_el,
+                st.table_nod, st.type_el, st.x_arr, itermax, FEM_debug, st.mesh_file, self.n_effs,
+                self.k_pll_norm()[0], self.k_pll_norm()[1], shift, st.plotting_fields, st.plot_real,
+                st.plot_imag, st.plot_abs, num_pw_per_pol, num_pw_per_pol_2d, world_1d)
+            self.k_z, J, J_dag, J_2d, J_dag_2d, self.sol1 = resm
+            if st.world_1d is True:
+                self.J, self.J_dag = np.mat(J), np.mat(J_dag)
+            else:
+                self.J, self.J_dag = np.mat(J_2d), np.mat(J_dag_2d)
+            if st.plotting_fields != 1:
+                self.sol1 = None
+                self.n_effs = None
+                self.E_H_field = None
+            return
         mesh_path = paths.msh_path + st.mesh_file
         with open(mesh_path) as f:
             self.n_msh_pts, self.n_msh_el = [int(i) for i in f.readline().split()]

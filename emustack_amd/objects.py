@@ -121,10 +121,23 @@ class NanoStruct(object):
         self.plot_field_conc = plot_field_conc
 
     def make_mesh(self):
-        """Generate a .mail mesh with the project mesher (replaces the
-        reference's gmsh templates, objects.py:366-1052)."""
+        """Generate the mesh: a .mail file from the project mesher for 2D
+        arrays (replaces the reference's gmsh templates, objects.py:366-1052),
+        the in-memory element arrays for 1D gratings (objects.py:1054-1530)."""
+        if self.periodicity == '1D_array':
+            from . import mesher
+            ds = [self.diameter1] + [getattr(self, 'diameter%d' % i) for i in range(2, 7)]
+            x, table_nod, type_el, name = mesher.grating_1d(self.period, ds, self.lc, self.small_space,
+                                                            self.split_touching_incs)
+            self.n_msh_el = type_el.size
+            self.n_msh_pts = x.size
+            self.table_nod = table_nod
+            self.type_el = type_el
+            self.x_arr = x
+            self.mesh_file = name
+            return
         if self.periodicity != '2D_array':
-            raise NotImplementedError("only 2D_array NanoStructs are on this build's GPU path")
+            raise ValueError("periodicity must be '1D_array' or '2D_array'")
         if self.inc_shape not in ('circle', 'ellipse'):
             raise NotImplementedError("mesher supports 'circle' and 'ellipse' inclusions; "
                                       "pass make_mesh_now=False with a premade mesh_file")

@@ -118,6 +118,43 @@ int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch,
                             int64_t neq_PW, const double *opts, double *beta1,
                             double *overlap_J, double *overlap_J_dagger, int64_t *stats);
 
+/* 1D gratings.  Create a mesh handle from the arrays the reference passes to
+ * calc_modes_1d (py_calc_modes_1d.f:2-12, built by objects.py:1054-1530):
+ * x_P2[npt_P2] node coordinates in [0, 1], table_nod[3*nel] (element-major,
+ * 1-based: left, right, mid node), type_el[nel] (1-based material index).
+ * Requires npt_P2 = 2 nel + 1 (py_calc_modes_1d.f:186-191).  The handle
+ * caches the DOF numbering of periodic_cond_1d.f, the sparse pattern and
+ * the symbolic LU; free it with emu_mesh_free. */
+int emu_mesh_create_1d(emu_ctx *ctx, int64_t npt_P2, int64_t nel, const int64_t *table_nod,
+                       const int64_t *type_el, const double *x_P2, int leaf_size, emu_mesh **out);
+
+/* Replaces calc_modes_1d (py_calc_modes_1d.f:2-12, called at
+ * mode_calcs.py:333-342).  Same meaning as the reference arguments:
+ * lambda = wl / period, n_eff[2*nb_typ_el] (complex interleaved),
+ * bloch_vec_x / bloch_vec_y, shift[2] (complex), neq_PW = 2 ordre_ls + 1,
+ * neq_PW_2d = the 2D circular plane-wave count (used when world_1d = 0).
+ * Outputs: beta1[2*nval]; overlap_J[2*2neq_PW*nval] (F-order 2neq_PW x nval);
+ * overlap_J_dagger[2*nval*2neq_PW] (F-order nval x 2neq_PW); when
+ * world_1d = 0 also overlap_J_2d (2neq_PW_2d x nval) and overlap_J_dagger_2d
+ * (nval x 2neq_PW_2d), else these two may be NULL.  The E-field formulation
+ * is the only one (py_calc_modes_1d.f:146). */
+int emu_calc_modes_1d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval, int64_t ordre_ls,
+                      int64_t nb_typ_el, const double *n_eff, double bloch_vec_x, double bloch_vec_y,
+                      const double *shift, int64_t itermax, int64_t neq_PW, int64_t neq_PW_2d,
+                      int64_t world_1d, double *beta1, double *overlap_J, double *overlap_J_dagger,
+                      double *overlap_J_2d, double *overlap_J_dagger_2d);
+
+/* Batched form of emu_calc_modes_1d, instance-major like
+ * emu_calc_modes_2d_batch: bloch_vec[nbatch*2] = (x, y) per instance; the
+ * 2D outputs are nbatch blocks of 2*2neq_PW_2d*nval doubles.  opts and
+ * stats as for emu_calc_modes_2d_batch. */
+int emu_calc_modes_1d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *lambda,
+                            int64_t nval, int64_t ordre_ls, int64_t nb_typ_el, const double *n_eff,
+                            const double *bloch_vec, const double *shift, int64_t itermax,
+                            int64_t neq_PW, int64_t neq_PW_2d, int64_t world_1d, const double *opts,
+                            double *beta1, double *overlap_J, double *overlap_J_dagger,
+                            double *overlap_J_2d, double *overlap_J_dagger_2d, int64_t *stats);
+
 /* Live kernel timing with HIP events on the context's stream (benchmarks).
  * enable = 1 resets the counters and starts recording every `every`-th
  * solve sweep; enable = 0 stops.  out[k*3 + {0,1,2}] (may be NULL) for kernel

@@ -1,0 +1,121 @@
+"""GPU parity of the 1D-grating path (emu_calc_modes_1d) against the oracle
+(oracle/fem1d.py) on the stacked gratings of the reference's case 4
+(tests/test_case_4_stacked_gratings.py: period 760 nm, lambda 800 nm,
+max_order_PWs 6, 46 Bloch modes, 200 elements).  Tolerances: beta 1e-9
+relative; J J^dagger (mode-subspace projector, invariant to the per-mode
+normalisation) 1e-7 of its largest entry; R/T/A 1e-8 against the oracle; the
+2D embedding (world_1d = 0) equal to the oracle's."""
+import numpy as np
+import pytest
+
+from oracle import fem1d, stack_ref
+
+pytestmark = pytest.mark.gpu
+
+WL, PERIOD, ORDER = 800.0, 760.0, 6
+
+
+def _kp(wl=WL):
+    th = ph = 1e-5 * np.pi / 180
+    k = 2 * np.pi / wl
+    return k * np.sin(th) * np.array([np.cos(ph), np.sin(ph)]) * PERIOD
+
+
+def _case(diam, neff, wl=WL):
+    x, tn, te = fem1d.grating_mesh_1d(PERIOD, diam, 0.005)
+    wln = wl / PERIOD
+    k0 = 2 * np.pi / wln
+    kp = _kp(wl)
+    neff = np.asarray(neff, complex)
+    shift = 1.1 * np.real(neff).max() ** 2 * k0 ** 2 - kp @ kp
+    return x, tn, te, wln, kp, neff, shift
+
+
+@pytest.fixture(scope='module')
+def gratings(gpu_ctx):
+    from emustack_amd import _lib
+    out = {}
+    for name, diam, neff in (('g1', [190, 190], [3.61, 1.46, 1.46]), ('g2', [190], [3.61, 1.46])):
+        x, tn, te, wln, kp, neff, shift = _case(diam, neff)
+        mh = _lib.MeshHandle1D(x, tn, te, gpu_ctx)
+        g = gpu_ctx.calc_modes_1d_batch(mh, [wln], 46, ORDER, [neff], [kp], [shift], 2 * ORDER + 1)
+        m = fem1d.build_mesh_1d(x, tn, te)
+        o = fem1d.calc_modes_1d(wln, 46, ORDER, m, neff, kp[0], kp[1], shift)
+        out[name] = (g, o, (x, tn, te, wln, kp, neff, shift), mh)
+    return out
+
+
+def test_beta_matches_oracle(gratings):
+    for name, (g, o, _, _) in gratings.items():
+        beta, J, Jd, _, _, st = g
+        assert st[2] == 0, name
+        np.testing.assert_allclose(beta[0], o[0], rtol=1e-9, atol=1e-9 * np.abs(o[0]).max(), err_msg=name)
+
+
+def test_projectors_match_oracle(gratings):
+    for name, (g, o, _, _) in gratings.items():
+        beta, J, Jd, _, _, _ = g
+        Pg = J[0] @ Jd[0]
+        Po = o[1] @ o[2]
+        assert np.abs(Pg - Po).max() <= 1e-7 * np.abs(Po).max(), name
+
+
+def test_case4_rta_matches_oracle(gratings):
+    air = dict(kind='tf', n=1.0, h=None)
+    res = {}
+    for src in ('gpu', 'oracle'):
+        lays = []
+        for name, h in (('g1', 150.0), ('g2', 900.0)):
+            g, o, *_ = gratings[name]
+            b, J, Jd = (g[0][0], g[1][0], g[2][0]) if src == 'gpu' else o
+            lays.append(dict(kind='ns', kz=b, J=J, Jd=Jd, h=h / PERIOD))
+        r = stack_ref.calc_scat([air] + lays + [air], WL / PERIOD, _kp(), ORDER, pol='TE', world_1d=True)
+        res[src] = np.array([r['a_list'][-1], r['r_list'][-1], r['t_list'][-1]])
+    np.testing.assert_allclose(res['gpu'], res['oracle'], rtol=0, atol=1e-8)
+
+
+def test_world_2d_embedding(gpu_ctx, gratings):
+    g, o, (x, tn, te, wln, kp, neff, shift), mh = gratings['g2']
+    from oracle.stack_ref import grating_orders_2d
+    P2 = len(grating_orders_2d(ORDER)[0])
+    r = gpu_ctx.calc_modes_1d_batch(mh, [wln], 46, ORDER, [neff], [kp], [shift], 2 * ORDER + 1, neq_pw_2d=P2,
+                                    world_1d=False)
+    pxx = fem1d.build_mesh_1d(x, tn, te).period_x
+    J2o, Jd2o = fem1d.pw_1d_to_2d(r[1][0], r[2][0], ORDER, pxx, kp[0], kp[1], 2 * np.pi / wln)
+    np.testing.assert_array_equal(r[3][0], J2o)
+    np.testing.assert_array_equal(r[4][0], Jd2o)
+    assert np.count_nonzero(np.abs(r[3][0]).sum(axis=1)) <= 2 * (2 * ORDER + 1)
+
+
+def test_batched_wavelengths_equal_single(gpu_ctx, gratings):
+    _, _, (x, tn, te, wln, kp, neff, shift), mh = gratings['g2']
+    wls = [700.0, 900.0]
+    cases = [_case([190], [3.61, 1.46], wl) for wl in wls]
+    bb = gpu_ctx.calc_modes_1d_batch(mh, [c[3] for c in cases], 30, ORDER, [c[5] for c in cases],
+                                     [c[4] for c in cases], [c[6] for c in cases], 2 * ORDER + 1)
+    for i, c in enumerate(cases):
+        b1 = gpu_ctx.calc_modes_1d_batch(mh, [c[3]], 30, ORDER, [c[5]], [c[4]], [c[6]], 2 * ORDER + 1)
+        np.testing.assert_allclose(bb[0][i], b1[0][0], rtol=1e-11)
+
+
+def test_dropin_api_case4(gratings):
+    """tests/test_case_4_stacked_gratings.py through the drop-in objects,
+    against the oracle (the reference's case-4 text files are not reproduced
+    by the restatement; see tests/test_oracle_1d.py)."""
+    from emustack_amd import objects, materials, Stack
+    light = objects.Light(WL, max_order_PWs=ORDER, theta=0.0, phi=0.0)
+    sup = objects.ThinFilm(PERIOD, height_nm='semi_inf', world_1d=True, material=materials.Air, loss=False)
+    sub = objects.ThinFilm(PERIOD, height_nm='semi_inf', world_1d=True, material=materials.Air, loss=False)
+    g1 = objects.NanoStruct('1D_array', PERIOD, diameter1=190, diameter2=190, height_nm=150,
+                            inclusion_a=materials.Material(1.46 + 0.0j), inclusion_b=materials.Material(1.46 + 0.0j),
+                            background=materials.Material(3.61 + 0.0j), loss=True, lc_bkg=0.005)
+    g2 = objects.NanoStruct('1D_array', PERIOD, 190, height_nm=900, background=materials.Material(3.61 + 0.0j),
+                            inclusion_a=materials.Material(1.46 + 0.0j), loss=True, lc_bkg=0.005)
+    st = Stack((sub.calc_modes(light), g1.calc_modes(light), g2.calc_modes(light), sup.calc_modes(light)))
+    st.calc_scat(pol='TE')
+    air = dict(kind='tf', n=1.0, h=None)
+    lays = [dict(kind='ns', kz=gratings[n][1][0], J=gratings[n][1][1], Jd=gratings[n][1][2], h=h / PERIOD)
+            for n, h in (('g1', 150.0), ('g2', 900.0))]
+    r = stack_ref.calc_scat([air] + lays + [air], WL / PERIOD, light.k_pll * PERIOD, ORDER, pol='TE', world_1d=True)
+    np.testing.assert_allclose([float(st.r_list[-1]), float(st.t_list[-1]), float(st.a_list[-1])],
+                               [r['r_list'][-1], r['t_list'][-1], r['a_list'][-1]], rtol=0, atol=1e-8)
