@@ -19,12 +19,23 @@ _mesh_cache = {}
 _cache_lock = threading.Lock()
 
 
+_contexts = []
+
+
 def context(device=None):
     """The calling thread's GPU context (device from EMU_DEVICE, default 0)."""
     if getattr(_state, 'ctx', None) is None:
         dev = int(os.environ.get('EMU_DEVICE', '0')) if device is None else device
         _state.ctx = _lib.Context(dev)
+        with _cache_lock:
+            _contexts.append(_state.ctx)
     return _state.ctx
+
+
+def all_contexts():
+    """Every context created so far (one per host thread that used the GPU)."""
+    with _cache_lock:
+        return list(_contexts)
 
 
 def mesh(path, leaf_size=0):

@@ -1,7 +1,15 @@
 """Batched mode solves: many Light objects on one NanoStruct in one library
 call (the reference solves one wavelength per process, see
 tests/test_case_1_simple_struct.py:97-100).  Returns ordinary Simmo objects,
-so Stack.calc_scat is unchanged."""
+so Stack.calc_scat is unchanged.
+
+With groups > 1 the lights are split into that many groups, each solved by
+its own host thread on its own context (HIP stream and work buffers): while
+one group's projected Schur problems run on the host cores, another group's
+Krylov expansion runs on the GPU."""
+from concurrent.futures import ThreadPoolExecutor
+import threading
+
 import numpy as np
 
 from . import EMUstack, paths
@@ -9,33 +17,76 @@ from .mode_calcs import Simmo
 
 __all__ = ['calc_modes_many']
 
+_pools = {}
+_pool_lock = threading.Lock()
 
-def calc_modes_many(structure, lights, num_BMs=None, tol=0.0, itermax=30):
+
+def _pool(n):
+    with _pool_lock:
+        p = _pools.get(n)
+        if p is None:
+            p = ThreadPoolExecutor(max_workers=n, thread_name_prefix='emu-group')
+            _pools[n] = p
+        return p
+
+
+def _solve(structure, sims, inputs, nbm, order, P, tol, itermax):
+    st = structure
+    lam = [s.wl_norm() for s in sims]
+    n_effs = [x[0] for x in inputs]
+    kps = [s.k_pll_norm() for s in sims]
+    shifts = [x[3] for x in inputs]
+    if st.periodicity == '2D_array':
+        beta, J, Jd, stats = EMUstack.calc_modes_2d_batch(lam, nbm, order, paths.msh_path + st.mesh_file, n_effs,
+                                                          kps, shifts, P, tol=tol, itermax=itermax)
+        area_norm = st.period * st.period_y / st.period ** 2
+        return beta, [np.asmatrix(j) / area_norm for j in J], [np.asmatrix(j) for j in Jd], stats
+    h = EMUstack.mesh_1d(st.x_arr, st.table_nod, st.type_el)
+    w1 = st.world_1d is True
+    P2 = 1 if w1 else sims[0].calc_2d_grating_orders(order)[0].size
+    beta, J, Jd, J2, Jd2, stats = EMUstack.context().calc_modes_1d_batch(
+        h, lam, nbm, order, n_effs, kps, shifts, P, neq_pw_2d=P2, world_1d=w1, itermax=itermax, tol=tol)
+    if not w1:
+        J, Jd = J2, Jd2
+    return beta, [np.asmatrix(j) for j in J], [np.asmatrix(j) for j in Jd], stats
+
+
+def calc_modes_many(structure, lights, num_BMs=None, tol=0.0, itermax=30, groups=1):
     """Simmo objects for every light in `lights` (same max_order_PWs),
-    computed together on the GPU."""
-    if structure.periodicity != '2D_array':
-        raise NotImplementedError("batched path supports 2D_array NanoStructs")
+    computed together on the GPU (`groups` concurrent sub-batches)."""
+    if structure.periodicity not in ('1D_array', '2D_array'):
+        raise ValueError("NanoStruct layer must have periodicity of either '1D_array' or '2D_array'.")
     sims = [Simmo(structure, l) for l in lights]
     orders = {l.max_order_PWs for l in lights}
     if len(orders) != 1:
         raise ValueError("all lights of a batch need the same max_order_PWs")
+    order = orders.pop()
     inputs = [s.fem_inputs(num_BMs) for s in sims]
     nbms = {x[2] for x in inputs}
     if len(nbms) != 1:
         raise ValueError("all lights of a batch need the same num_BMs")
     nbm = nbms.pop()
     P = inputs[0][1]
-    beta, J, Jd, stats = EMUstack.calc_modes_2d_batch(
-        [s.wl_norm() for s in sims], nbm, orders.pop(), paths.msh_path + structure.mesh_file,
-        [x[0] for x in inputs], [s.k_pll_norm() for s in sims], [x[3] for x in inputs], P, tol=tol,
-        itermax=itermax)
-    area_norm = structure.period * structure.period_y / structure.period ** 2
-    for i, s in enumerate(sims):
-        s.num_BMs = nbm
-        s.k_z = beta[i]
-        s.J = np.mat(J[i]) / area_norm
-        s.J_dag = np.mat(Jd[i])
-        s.sol1 = s.mode_pol = s.table_nod = s.type_el = s.x_arr = None
+    groups = max(1, min(int(groups), len(sims)))
+    cuts = np.linspace(0, len(sims), groups + 1).round().astype(int)
+    parts = [(cuts[g], cuts[g + 1]) for g in range(groups)]
+    if groups == 1:
+        results = [_solve(structure, sims, inputs, nbm, order, P, tol, itermax)]
+    else:
+        futs = [_pool(groups).submit(_solve, structure, sims[a:b], inputs[a:b], nbm, order, P, tol, itermax)
+                for a, b in parts]
+        results = [f.result() for f in futs]
+    stats = np.zeros(8, np.int64)
+    for (a, b), (beta, J, Jd, st) in zip(parts, results):
+        stats[0] += st[0]
+        stats[1] = max(stats[1], st[1])
+        stats[2:] += st[2:]
+        for i, s in enumerate(sims[a:b]):
+            s.num_BMs = nbm
+            s.k_z = beta[i]
+            s.J = J[i]
+            s.J_dag = Jd[i]
+            s.sol1 = s.mode_pol = s.table_nod = s.type_el = s.x_arr = None
     calc_modes_many.last_stats = stats
     return sims
 

@@ -168,6 +168,7 @@ int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const 
     a.shift = shift;
     if (opts && opts[0] > 0) a.tol = opts[0];
     if (opts && opts[2] > 0) a.block = (int64_t)opts[2];
+    if (opts && opts[3] > 0) a.ncv = (int64_t)opts[3];
     if (a.block != 1 && a.block != 2 && a.block != 4 && a.block != 8)
       return fail(EMU_E_ARG, "Krylov block size must be 1, 2, 4 or 8");
     a.beta = beta1;
@@ -233,6 +234,7 @@ int emu_calc_modes_1d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const 
     a.shift = shift;
     if (opts && opts[0] > 0) a.tol = opts[0];
     if (opts && opts[2] > 0) a.block = (int64_t)opts[2];
+    if (opts && opts[3] > 0) a.ncv = (int64_t)opts[3];
     if (a.block != 1 && a.block != 2 && a.block != 4 && a.block != 8)
       return fail(EMU_E_ARG, "Krylov block size must be 1, 2, 4 or 8");
     a.beta = beta1;

@@ -196,14 +196,14 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
       }
       L.ea_cnt[sl] = ((int64_t)tasks.size() - L.ea_off[sl]) / 2;
     }
-    // solve row tiles
+    // solve row slabs (256 rows = 4 waves of 64 per workgroup)
     L.sf_off = (int64_t)tasks.size();
     for (int32_t t : LP.fronts)
-      for (int32_t r0 = 0; r0 < sy.fronts[t].f; r0 += 64) { tasks.push_back(t); tasks.push_back(r0); }
+      for (int32_t r0 = 0; r0 < sy.fronts[t].f; r0 += 256) { tasks.push_back(t); tasks.push_back(r0); }
     L.sf_cnt = ((int64_t)tasks.size() - L.sf_off) / 2;
     L.sb_off = (int64_t)tasks.size();
     for (int32_t t : LP.fronts)
-      for (int32_t r0 = 0; r0 < sy.fronts[t].s; r0 += 64) { tasks.push_back(t); tasks.push_back(r0); }
+      for (int32_t r0 = 0; r0 < sy.fronts[t].s; r0 += 256) { tasks.push_back(t); tasks.push_back(r0); }
     L.sb_cnt = ((int64_t)tasks.size() - L.sb_off) / 2;
     // algorithmic bytes of one solve sweep of this level (per instance):
     // triangle incl. diagonal + off-diagonal block of the inverted form once
@@ -429,6 +429,8 @@ struct KSState {
   std::vector<cplx> H;     // (m+bq) x m, column-major, ld = m+bq
   std::vector<cplx> T, Q;  // m x m Schur form and vectors
   std::vector<cplx> theta;
+  int nlock = 0;           // leading Schur vectors locked (their residual coupling is zero in H)
+  int nlock_next = 0;      // locked after this projection (applied at the restart)
 };
 
 // Schur-decompose H[:m,:m], order the `keep` largest |theta| first and the
@@ -436,33 +438,64 @@ struct KSState {
 // ||B Q y_i|| <= tol |theta_i| of the leading nev Ritz pairs (B = last bq rows
 // of H).  Optionally return W = Q[:, :nev] Y (Ritz vectors in the basis),
 // always or only when converged.
+//
+// Locking: the first st.nlock columns of H are a converged upper-triangular
+// Schur block whose coupling to the next block row was zeroed at the last
+// restart, so H = [T_LL X; 0 H_AA] and only the active block H_AA needs a
+// Schur decomposition (the expensive O(m^3) host step); T_LA = X Q_AA.  Schur
+// vectors of the wanted set whose own residual ||B q_i|| is below tol |theta_i|
+// (a leading run) are locked for the next cycle.
 static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol, std::vector<cplx> *ritzvec,
                        double *maxres, bool ritz_always = true) {
   const int ld = m + bq;
-  st.T.assign((size_t)m * m, 0);
-  for (int j = 0; j < m; ++j)
-    for (int i = 0; i < m; ++i) st.T[i + (size_t)j * m] = st.H[i + (size_t)j * ld];
-  st.Q.assign((size_t)m * m, 0);
-  st.theta.assign(m, 0);
-  int n = m, lda = m, sdim = 0, ldvs = m, info = 0, lwork = -1;
+  const int a0 = std::min(st.nlock, nev), na = m - a0;
+  std::vector<cplx> TA((size_t)na * na), QA((size_t)na * na);
+  for (int j = 0; j < na; ++j)
+    for (int i = 0; i < na; ++i) TA[i + (size_t)j * na] = st.H[(a0 + i) + (size_t)(a0 + j) * ld];
+  std::vector<cplx> thA(na);
+  int n = na, lda = na, sdim = 0, ldvs = na, info = 0, lwork = -1;
   char jobvs = 'V', sort = 'N';
-  std::vector<double> rwork(m);
-  std::vector<int> bwork(m);
+  std::vector<double> rwork(na);
+  std::vector<int> bwork(na);
   cplx wq;
-  g_zgees(&jobvs, &sort, dummy_select, &n, st.T.data(), &lda, &sdim, st.theta.data(), st.Q.data(), &ldvs, &wq,
-          &lwork, rwork.data(), bwork.data(), &info);
+  g_zgees(&jobvs, &sort, dummy_select, &n, TA.data(), &lda, &sdim, thA.data(), QA.data(), &ldvs, &wq, &lwork,
+          rwork.data(), bwork.data(), &info);
   lwork = std::max(1, (int)wq.real());
   std::vector<cplx> work(lwork);
-  g_zgees(&jobvs, &sort, dummy_select, &n, st.T.data(), &lda, &sdim, st.theta.data(), st.Q.data(), &ldvs,
-          work.data(), &lwork, rwork.data(), bwork.data(), &info);
+  g_zgees(&jobvs, &sort, dummy_select, &n, TA.data(), &lda, &sdim, thA.data(), QA.data(), &ldvs, work.data(),
+          &lwork, rwork.data(), bwork.data(), &info);
   if (info != 0) throw std::runtime_error("zgees failed, info=" + std::to_string(info));
+  st.T.assign((size_t)m * m, 0);
+  st.Q.assign((size_t)m * m, 0);
+  for (int j = 0; j < a0; ++j) {
+    for (int i = 0; i <= j; ++i) st.T[i + (size_t)j * m] = st.H[i + (size_t)j * ld];
+    st.Q[j + (size_t)j * m] = 1.0;
+  }
+  for (int j = 0; j < na; ++j) {
+    for (int i = 0; i < a0; ++i) {
+      cplx acc = 0;
+      for (int k = 0; k < na; ++k) acc += st.H[i + (size_t)(a0 + k) * ld] * QA[k + (size_t)j * na];
+      st.T[i + (size_t)(a0 + j) * m] = acc;
+    }
+    for (int i = 0; i < na; ++i) {
+      st.T[(a0 + i) + (size_t)(a0 + j) * m] = TA[i + (size_t)j * na];
+      st.Q[(a0 + i) + (size_t)(a0 + j) * m] = QA[i + (size_t)j * na];
+    }
+  }
+  st.theta.assign(m, 0);
+  for (int i = 0; i < m; ++i) st.theta[i] = st.T[i + (size_t)i * m];
+  n = m;
+  lda = m;
+  ldvs = m;
   auto reorder = [&](int count) {
-    std::vector<int> idx(m);
-    std::iota(idx.begin(), idx.end(), 0);
+    // locked columns stay selected (and first); the rest by largest |theta|
+    std::vector<int> idx(na);
+    std::iota(idx.begin(), idx.end(), a0);
     std::stable_sort(idx.begin(), idx.end(),
                      [&](int a, int b) { return std::abs(st.theta[a]) > std::abs(st.theta[b]); });
     std::vector<int> sel(m, 0);
-    for (int i = 0; i < count; ++i) sel[idx[i]] = 1;
+    for (int i = 0; i < a0; ++i) sel[i] = 1;
+    for (int i = 0; i < count - a0; ++i) sel[idx[i]] = 1;
     char job = 'N', compq = 'V';
     int mm = 0, lw = std::max(1, m), inf = 0;
     double sv = 0, sep = 0;
@@ -493,6 +526,14 @@ static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol
       if (qv == 0.0) continue;
       for (int r = 0; r < bq; ++r) BQ[r + (size_t)j * bq] += st.H[(m + r) + (size_t)i * ld] * qv;
     }
+  // locking candidates: leading run of Schur vectors with small own residual
+  st.nlock_next = 0;
+  for (int k = 0; k < nev; ++k) {
+    double r2 = 0;
+    for (int r = 0; r < bq; ++r) r2 += std::norm(BQ[r + (size_t)k * bq]);
+    if (!(std::sqrt(r2) <= tol * std::abs(st.theta[k]))) break;
+    st.nlock_next = k + 1;
+  }
   bool conv = true;
   double worst = 0;
   for (int k = 0; k < nev; ++k) {
@@ -694,10 +735,12 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
         }
         for (int r = 0; r < bq; ++r) {
           cplx acc = 0;
-          for (int i = 0; i < m; ++i) acc += S.H[(m + r) + (size_t)i * ld] * S.Q[i + (size_t)j * m];
+          if (j >= S.nlock_next)  // locked: coupling deflated to zero
+            for (int i = 0; i < m; ++i) acc += S.H[(m + r) + (size_t)i * ld] * S.Q[i + (size_t)j * m];
           Hn[(size_t)b * sH + (keep + r) + (size_t)j * ld] = zmk(acc.real(), acc.imag());
         }
       }
+      S.nlock = S.nlock_next;
     }
     EMU_HIP(hipMemcpyAsync(Qd, Qh.data(), sizeof(zd) * Qh.size(), hipMemcpyHostToDevice, s));
     EMU_HIP(hipMemcpyAsync(Hd, Hn.data(), sizeof(zd) * Hn.size(), hipMemcpyHostToDevice, s));
@@ -925,7 +968,7 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   // Krylov dimension: ARPACK's nvect = 2 nval + nval/2 + 3 (py_calc_modes.f:256),
   // rounded up to a whole number of blocks
   const int bq = (int)a.block;
-  int m = 2 * nval + nval / 2 + 3;
+  int m = a.ncv > 0 ? (int)a.ncv : 2 * nval + nval / 2 + 3;
   m = ((m + bq - 1) / bq) * bq;
   if (m + bq >= n || nval >= m) throw std::invalid_argument("calc_modes: num_BMs too large for the mesh");
   std::vector<cplx> pp, qq, sh;
@@ -1113,7 +1156,7 @@ void calc_modes_1d(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   const int nl = (int)a.nl, nb = 2 * nl, ntyp = (int)a.ntyp, nval = (int)a.nval;
   if (nl < 1 || nval < 1 || ntyp < dm.ntyp) throw std::invalid_argument("calc_modes_1d: bad sizes");
   const int bq = (int)a.block;
-  int m = 2 * nval + nval / 2 + 3;  // nvect of py_calc_modes_1d.f:145
+  int m = a.ncv > 0 ? (int)a.ncv : 2 * nval + nval / 2 + 3;  // nvect of py_calc_modes_1d.f:145
   m = ((m + bq - 1) / bq) * bq;
   if (m + bq >= n || nval >= m) throw std::invalid_argument("calc_modes_1d: num_BMs too large for the mesh");
   const int P = 2 * (int)a.ordre + 1;

@@ -47,6 +47,7 @@ struct ModesArgs {
   const double *shift = nullptr;   // nl complex
   double tol = 1e-12;
   int64_t block = 8;               // Krylov block size (1, 2, 4 or 8)
+  int64_t ncv = 0;                 // Krylov dimension (0: the reference's 2 nval + nval/2 + 3)
   double *beta = nullptr;          // nl x nval complex
   double *J = nullptr;             // nl x (2P x nval) complex
   double *Jd = nullptr;            // nl x (nval x 2P) complex

@@ -5,26 +5,58 @@
 namespace emu {
 
 // --------------------------------------------------------------- SpMM (CSR)
+// y[b][c][r] = sum_k A[b][r][k] x[b][c][k] for a block of SM_C columns.
+// Eight lanes share a row: each lane walks every eighth nonzero (coalesced
+// value / index reads, one read of the row per column block), accumulates
+// the SM_C column sums, and the eight partial sums are combined with
+// cross-lane shuffles; lane j then writes column j.  A wavefront covers 8
+// rows, a workgroup 32.
+constexpr int SM_C = 8;
+
+__device__ __forceinline__ zd shfl_xor_z(zd v, int m) {
+  return zmk(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+
 __global__ __launch_bounds__(256) void k_spmm(int64_t n, int ncol, const int64_t *__restrict__ rp,
                                               const int32_t *__restrict__ ci, const zd *__restrict__ vals,
                                               int64_t vstride, const zd *__restrict__ x, int64_t xstride,
                                               zd *__restrict__ y, int64_t ystride) {
-  int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  int b = blockIdx.y;
-  if (r >= n) return;
+  const int lane8 = threadIdx.x & 7;
+  const int64_t r = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 3);
+  const int b = blockIdx.y;
+  const int c0 = blockIdx.z * SM_C;
+  const int nc = min(SM_C, ncol - c0);
   const zd *v = vals + b * vstride;
-  const int64_t k0 = rp[r], k1 = rp[r + 1];
-  for (int c = 0; c < ncol; ++c) {
-    const zd *xx = x + b * xstride + (int64_t)c * n;
-    zd acc = zmk(0, 0);
-    for (int64_t k = k0; k < k1; ++k) acc = zfma(v[k], xx[ci[k]], acc);
-    y[b * ystride + (int64_t)c * n + r] = acc;
+  const zd *xb = x + b * xstride + (int64_t)c0 * n;
+  zd acc[SM_C];
+#pragma unroll
+  for (int c = 0; c < SM_C; ++c) acc[c] = zmk(0, 0);
+  if (r < n) {
+    const int64_t k0 = rp[r], k1 = rp[r + 1];
+    for (int64_t k = k0 + lane8; k < k1; k += 8) {
+      const zd a = v[k];
+      const int64_t col = ci[k];
+#pragma unroll
+      for (int c = 0; c < SM_C; ++c)
+        if (c < nc) acc[c] = zfma(a, xb[(int64_t)c * n + col], acc[c]);
+    }
   }
+#pragma unroll
+  for (int c = 0; c < SM_C; ++c) {
+    acc[c] = zadd(acc[c], shfl_xor_z(acc[c], 1));
+    acc[c] = zadd(acc[c], shfl_xor_z(acc[c], 2));
+    acc[c] = zadd(acc[c], shfl_xor_z(acc[c], 4));
+  }
+  zd out = acc[0];
+#pragma unroll
+  for (int c = 1; c < SM_C; ++c)
+    if (lane8 == c) out = acc[c];
+  if (r < n && lane8 < nc) y[b * ystride + (int64_t)(c0 + lane8) * n + r] = out;
 }
 
 void launch_spmm(int nb, int64_t n, int ncol, const int64_t *rp, const int32_t *ci, const zd *vals,
                  int64_t vstride, const zd *x, int64_t xstride, zd *y, int64_t ystride, hipStream_t s) {
-  dim3 g((unsigned)((n + 255) / 256), nb);
+  dim3 g((unsigned)((n + 31) / 32), nb, (unsigned)((ncol + SM_C - 1) / SM_C));
   hipLaunchKernelGGL(k_spmm, g, dim3(256), 0, s, n, ncol, rp, ci, vals, vstride, x, xstride, y, ystride);
   EMU_LAUNCH_CHECK();
 }

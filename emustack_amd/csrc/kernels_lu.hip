@@ -470,16 +470,18 @@ void lu_invert(DevMesh &dm, int nb, const zd *pool, zd *inv, hipStream_t s) {
 
 // ---------------------------------------------------------------- solves
 //
-// One workgroup per (front row tile of RT rows, instance, chunk of RC
-// right-hand sides).  The front's input vector is staged through LDS in
-// chunks of KC entries, gathered on the fly (forward: b at the pivot rows in
-// pivot order plus the children's update vectors; backward: y at the own
-// rows, x at the ancestors' rows); the four waves split each chunk and
-// reduce at the end.  Row tiles of one level never write what another
-// reads, so a level is one launch.
+// One workgroup per (256-row slab of a front, instance, chunk of RC
+// right-hand sides); each of the four wavefronts owns 64 consecutive rows and
+// runs over the whole column range of its rows, so no cross-wave reduction
+// is needed.  The front's input vector is staged through LDS in chunks of KC
+// entries, gathered on the fly (forward: b at the pivot rows in pivot order
+// plus the children's update vectors; backward: y at the own rows, x at the
+// ancestors' rows) and shared by the four waves.  Structural zeros of the
+// triangular inverses are skipped per wave.  Slabs of one level never write
+// what another reads, so a level is one launch.
 
-constexpr int RT = 64;
-constexpr int KC = 64;
+constexpr int RT = 256;  // rows per workgroup (4 waves x 64)
+constexpr int KC = 64;   // staged input chunk
 
 template <int RC>
 __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, FrontArrays A,
@@ -489,17 +491,17 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
   const int t = tasks[2 * blockIdx.x], r0 = tasks[2 * blockIdx.x + 1];
   const int b = blockIdx.y;
   const int q0 = blockIdx.z * RC;
-  const int tid = threadIdx.x, lr = tid & 63, kg = tid >> 6;
+  const int tid = threadIdx.x, wave = tid >> 6;
   const int s = A.s[t], f = A.f[t], fb = A.b[t];
   const zd *Fw = inv + b * isz + A.ioff[t];
   const int32_t *sg = sig + b * npiv + A.pivoff[t];
   const int32_t *cm = A.cmap + 2 * A.fvoff[t];
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
-  const int row = r0 + lr;
-  const int kend = min(s, r0 + RT);
+  const int row = r0 + tid;
+  const int kend = min(s, r0 + RT);                      // workgroup's column range
+  const int kend_w = min(s, r0 + 64 * (wave + 1));       // lower-triangular rows need k <= row
   __shared__ zd xin[KC][RC];
-  __shared__ zd red[3][RT][RC];
   zd acc[RC];
 #pragma unroll
   for (int q = 0; q < RC; ++q) acc[q] = zmk(0, 0);
@@ -518,11 +520,11 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
       xin[k][q] = v;
     }
     __syncthreads();
-    const int kl = min(KC, kend - kc);
-    if (row < f) {
+    const int kl = min(KC, kend_w - kc);
+    if (row < f && kl > 0) {
       const zd *col = Fw + row + (int64_t)kc * f;
-#pragma unroll 4
-      for (int k = kg; k < kl; k += 4) {
+#pragma unroll 8
+      for (int k = 0; k < kl; ++k) {
         const zd a = col[(int64_t)k * f];
 #pragma unroll
         for (int q = 0; q < RC; ++q) acc[q] = zfma(a, xin[k][q], acc[q]);
@@ -530,13 +532,9 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
     }
     __syncthreads();
   }
-  if (kg > 0)
-#pragma unroll
-    for (int q = 0; q < RC; ++q) red[kg - 1][lr][q] = acc[q];
-  __syncthreads();
-  if (kg == 0 && row < f) {
+  if (row < f) {
     for (int q = 0; q < nq; ++q) {
-      zd v = zadd(zadd(acc[q], red[0][lr][q]), zadd(red[1][lr][q], red[2][lr][q]));
+      zd v = acc[q];
       if (row < s) {
         y[(col0 + q) * n + fb + row] = v;
       } else {
@@ -556,15 +554,15 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, 
   const int t = tasks[2 * blockIdx.x], r0 = tasks[2 * blockIdx.x + 1];
   const int b = blockIdx.y;
   const int q0 = blockIdx.z * RC;
-  const int tid = threadIdx.x, lr = tid & 63, kg = tid >> 6;
+  const int tid = threadIdx.x, wave = tid >> 6;
   const int s = A.s[t], f = A.f[t], fb = A.b[t];
   const zd *Bw = inv + b * isz + A.ioff[t] + (int64_t)f * s;
   const int32_t *ui = A.uidx + A.uoff[t];
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
-  const int row = r0 + lr;
+  const int row = r0 + tid;
+  const int kbeg_w = r0 + 64 * wave;  // upper-triangular rows need k >= row
   __shared__ zd xin[KC][RC];
-  __shared__ zd red[3][RT][RC];
   zd acc[RC];
 #pragma unroll
   for (int q = 0; q < RC; ++q) acc[q] = zmk(0, 0);
@@ -577,11 +575,11 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, 
       xin[k][q] = v;
     }
     __syncthreads();
-    const int kl = min(KC, f - kc);
-    if (row < s) {
+    const int klo = max(0, kbeg_w - kc), khi = min(KC, f - kc);
+    if (row < s && klo < khi) {
       const zd *col = Bw + row + (int64_t)kc * s;
-#pragma unroll 4
-      for (int k = kg; k < kl; k += 4) {
+#pragma unroll 8
+      for (int k = klo; k < khi; ++k) {
         const zd a = col[(int64_t)k * s];
 #pragma unroll
         for (int q = 0; q < RC; ++q) acc[q] = zfma(a, xin[k][q], acc[q]);
@@ -589,13 +587,8 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, 
     }
     __syncthreads();
   }
-  if (kg > 0)
-#pragma unroll
-    for (int q = 0; q < RC; ++q) red[kg - 1][lr][q] = acc[q];
-  __syncthreads();
-  if (kg == 0 && row < s)
-    for (int q = 0; q < nq; ++q)
-      x[(col0 + q) * n + fb + row] = zadd(zadd(acc[q], red[0][lr][q]), zadd(red[1][lr][q], red[2][lr][q]));
+  if (row < s)
+    for (int q = 0; q < nq; ++q) x[(col0 + q) * n + fb + row] = acc[q];
 }
 
 static int pick_rc(int nrhs) {

@@ -46,14 +46,14 @@ class Modes(object):
         return px_m[keep], py_m[keep]
 
     def prop_fwd(self, height_norm):
-        return np.mat(np.diag(np.exp(1j * self.k_z * height_norm)))
+        return np.asmatrix(np.diag(np.exp(1j * self.k_z * height_norm)))
 
     def shear_transform(self, coords):
         alphas = np.append(self.air_ref().alphas, self.air_ref().alphas)
         if np.shape(coords) == (1,):
-            return np.mat(np.diag(np.exp(1j * (alphas * coords[0]))))
+            return np.asmatrix(np.diag(np.exp(1j * (alphas * coords[0]))))
         betas = np.append(self.air_ref().betas, self.air_ref().betas)
-        return np.mat(np.diag(np.exp(1j * (alphas * coords[0] + betas * coords[1]))))
+        return np.asmatrix(np.diag(np.exp(1j * (alphas * coords[0] + betas * coords[1]))))
 
     def __del__(self):
         if _interfaces_i_have_known is not None:
@@ -137,7 +137,7 @@ class Anallo(Modes):
         """Unit-amplitude zeroth-order incidence vector (mode_calcs.py:220-250)."""
         te = self.specular_order
         tm = self.specular_order + self.structure.num_pw_per_pol
-        inc = np.mat(np.zeros(self.structure.num_pw_per_pol * 2, dtype='complex128')).T
+        inc = np.asmatrix(np.zeros(self.structure.num_pw_per_pol * 2, dtype='complex128')).T
         r2 = 1 / np.sqrt(2.)
         amps = {'TE': ((te, 1),), 'TM': ((tm, 1),), 'un': ((te, r2), (tm, r2)),
                 'R Circ': ((te, r2), (tm, 1j * r2)), 'L Circ': ((te, r2), (tm, -1j * r2))}
@@ -209,9 +209,9 @@ class Simmo(Modes):
                 st.plot_imag, st.plot_abs, num_pw_per_pol, num_pw_per_pol_2d, world_1d)
             self.k_z, J, J_dag, J_2d, J_dag_2d, self.sol1 = resm
             if st.world_1d is True:
-                self.J, self.J_dag = np.mat(J), np.mat(J_dag)
+                self.J, self.J_dag = np.asmatrix(J), np.asmatrix(J_dag)
             else:
-                self.J, self.J_dag = np.mat(J_2d), np.mat(J_dag_2d)
+                self.J, self.J_dag = np.asmatrix(J_2d), np.asmatrix(J_dag_2d)
             if st.plotting_fields != 1:
                 self.sol1 = None
                 self.n_effs = None
@@ -228,7 +228,7 @@ class Simmo(Modes):
             num_pw_per_pol, cmplx_max, real_max, int_max)
         self.k_z, J, J_dag, self.sol1, self.mode_pol, self.table_nod, self.type_el, self.x_arr = resm
         area_norm = st.period * st.period_y / st.period ** 2
-        self.J, self.J_dag = np.mat(J) / area_norm, np.mat(J_dag)
+        self.J, self.J_dag = np.asmatrix(J) / area_norm, np.asmatrix(J_dag)
         if not st.plot_field_conc:
             self.mode_pol = None
         if st.plotting_fields != 1:
@@ -271,8 +271,8 @@ def r_t_mat_anallo(an1, an2):
     if len(an1.k_z) != len(an2.k_z):
         raise ValueError("Need the same number of plane waves in Anallos %s and %s" % (an1, an2))
     Z1, Z2 = an1.Z(), an2.Z()
-    R12 = np.mat(np.diag((Z2 - Z1) / (Z2 + Z1)))
-    T12 = np.mat(np.diag(2. * np.sqrt(Z2) * np.sqrt(Z1) / (Z2 + Z1)))
+    R12 = np.asmatrix(np.diag((Z2 - Z1) / (Z2 + Z1)))
+    T12 = np.asmatrix(np.diag(2. * np.sqrt(Z2) * np.sqrt(Z1) / (Z2 + Z1)))
     return R12, T12, -R12, T12
 
 
@@ -283,12 +283,12 @@ def r_t_mat_tf_ns(an1, sim2):
     if np.shape(zs)[1] != np.shape(sim2.J.A)[0]:
         raise ValueError("Scattering matrices of layers are not consistent,\n"
                          "some layers are 1D and others 2D. Check that world_1d status.")
-    A = np.mat(zs.T * sim2.J.A)
-    B = np.mat(sim2.J_dag.A * zs)
+    A = np.asmatrix(zs.T * sim2.J.A)
+    B = np.asmatrix(sim2.J_dag.A * zs)
     D = np.eye(len(B)) + B.dot(A)
     DB = np.linalg.solve(D, B)
     R12 = -np.eye(len(A)) + 2 * A * DB
     T12 = 2 * DB
     R21 = np.linalg.solve(D, (np.eye(len(B)) - B * A))
     T21 = 2 * A * D.I
-    return np.mat(R12), np.mat(T12), np.mat(R21), np.mat(T21)
+    return np.asmatrix(R12), np.asmatrix(T12), np.asmatrix(R21), np.asmatrix(T21)

@@ -108,7 +108,8 @@ int emu_calc_modes_2d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval,
  * overlap_J[nbatch*2*2neq_PW*nval], overlap_J_dagger[nbatch*2*nval*2neq_PW].
  * opts (may be NULL; 4 entries): opts[0] = eigen tolerance (<= 0: default
  * 1e-12); opts[1] = precision (0 = complex128 everywhere); opts[2] = Krylov
- * block size (1, 2, 4 or 8; <= 0: default 8); opts[3] reserved (0).  stats[0..7] (may be NULL): OP applications,
+ * block size (1, 2, 4 or 8; <= 0: default 8); opts[3] = Krylov dimension
+ * (<= 0: the reference's 2 nval + nval/2 + 3, rounded up to whole blocks).  stats[0..7] (may be NULL): OP applications,
  * restarts (max over instances), pivot perturbations, Arnoldi steps,
  * 0, 0, 0, 0. */
 int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch,

@@ -126,7 +126,7 @@ def test_batched_equals_single(gpu_ctx, mesh600, mats):
                                               [p[4] for p in P], 5)
     for i, p in enumerate(P):
         b1, J1, Jd1, _ = gpu_ctx.calc_modes_batch(mh, [p[0]], 20, 1, [p[2]], [p[3]], [p[4]], 5)
-        np.testing.assert_allclose(bb[i], b1[0], rtol=1e-11)
+        np.testing.assert_allclose(bb[i], b1[0], rtol=1e-9)
         np.testing.assert_allclose(JJ[i] @ JJd[i], J1[0] @ Jd1[0], rtol=0, atol=1e-8 * np.abs(J1[0] @ Jd1[0]).max())
 
 
