@@ -11,8 +11,9 @@ polarisation, S-matrix dimension 98); 512 wavelengths in 400-1000 nm at
 normal incidence.  Data are synthetic.
 
 A step solves the modes of all three layers for a batch of wavelength
-points (prime and adjoint problems, one batched library call per layer)
-and runs the star-product cascade with fluxes for every point.  `value` is
+points (prime and adjoint problems, batched library calls per layer) and
+runs the star-product cascade with fluxes for every point (one batched
+cascade on the GPU).  `value` is
 wavelength-points per second.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
@@ -71,18 +72,16 @@ def run_step(ns, film, sub, sup, wls, groups=1):
     """All three layers for the wavelength batch `wls` (`groups` concurrent
     sub-batches per layer), then the cascade."""
     from emustack_amd import objects, Stack
-    from emustack_amd.batch import calc_modes_many
+    from emustack_amd.batch import calc_modes_many, calc_scat_many
     lights = [objects.Light(wl, max_order_PWs=ORDER, theta=0.0, phi=0.0) for wl in wls]
     sims, ops = [], 0
     for s in ns:
         sims.append(calc_modes_many(s, lights, num_BMs=NUM_BMS, groups=groups))
         ops += int(calc_modes_many.last_stats[0])
-    out = []
-    for i, light in enumerate(lights):
-        st = Stack((sub.calc_modes(light), sims[0][i], film.calc_modes(light), sims[1][i],
-                    film.calc_modes(light), sims[2][i], sup.calc_modes(light)))
-        st.calc_scat(pol='TE')
-        out.append((float(st.a_list[-1]), float(st.r_list[-1]), float(st.t_list[-1])))
+    stacks = [Stack((sub.calc_modes(light), sims[0][i], film.calc_modes(light), sims[1][i],
+                     film.calc_modes(light), sims[2][i], sup.calc_modes(light))) for i, light in enumerate(lights)]
+    calc_scat_many(stacks, pol='TE')  # batched star-product cascade on the GPU
+    out = [(float(st.a_list[-1]), float(st.r_list[-1]), float(st.t_list[-1])) for st in stacks]
     return out, ops
 
 

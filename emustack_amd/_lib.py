@@ -38,7 +38,7 @@ EXPORTS = [
     'emu_last_error', 'emu_ctx_create', 'emu_ctx_destroy', 'emu_set_lapack', 'emu_mesh_load',
     'emu_mesh_analyze', 'emu_mesh_symbolic', 'emu_mesh_free', 'emu_mesh_info', 'emu_mesh_arrays',
     'emu_mesh_geometry', 'emu_calc_modes_2d', 'emu_calc_modes_2d_batch', 'emu_assemble', 'emu_lu_solve',
-    'emu_profile', 'emu_mesh_create_1d', 'emu_calc_modes_1d', 'emu_calc_modes_1d_batch',
+    'emu_profile', 'emu_mesh_create_1d', 'emu_calc_modes_1d', 'emu_calc_modes_1d_batch', 'emu_star_cascade_batch',
 ]
 
 
@@ -115,6 +115,9 @@ def lib():
         L.emu_calc_modes_1d.argtypes = [c_vp, c_vp, ctypes.c_double, c_i64, c_i64, c_i64, c_dp, ctypes.c_double,
                                         ctypes.c_double, c_dp, c_i64, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp,
                                         c_dp]
+        c_dpp = ctypes.POINTER(c_dp)
+        L.emu_star_cascade_batch.argtypes = [c_vp, c_i64, c_i64, c_i64, c_i64p, c_i64p, c_dpp, c_dpp, c_dpp, c_dpp,
+                                             c_dp, c_dp, c_dp, c_i64p, c_dp, c_dp, c_dp, c_dp, c_dp]
         L.emu_calc_modes_1d_batch.argtypes = [c_vp, c_vp, c_i64, c_dp, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp,
                                               c_i64, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp,
                                               c_i64p]
@@ -229,6 +232,53 @@ class Context:
                 ptr(st, c_i64p)))
         out2 = (J2.transpose(0, 2, 1), Jd2.transpose(0, 2, 1)) if w2 else (None, None)
         return beta, J.transpose(0, 2, 1), Jd.transpose(0, 2, 1), out2[0], out2[1], st
+
+    def star_cascade(self, kinds, kz, Z, J, Jd, heights, Zair, d_minus, nprop, want_nets=False):
+        """Batched star-product cascade (emu_star_cascade_batch).  Per layer
+        (bottom to top) arrays with a leading batch axis: kz[l] (nb, dim),
+        Z[l] (nb, 2P) for films, J[l] (nb, 2P, modes), Jd[l] (nb, modes, 2P)
+        for structured layers (None elsewhere).  Returns a_list (nb, L-1),
+        r_list (nb,), t_list (nb, L-1) and, if asked, R_net, T_net."""
+        L = len(kinds)
+        nb = np.asarray(Zair).shape[0]
+        d = np.asarray(Zair).shape[1]
+        keep = []
+
+        def cptr(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.complex128)
+            keep.append(a)
+            return a.view(np.float64).ctypes.data_as(c_dp)
+
+        def fptr(a):  # (nb, r, c) -> per point F-order
+            if a is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(a, np.complex128).transpose(0, 2, 1))
+            keep.append(a)
+            return a.view(np.float64).ctypes.data_as(c_dp)
+        arr = lambda xs: (c_dp * L)(*xs)
+        kz_p = arr([cptr(k) for k in kz])
+        Z_p = arr([cptr(z) for z in Z])
+        J_p = arr([fptr(j) for j in J])
+        Jd_p = arr([fptr(j) for j in Jd])
+        kinds = np.asarray(kinds, np.int64)
+        nmodes = np.array([np.asarray(k).shape[1] for k in kz], np.int64)
+        heights = np.ascontiguousarray(heights, np.float64)
+        nprop = np.ascontiguousarray(nprop, np.int64)
+        a_l = np.zeros((nb, L - 1))
+        r_l = np.zeros(nb)
+        t_l = np.zeros((nb, L - 1))
+        R = np.zeros((nb, d, d), np.complex128) if want_nets else None
+        T = np.zeros((nb, d, d), np.complex128) if want_nets else None
+        check(lib().emu_star_cascade_batch(self.h, nb, L, d // 2, ptr(kinds, c_i64p), ptr(nmodes, c_i64p), kz_p,
+                                           Z_p, J_p, Jd_p, ptr(heights), cptr(Zair), cptr(d_minus),
+                                           ptr(nprop, c_i64p), ptr(R.view(np.float64)) if want_nets else None,
+                                           ptr(T.view(np.float64)) if want_nets else None, ptr(a_l), ptr(r_l),
+                                           ptr(t_l)))
+        if want_nets:  # stored per point column-major
+            R, T = R.transpose(0, 2, 1), T.transpose(0, 2, 1)
+        return a_l, r_l, t_l, R, T
 
     PROFILE_CLASSES = ('solve_fwd', 'solve_bwd', 'gs_project', 'gs_update', 'spmv_M')
 

@@ -15,7 +15,7 @@ import numpy as np
 from . import EMUstack, paths
 from .mode_calcs import Simmo
 
-__all__ = ['calc_modes_many']
+__all__ = ['calc_modes_many', 'calc_scat_many']
 
 _pools = {}
 _pool_lock = threading.Lock()
@@ -92,3 +92,52 @@ def calc_modes_many(structure, lights, num_BMs=None, tol=0.0, itermax=30, groups
 
 
 calc_modes_many.last_stats = None
+
+
+def calc_scat_many(stacks, pol='TE', want_nets=False):
+    """Stack.calc_scat (stack.py:162-427, shears = None, fluxes on) for many
+    stacks of identical layer structure at once, as a batched star-product
+    cascade on the GPU.  Fills a_list, r_list, t_list (and R_net, T_net when
+    want_nets) on every stack, as Stack.calc_scat does.  The bottom and top
+    layers must be thin films."""
+    from .mode_calcs import Anallo
+    if not stacks:
+        return stacks
+    first = stacks[0]
+    L = len(first.layers)
+    kinds = [0 if isinstance(l, Anallo) else 1 for l in first.layers]
+    if kinds[0] != 0 or kinds[-1] != 0:
+        raise ValueError("calc_scat_many needs thin films as top and bottom layers")
+    for st in stacks:
+        if [0 if isinstance(l, Anallo) else 1 for l in st.layers] != kinds:
+            raise ValueError("all stacks of a batch need the same layer structure")
+        if st.shears is not None:
+            raise NotImplementedError("sheared stacks are not batched")
+    kz, Z, J, Jd = [], [], [], []
+    for li in range(L):
+        lays = [st.layers[li] for st in stacks]
+        if kinds[li] == 0:
+            kz.append(np.array([l.k_z for l in lays]))  # both polarisations (Anallo.calc_modes)
+            Z.append(np.array([l.Z() for l in lays]))
+            J.append(None)
+            Jd.append(None)
+        else:
+            kz.append(np.array([np.asarray(l.k_z).reshape(-1) for l in lays]))
+            Z.append(None)
+            J.append(np.array([np.asarray(l.J) for l in lays]))
+            Jd.append(np.array([np.asarray(l.J_dag) for l in lays]))
+    heights = [0.0] + [h / first.period for h in first.heights_nm()] + [0.0]
+    top = [st.layers[-1] for st in stacks]
+    Zair = np.array([t.air_ref().Z() for t in top])
+    d_minus = np.array([np.asarray(t.specular_incidence(pol)).reshape(-1) for t in top])
+    nprop = np.array([[t.air_ref().num_prop_pw_per_pol, t.num_prop_pw_per_pol,
+                       st.layers[0].num_prop_pw_per_pol] for t, st in zip(top, stacks)], np.int64)
+    a_l, r_l, t_l, R, T = EMUstack.context().star_cascade(kinds, kz, Z, J, Jd, heights, Zair, d_minus, nprop,
+                                                         want_nets=want_nets)
+    for i, st in enumerate(stacks):
+        st.a_list = list(a_l[i])
+        st.r_list = [r_l[i]]
+        st.t_list = list(t_l[i])
+        if want_nets:
+            st.R_net, st.T_net = np.asmatrix(R[i]), np.asmatrix(T[i])
+    return stacks

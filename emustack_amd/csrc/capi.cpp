@@ -260,6 +260,46 @@ int emu_calc_modes_1d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval,
                                  overlap_J_2d, overlap_J_dagger_2d, nullptr);
 }
 
+int emu_star_cascade_batch(emu_ctx *ctx, int64_t nbatch, int64_t nlayers, int64_t neq_PW, const int64_t *kind,
+                           const int64_t *nmodes, const double *const *kz, const double *const *Z,
+                           const double *const *J, const double *const *J_dagger, const double *heights,
+                           const double *Zair, const double *d_minus, const int64_t *nprop, double *R_net,
+                           double *T_net, double *a_list, double *r_list, double *t_list) {
+  if (!ctx || !kind || !nmodes || !kz || !heights || !Zair || !d_minus || !nprop || !a_list || !r_list || !t_list)
+    return fail(EMU_E_ARG, "emu_star_cascade_batch: NULL argument");
+  if (nbatch < 1 || nlayers < 2 || neq_PW < 1) return fail(EMU_E_ARG, "emu_star_cascade_batch: invalid size");
+  for (int64_t l = 0; l < nlayers; ++l) {
+    if (kind[l] != 0 && kind[l] != 1) return fail(EMU_E_ARG, "layer kind must be 0 (film) or 1 (structured)");
+    if (!kz[l]) return fail(EMU_E_ARG, "kz missing for a layer");
+    if (kind[l] == 0 && (!Z || !Z[l] || nmodes[l] != 2 * neq_PW))
+      return fail(EMU_E_ARG, "film layers need Z and nmodes = 2 neq_PW");
+    if (kind[l] == 1 && (!J || !J_dagger || !J[l] || !J_dagger[l] || nmodes[l] < 1))
+      return fail(EMU_E_ARG, "structured layers need J, J_dagger and nmodes");
+  }
+  EMU_GUARD({
+    emu::CascadeArgs a;
+    a.nbatch = nbatch;
+    a.nlayers = nlayers;
+    a.neq_pw = neq_PW;
+    a.kind = kind;
+    a.nmodes = nmodes;
+    a.kz = kz;
+    a.Z = Z;
+    a.J = J;
+    a.Jd = J_dagger;
+    a.heights = heights;
+    a.Zair = Zair;
+    a.d_minus = d_minus;
+    a.nprop = nprop;
+    a.R_net = R_net;
+    a.T_net = T_net;
+    a.a_list = a_list;
+    a.r_list = r_list;
+    a.t_list = t_list;
+    emu::star_cascade(ctx->c, a);
+  })
+}
+
 int emu_assemble(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, int64_t nb_typ_el, const double *pp,
                  const double *qq, const double *shift, const double *bloch, double *mat1, double *mat2) {
   if (!ctx || !mesh || !pp || !qq || !shift || !bloch) return fail(EMU_E_ARG, "emu_assemble: NULL argument");

@@ -3,7 +3,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <map>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -158,7 +161,47 @@ struct DevMesh {
   DBuf<int64_t> table_nod;           // nel x 6 (1-based)
 };
 
+// A device context: one HIP stream and named, grow-only work buffers.
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Profiler prof;
+  std::vector<void *> bufs;
+  std::vector<size_t> sizes;
+  std::vector<std::string> names;
+  std::map<int, DevMesh *> dense;  // single-front LU handles of dense n x n batches
+  void *get(const std::string &name, size_t bytes) {
+    for (size_t i = 0; i < names.size(); ++i)
+      if (names[i] == name) {
+        if (sizes[i] >= bytes) return bufs[i];
+        (void)hipFree(bufs[i]);
+        bufs[i] = nullptr;
+        if (hipMalloc(&bufs[i], bytes) != hipSuccess) throw HipError("hipMalloc failed for " + name);
+        sizes[i] = bytes;
+        return bufs[i];
+      }
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) throw HipError("hipMalloc failed for " + name);
+    names.push_back(name);
+    bufs.push_back(p);
+    sizes.push_back(bytes);
+    return p;
+  }
+  template <typename T>
+  T *buf(const std::string &name, size_t count) {
+    return reinterpret_cast<T *>(get(name, count * sizeof(T)));
+  }
+  ~Ctx() {
+    for (auto &kv : dense) delete kv.second;
+    for (void *p : bufs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
 void devmesh_build(DevMesh &dm, hipStream_t s);
+// batched dense solve X = M^-1 R (R = identity when null): M n x n, R n x r,
+// column-major, point-major; factorised by the multifrontal kernels as one front
+void solve_dense(Ctx *c, int nb, int n, const zd *M, const zd *R, int r, zd *X);
 
 // ---------------- FEM kernels (kernels_fem.hip) ----------------
 void launch_elem_integrals(DevMesh &dm, hipStream_t s);

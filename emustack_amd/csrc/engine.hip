@@ -269,40 +269,6 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
 }
 
 // ------------------------------------------------------------- workspace
-struct Ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  Profiler prof;
-  std::vector<void *> bufs;
-  std::vector<size_t> sizes;
-  std::vector<std::string> names;
-  void *get(const std::string &name, size_t bytes) {
-    for (size_t i = 0; i < names.size(); ++i)
-      if (names[i] == name) {
-        if (sizes[i] >= bytes) return bufs[i];
-        (void)hipFree(bufs[i]);
-        bufs[i] = nullptr;
-        if (hipMalloc(&bufs[i], bytes) != hipSuccess) throw HipError("hipMalloc failed for " + name);
-        sizes[i] = bytes;
-        return bufs[i];
-      }
-    void *p = nullptr;
-    if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) throw HipError("hipMalloc failed for " + name);
-    names.push_back(name);
-    bufs.push_back(p);
-    sizes.push_back(bytes);
-    return p;
-  }
-  template <typename T>
-  T *buf(const std::string &name, size_t count) {
-    return reinterpret_cast<T *>(get(name, count * sizeof(T)));
-  }
-  ~Ctx() {
-    for (void *p : bufs) (void)hipFree(p);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
-
 Ctx *ctx_create(int device) {
   int ndev = 0;
   EMU_HIP(hipGetDeviceCount(&ndev));

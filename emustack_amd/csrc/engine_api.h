@@ -62,4 +62,24 @@ DevMesh *mesh_create_1d(Ctx *c, int64_t npt, int64_t nel, const int64_t *table_n
                         const double *x, int leaf);
 void calc_modes_1d(Ctx *c, DevMesh &dm, const ModesArgs &a);
 
+// Star-product cascade of a batch of stacks with identical layer structure
+// (bottom to top; layers 0 and nlayers-1 are thin films).  Complex arrays
+// interleaved, point-major.
+struct CascadeArgs {
+  int64_t nbatch = 1, nlayers = 0, neq_pw = 0;
+  const int64_t *kind = nullptr;     // per layer: 0 thin film, 1 structured
+  const int64_t *nmodes = nullptr;   // per layer: Bloch modes (structured) or 2 neq_pw
+  const double *const *kz = nullptr; // per layer: nbatch x dim
+  const double *const *Z = nullptr;  // per film layer: nbatch x 2neq_pw impedances
+  const double *const *J = nullptr;  // per structured layer: nbatch x (2neq_pw x modes), F-order
+  const double *const *Jd = nullptr; // per structured layer: nbatch x (modes x 2neq_pw)
+  const double *heights = nullptr;   // per layer, normalised by the period
+  const double *Zair = nullptr;      // nbatch x 2neq_pw: air reference impedances
+  const double *d_minus = nullptr;   // nbatch x 2neq_pw incident amplitudes (top)
+  const int64_t *nprop = nullptr;    // nbatch x 3: propagating orders of air, top film, bottom film
+  double *R_net = nullptr, *T_net = nullptr;  // nbatch x (2neq_pw)^2 (optional)
+  double *a_list = nullptr, *r_list = nullptr, *t_list = nullptr;  // nbatch x (nlayers-1), nbatch, nbatch x (nlayers-1)
+};
+void star_cascade(Ctx *c, const CascadeArgs &a);
+
 }  // namespace emu

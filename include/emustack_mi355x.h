@@ -156,6 +156,30 @@ int emu_calc_modes_1d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const 
                             double *beta1, double *overlap_J, double *overlap_J_dagger,
                             double *overlap_J_2d, double *overlap_J_dagger_2d, int64_t *stats);
 
+/* Star-product cascade with energy fluxes for nbatch stacks of identical
+ * layer structure (Stack.calc_scat, stack.py:162-427, with the interface
+ * matrices of r_t_mat_anallo / r_t_mat_tf_ns, mode_calcs.py:451-515), on the
+ * GPU as batched dense LU + GEMM.  Layers bottom to top; layers 0 and
+ * nlayers-1 must be thin films.  Per layer l: kind[l] (0 thin film, 1
+ * structured), nmodes[l] (structured: Bloch modes; films: 2 neq_PW),
+ * kz[l] (nbatch x nmodes[l] complex), Z[l] (films: nbatch x 2neq_PW complex
+ * impedances, Anallo.Z; NULL for structured), J[l] / J_dagger[l] (structured:
+ * nbatch x (2neq_PW x nmodes) and nbatch x (nmodes x 2neq_PW), F-order,
+ * J already divided by area_norm as Simmo.J holds it; NULL for films),
+ * heights[l] (height / period; ignored for layers 0 and nlayers-1).
+ * Zair[nbatch*2neq_PW]: impedances of the air reference; d_minus[nbatch *
+ * 2neq_PW]: incident amplitudes in the top layer (specular_incidence);
+ * nprop[nbatch*3]: propagating orders per polarisation of the air
+ * reference, the top and the bottom film.  Outputs (complex interleaved
+ * where complex): R_net, T_net [nbatch * (2neq_PW)^2] (may be NULL), a_list
+ * and t_list [nbatch*(nlayers-1)], r_list [nbatch] (the reference's
+ * a_list, t_list, r_list of each stack). */
+int emu_star_cascade_batch(emu_ctx *ctx, int64_t nbatch, int64_t nlayers, int64_t neq_PW, const int64_t *kind,
+                           const int64_t *nmodes, const double *const *kz, const double *const *Z,
+                           const double *const *J, const double *const *J_dagger, const double *heights,
+                           const double *Zair, const double *d_minus, const int64_t *nprop, double *R_net,
+                           double *T_net, double *a_list, double *r_list, double *t_list);
+
 /* Live kernel timing with HIP events on the context's stream (benchmarks).
  * enable = 1 resets the counters and starts recording every `every`-th
  * solve sweep; enable = 0 stops.  out[k*3 + {0,1,2}] (may be NULL) for kernel
