@@ -196,15 +196,27 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
       }
       L.ea_cnt[sl] = ((int64_t)tasks.size() - L.ea_off[sl]) / 2;
     }
-    // solve row slabs (256 rows = 4 waves of 64 per workgroup)
+    // solve tasks (front, r0, tiles per workgroup): 64-row tiles, 4 per
+    // workgroup when the front has at least 4, else 2 or 1 with the waves
+    // splitting each tile's columns
+    auto solve_tasks = [&](bool fwd) {
+      for (int32_t t : LP.fronts) {
+        const int32_t R = fwd ? sy.fronts[t].f : sy.fronts[t].s;
+        const int32_t T = (R + 63) / 64;
+        const int32_t tpw = T >= 4 ? 4 : (T >= 2 ? 2 : 1);
+        for (int32_t r0 = 0; r0 < R; r0 += 64 * tpw) {
+          tasks.push_back(t);
+          tasks.push_back(r0);
+          tasks.push_back(tpw);
+        }
+      }
+    };
     L.sf_off = (int64_t)tasks.size();
-    for (int32_t t : LP.fronts)
-      for (int32_t r0 = 0; r0 < sy.fronts[t].f; r0 += 256) { tasks.push_back(t); tasks.push_back(r0); }
-    L.sf_cnt = ((int64_t)tasks.size() - L.sf_off) / 2;
+    solve_tasks(true);
+    L.sf_cnt = ((int64_t)tasks.size() - L.sf_off) / 3;
     L.sb_off = (int64_t)tasks.size();
-    for (int32_t t : LP.fronts)
-      for (int32_t r0 = 0; r0 < sy.fronts[t].s; r0 += 256) { tasks.push_back(t); tasks.push_back(r0); }
-    L.sb_cnt = ((int64_t)tasks.size() - L.sb_off) / 2;
+    solve_tasks(false);
+    L.sb_cnt = ((int64_t)tasks.size() - L.sb_off) / 3;
     // algorithmic bytes of one solve sweep of this level (per instance):
     // triangle incl. diagonal + off-diagonal block of the inverted form once
     // per RHS chunk; b/y/x/update vectors per RHS

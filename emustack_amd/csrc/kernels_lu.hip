@@ -470,38 +470,41 @@ void lu_invert(DevMesh &dm, int nb, const zd *pool, zd *inv, hipStream_t s) {
 
 // ---------------------------------------------------------------- solves
 //
-// One workgroup per (256-row slab of a front, instance, chunk of RC
-// right-hand sides); each of the four wavefronts owns 64 consecutive rows and
-// runs over the whole column range of its rows, so no cross-wave reduction
-// is needed.  The front's input vector is staged through LDS in chunks of KC
-// entries, gathered on the fly (forward: b at the pivot rows in pivot order
-// plus the children's update vectors; backward: y at the own rows, x at the
-// ancestors' rows) and shared by the four waves.  Structural zeros of the
-// triangular inverses are skipped per wave.  Slabs of one level never write
-// what another reads, so a level is one launch.
+// Rows of a front are cut into 64-row tiles, one per wavefront.  A workgroup
+// (4 waves) takes `tpw` consecutive tiles of one front (task (front, r0,
+// tpw), tpw = 4, 2 or 1 chosen per front on the host); the 4 / tpw waves that
+// share a tile split its column range and add their partial sums through
+// LDS.  Small-front levels thus keep all waves busy on distinct rows, the
+// few large fronts of the top levels get the column split.  The front's
+// input vector is staged through LDS in chunks of KC entries, gathered on
+// the fly (forward: b at the pivot rows in pivot order plus the children's
+// update vectors; backward: y at the own rows, x at the ancestors' rows).
+// Structural zeros of the triangular inverses are skipped per tile.  Tasks
+// of one level never write what another reads, so a level is one launch.
 
-constexpr int RT = 256;  // rows per workgroup (4 waves x 64)
-constexpr int KC = 64;   // staged input chunk
+constexpr int KC = 64;  // staged input chunk
 
 template <int RC>
 __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, FrontArrays A,
                                              const zd *__restrict__ inv, int64_t isz, const int32_t *__restrict__ sig,
                                              int64_t npiv, int nrhs, int64_t n, int64_t nw,
                                              const zd *__restrict__ bv, zd *__restrict__ y, zd *__restrict__ wbuf) {
-  const int t = tasks[2 * blockIdx.x], r0 = tasks[2 * blockIdx.x + 1];
+  const int t = tasks[3 * blockIdx.x], r0 = tasks[3 * blockIdx.x + 1], tpw = tasks[3 * blockIdx.x + 2];
   const int b = blockIdx.y;
   const int q0 = blockIdx.z * RC;
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ks = 4 / tpw, tile = wave / ks, kp = wave - tile * ks;
   const int s = A.s[t], f = A.f[t], fb = A.b[t];
   const zd *Fw = inv + b * isz + A.ioff[t];
   const int32_t *sg = sig + b * npiv + A.pivoff[t];
   const int32_t *cm = A.cmap + 2 * A.fvoff[t];
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
-  const int row = r0 + tid;
-  const int kend = min(s, r0 + RT);                      // workgroup's column range
-  const int kend_w = min(s, r0 + 64 * (wave + 1));       // lower-triangular rows need k <= row
+  const int row = r0 + 64 * tile + lane;
+  const int kend = min(s, r0 + 64 * tpw);             // workgroup's column range
+  const int kend_w = min(s, r0 + 64 * (tile + 1));    // lower-triangular rows need k <= row
   __shared__ zd xin[KC][RC];
+  __shared__ zd red[3][64][RC];
   zd acc[RC];
 #pragma unroll
   for (int q = 0; q < RC; ++q) acc[q] = zmk(0, 0);
@@ -523,8 +526,8 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
     const int kl = min(KC, kend_w - kc);
     if (row < f && kl > 0) {
       const zd *col = Fw + row + (int64_t)kc * f;
-#pragma unroll 8
-      for (int k = 0; k < kl; ++k) {
+#pragma unroll 4
+      for (int k = kp; k < kl; k += ks) {
         const zd a = col[(int64_t)k * f];
 #pragma unroll
         for (int q = 0; q < RC; ++q) acc[q] = zfma(a, xin[k][q], acc[q]);
@@ -532,7 +535,17 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
     }
     __syncthreads();
   }
-  if (row < f) {
+  if (ks > 1) {
+    if (kp > 0)
+#pragma unroll
+      for (int q = 0; q < RC; ++q) red[wave - 1][lane][q] = acc[q];
+    __syncthreads();
+    if (kp == 0)
+      for (int j = 1; j < ks; ++j)
+#pragma unroll
+        for (int q = 0; q < RC; ++q) acc[q] = zadd(acc[q], red[wave + j - 1][lane][q]);
+  }
+  if (kp == 0 && row < f) {
     for (int q = 0; q < nq; ++q) {
       zd v = acc[q];
       if (row < s) {
@@ -551,18 +564,20 @@ template <int RC>
 __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, FrontArrays A,
                                              const zd *__restrict__ inv, int64_t isz, int nrhs, int64_t n,
                                              const zd *__restrict__ y, zd *__restrict__ x) {
-  const int t = tasks[2 * blockIdx.x], r0 = tasks[2 * blockIdx.x + 1];
+  const int t = tasks[3 * blockIdx.x], r0 = tasks[3 * blockIdx.x + 1], tpw = tasks[3 * blockIdx.x + 2];
   const int b = blockIdx.y;
   const int q0 = blockIdx.z * RC;
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ks = 4 / tpw, tile = wave / ks, kp = wave - tile * ks;
   const int s = A.s[t], f = A.f[t], fb = A.b[t];
   const zd *Bw = inv + b * isz + A.ioff[t] + (int64_t)f * s;
   const int32_t *ui = A.uidx + A.uoff[t];
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
-  const int row = r0 + tid;
-  const int kbeg_w = r0 + 64 * wave;  // upper-triangular rows need k >= row
+  const int row = r0 + 64 * tile + lane;
+  const int kbeg_w = r0 + 64 * tile;  // upper-triangular rows need k >= row
   __shared__ zd xin[KC][RC];
+  __shared__ zd red[3][64][RC];
   zd acc[RC];
 #pragma unroll
   for (int q = 0; q < RC; ++q) acc[q] = zmk(0, 0);
@@ -578,8 +593,8 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, 
     const int klo = max(0, kbeg_w - kc), khi = min(KC, f - kc);
     if (row < s && klo < khi) {
       const zd *col = Bw + row + (int64_t)kc * s;
-#pragma unroll 8
-      for (int k = klo; k < khi; ++k) {
+#pragma unroll 4
+      for (int k = klo + kp; k < khi; k += ks) {
         const zd a = col[(int64_t)k * s];
 #pragma unroll
         for (int q = 0; q < RC; ++q) acc[q] = zfma(a, xin[k][q], acc[q]);
@@ -587,14 +602,28 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, 
     }
     __syncthreads();
   }
-  if (row < s)
+  if (ks > 1) {
+    if (kp > 0)
+#pragma unroll
+      for (int q = 0; q < RC; ++q) red[wave - 1][lane][q] = acc[q];
+    __syncthreads();
+    if (kp == 0)
+      for (int j = 1; j < ks; ++j)
+#pragma unroll
+        for (int q = 0; q < RC; ++q) acc[q] = zadd(acc[q], red[wave + j - 1][lane][q]);
+  }
+  if (kp == 0 && row < s)
     for (int q = 0; q < nq; ++q) x[(col0 + q) * n + fb + row] = acc[q];
 }
 
-static int pick_rc(int nrhs) {
-  for (int rc : {8, 4, 2})
-    if (nrhs >= rc) return rc;
-  return 1;
+// RHS columns per workgroup: 8 when the level has enough workgroups to fill
+// the GPU; fewer (more workgroups, each re-reading the front from cache) on
+// the top levels whose few large fronts would otherwise leave most CUs idle
+static int pick_rc(int nrhs, int64_t tasks, int nb) {
+  int rc = 8;
+  while (rc > nrhs) rc >>= 1;
+  while (rc > 1 && tasks * nb * ((nrhs + rc - 1) / rc) < 1024) rc >>= 1;
+  return rc;
 }
 
 template <int RC>
@@ -618,28 +647,29 @@ static void launch_bwd(const LevelSched &L, DevMesh &dm, FrontArrays A, int nb, 
 void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *sig, const zd *bnew, zd *y, zd *x,
               zd *w, hipStream_t s, Profiler *prof) {
   FrontArrays A = fa(dm);
-  const int rc = pick_rc(nrhs);
-  const double chunks = (double)nb * ((nrhs + rc - 1) / rc), vecs = (double)nb * nrhs;
+  const double vecs = (double)nb * nrhs;
   for (const LevelSched &L : dm.sched) {
     int ev = prof ? prof->begin(s) : -1;
+    const int rc = pick_rc(nrhs, L.sf_cnt, nb);
     switch (rc) {
       case 8: launch_fwd<8>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, s); break;
       case 4: launch_fwd<4>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, s); break;
       case 2: launch_fwd<2>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, s); break;
       default: launch_fwd<1>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, s); break;
     }
-    if (prof) prof->end(s, ev, 0, L.fwd_mat * chunks + L.fwd_vec * vecs);
+    if (prof) prof->end(s, ev, 0, L.fwd_mat * nb * ((nrhs + rc - 1) / rc) + L.fwd_vec * vecs);
   }
   EMU_LAUNCH_CHECK();
   for (auto it = dm.sched.rbegin(); it != dm.sched.rend(); ++it) {
     int ev = prof ? prof->begin(s) : -1;
+    const int rc = pick_rc(nrhs, it->sb_cnt, nb);
     switch (rc) {
       case 8: launch_bwd<8>(*it, dm, A, nb, nrhs, inv, y, x, s); break;
       case 4: launch_bwd<4>(*it, dm, A, nb, nrhs, inv, y, x, s); break;
       case 2: launch_bwd<2>(*it, dm, A, nb, nrhs, inv, y, x, s); break;
       default: launch_bwd<1>(*it, dm, A, nb, nrhs, inv, y, x, s); break;
     }
-    if (prof) prof->end(s, ev, 1, it->bwd_mat * chunks + it->bwd_vec * vecs);
+    if (prof) prof->end(s, ev, 1, it->bwd_mat * nb * ((nrhs + rc - 1) / rc) + it->bwd_vec * vecs);
   }
   EMU_LAUNCH_CHECK();
 }
