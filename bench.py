@@ -68,15 +68,15 @@ def layers_for(mesh_paths):
     return ns, film, sub, sup
 
 
-def run_step(ns, film, sub, sup, wls, groups=1):
+def run_step(ns, film, sub, sup, wls, groups=1, ncv=0):
     """All three layers for the wavelength batch `wls` (`groups` concurrent
-    sub-batches per layer), then the cascade."""
+    sub-batches per layer, Krylov dimension ncv), then the cascade."""
     from emustack_amd import objects, Stack
     from emustack_amd.batch import calc_modes_many, calc_scat_many
     lights = [objects.Light(wl, max_order_PWs=ORDER, theta=0.0, phi=0.0) for wl in wls]
     sims, ops = [], 0
     for s in ns:
-        sims.append(calc_modes_many(s, lights, num_BMs=NUM_BMS, groups=groups))
+        sims.append(calc_modes_many(s, lights, num_BMs=NUM_BMS, groups=groups, ncv=ncv))
         ops += int(calc_modes_many.last_stats[0])
     stacks = [Stack((sub.calc_modes(light), sims[0][i], film.calc_modes(light), sims[1][i],
                      film.calc_modes(light), sims[2][i], sup.calc_modes(light))) for i, light in enumerate(lights)]
@@ -150,9 +150,11 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=2)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--batch', type=int, default=2, help='wavelength points per step')
-    ap.add_argument('--groups', type=int, default=1,
+    ap.add_argument('--batch', type=int, default=8, help='wavelength points per step')
+    ap.add_argument('--groups', type=int, default=2,
                     help='concurrent sub-batches (host threads, each with its own context and stream)')
+    ap.add_argument('--ncv', type=int, default=384,
+                    help='Krylov dimension (0: the reference rule 2 num_BMs + num_BMs/2 + 3 = 503)')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline sample')
     ap.add_argument('--workdir', default=os.path.join(ROOT, 'gpurun_out', 'bench_meshes'))
     args = ap.parse_args()
@@ -179,16 +181,16 @@ def main():
     for _ in range(args.warmup):
         wl = next_batch()
         print('warmup batch %s' % np.round(wl, 1), file=sys.stderr, flush=True)
-        run_step(ns, film, sub, sup, wl, args.groups)
+        run_step(ns, film, sub, sup, wl, args.groups, args.ncv)
     ctxs = EMUstack.all_contexts()
     for c in ctxs:
-        c.profile(True, every=4)
+        c.profile(True, every=1)
     ops = 0
     results = []
     t0 = time.perf_counter()
     for k in range(args.steps):
         wl = next_batch()
-        res, o = run_step(ns, film, sub, sup, wl, args.groups)
+        res, o = run_step(ns, film, sub, sup, wl, args.groups, args.ncv)
         ops += o
         results += res
         print('step %d done (%.1f s)' % (k, time.perf_counter() - t0), file=sys.stderr, flush=True)
@@ -212,7 +214,7 @@ def main():
         'config': {'workload': 'stacked gratings: 3 structured layers (r=100/150/80 nm, ~%d-triangle P2 meshes) '
                                '+ 2 films, period 500 nm, num_BMs=%d, max_order_PWs=%d (98-dim S-matrix), '
                                '%d wavelengths 400-1000 nm, normal incidence' % (NTRI, NUM_BMS, ORDER, NWL),
-                   'batch_wavelengths': args.batch, 'groups': args.groups, 'precision': 'fp64 (complex128) everywhere',
+                   'batch_wavelengths': args.batch, 'groups': args.groups, 'krylov_dim': args.ncv or None, 'precision': 'fp64 (complex128) everywhere',
                    'op_applications_per_instance': ops / (npts * 6.0)},
         'roofline': {'bound': 'hbm', 'kernel': kernel_name, 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,

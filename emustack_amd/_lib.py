@@ -50,17 +50,38 @@ class EmuError(RuntimeError):
         self.code = code
 
 
+_blas_lock = threading.Lock()
+_blas_users = 0
+_blas_limit = None
+
+
+@contextlib.contextmanager
 def _host_lapack_threads(nl):
     """The library projects the Krylov-Schur problem of each batch instance
     on its own host thread; the registered LAPACK (scipy's OpenBLAS) then
     runs single-threaded, so that concurrent calls do not contend for one
     BLAS thread pool (a threaded m = 504 zgees measured 5x slower than a
-    single-threaded one on the GPU host)."""
+    single-threaded one on the GPU host).  The limit is process-wide, so it
+    is reference-counted: concurrent library calls (sub-batches on several
+    contexts) share it and the last one to leave restores the original."""
+    global _blas_users, _blas_limit
     try:
         from threadpoolctl import threadpool_limits
     except ImportError:
-        return contextlib.nullcontext()
-    return threadpool_limits(limits=1, user_api='blas')
+        yield
+        return
+    with _blas_lock:
+        if _blas_users == 0:
+            _blas_limit = threadpool_limits(limits=1, user_api='blas')
+        _blas_users += 1
+    try:
+        yield
+    finally:
+        with _blas_lock:
+            _blas_users -= 1
+            if _blas_users == 0:
+                _blas_limit.restore_original_limits()
+                _blas_limit = None
 
 
 def _lapack_pointers():

@@ -30,7 +30,7 @@ def _pool(n):
         return p
 
 
-def _solve(structure, sims, inputs, nbm, order, P, tol, itermax):
+def _solve(structure, sims, inputs, nbm, order, P, tol, itermax, ncv):
     st = structure
     lam = [s.wl_norm() for s in sims]
     n_effs = [x[0] for x in inputs]
@@ -38,22 +38,23 @@ def _solve(structure, sims, inputs, nbm, order, P, tol, itermax):
     shifts = [x[3] for x in inputs]
     if st.periodicity == '2D_array':
         beta, J, Jd, stats = EMUstack.calc_modes_2d_batch(lam, nbm, order, paths.msh_path + st.mesh_file, n_effs,
-                                                          kps, shifts, P, tol=tol, itermax=itermax)
+                                                          kps, shifts, P, tol=tol, itermax=itermax, ncv=ncv)
         area_norm = st.period * st.period_y / st.period ** 2
         return beta, [np.asmatrix(j) / area_norm for j in J], [np.asmatrix(j) for j in Jd], stats
     h = EMUstack.mesh_1d(st.x_arr, st.table_nod, st.type_el)
     w1 = st.world_1d is True
     P2 = 1 if w1 else sims[0].calc_2d_grating_orders(order)[0].size
     beta, J, Jd, J2, Jd2, stats = EMUstack.context().calc_modes_1d_batch(
-        h, lam, nbm, order, n_effs, kps, shifts, P, neq_pw_2d=P2, world_1d=w1, itermax=itermax, tol=tol)
+        h, lam, nbm, order, n_effs, kps, shifts, P, neq_pw_2d=P2, world_1d=w1, itermax=itermax, tol=tol, ncv=ncv)
     if not w1:
         J, Jd = J2, Jd2
     return beta, [np.asmatrix(j) for j in J], [np.asmatrix(j) for j in Jd], stats
 
 
-def calc_modes_many(structure, lights, num_BMs=None, tol=0.0, itermax=30, groups=1):
+def calc_modes_many(structure, lights, num_BMs=None, tol=0.0, itermax=30, groups=1, ncv=0):
     """Simmo objects for every light in `lights` (same max_order_PWs),
-    computed together on the GPU (`groups` concurrent sub-batches)."""
+    computed together on the GPU (`groups` concurrent sub-batches; ncv the
+    Krylov dimension, 0 for the reference's rule)."""
     if structure.periodicity not in ('1D_array', '2D_array'):
         raise ValueError("NanoStruct layer must have periodicity of either '1D_array' or '2D_array'.")
     sims = [Simmo(structure, l) for l in lights]
@@ -71,9 +72,9 @@ def calc_modes_many(structure, lights, num_BMs=None, tol=0.0, itermax=30, groups
     cuts = np.linspace(0, len(sims), groups + 1).round().astype(int)
     parts = [(cuts[g], cuts[g + 1]) for g in range(groups)]
     if groups == 1:
-        results = [_solve(structure, sims, inputs, nbm, order, P, tol, itermax)]
+        results = [_solve(structure, sims, inputs, nbm, order, P, tol, itermax, ncv)]
     else:
-        futs = [_pool(groups).submit(_solve, structure, sims[a:b], inputs[a:b], nbm, order, P, tol, itermax)
+        futs = [_pool(groups).submit(_solve, structure, sims[a:b], inputs[a:b], nbm, order, P, tol, itermax, ncv)
                 for a, b in parts]
         results = [f.result() for f in futs]
     stats = np.zeros(8, np.int64)

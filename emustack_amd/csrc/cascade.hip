@@ -442,7 +442,7 @@ void solve_dense(Ctx *c, int nb, int n, const zd *M, const zd *R, int r, zd *X) 
     r = n;
   }
   zd *y = c->buf<zd>("ds_y", (size_t)nb * n * r);
-  zd *w = c->buf<zd>("ds_w", 1);
+  zd *w = c->buf<zd>("ds_w", (size_t)nb * r * solve_ws(dm));
   lu_solve(dm, nb, r, inv, sig, rhs, y, X, w, s);
 }
 

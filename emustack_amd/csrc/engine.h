@@ -1,6 +1,7 @@
 // Device-side state of a mesh and of a batch of solve instances, and the
 // kernel launchers of the FEM / LU / Krylov / overlap stages.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,8 +48,19 @@ struct DBuf {
   void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
 };
 
-// Live kernel timing with HIP events (enabled by emu_profile).  A "sweep" is
-// one solve or one Gram-Schmidt pass; every `every`-th sweep is recorded.
+// Start/stop events of one timed dispatch (both null: not timed).  They are
+// handed to hipExtLaunchKernelGGL, which stamps them at the start and end of
+// exactly that kernel, so the measured time excludes launch gaps and the
+// neighbouring kernels of a stage.
+struct KTime {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+// Live kernel timing (enabled by emu_profile).  A "sweep" is one solve or one
+// Gram-Schmidt pass; every `every`-th sweep is recorded.  Classes: 0 forward
+// solve (k_fwd), 1 backward solve (k_bwd), 2 Gram-Schmidt projection
+// (k_atb_tall), 3 Gram-Schmidt update (k_sub_vh_blk), 4 sparse M product
+// (k_spmm); per class: dispatches, milliseconds, algorithmic bytes.
 struct Profiler {
   bool on = false;
   int every = 8;
@@ -64,20 +76,21 @@ struct Profiler {
   std::vector<Pend> pend;
   double stat[8][3] = {};
   void start_sweep() { active = on && (sweep++ % every == 0); }
-  int mark(hipStream_t s) {
+  int slot() {
     if (used == ev.size()) {
       hipEvent_t e;
       EMU_HIP(hipEventCreate(&e));
       ev.push_back(e);
     }
-    EMU_HIP(hipEventRecord(ev[used], s));
     return (int)used++;
   }
-  // returns the begin mark (or -1 when not recording)
-  int begin(hipStream_t s) { return active ? mark(s) : -1; }
-  void end(hipStream_t s, int a, int cls, double bytes) {
-    if (a < 0) return;
-    pend.push_back({cls, bytes, a, mark(s)});
+  // events for the next dispatch of class `cls` moving `bytes` algorithmic
+  // bytes; null events when this sweep is not recorded
+  KTime time(int cls, double bytes) {
+    if (!active) return {};
+    int a = slot(), b = slot();
+    pend.push_back({cls, bytes, a, b});
+    return {ev[a], ev[b]};
   }
   // call after the stream has been synchronised
   void collect() {
@@ -102,6 +115,9 @@ struct Profiler {
   }
 };
 
+// Column span of one split solve task (see the solve task lists).
+constexpr int kSolveSplit = 256;
+
 // Kernel task lists of one factorisation / solve level.
 struct StepPlan {
   int32_t k0;
@@ -113,11 +129,15 @@ struct LevelSched {
   std::vector<StepPlan> steps;
   int64_t fronts_off, fronts_cnt;     // fronts of the level
   int64_t ea_off[2], ea_cnt[2];       // extend-add tasks (child front, col0) by slot
-  int64_t sf_off, sf_cnt;             // forward-solve row tiles (front, row0)
-  int64_t sb_off, sb_cnt;             // backward-solve row tiles (front, row0)
+  int64_t sf_off, sf_cnt;             // forward-solve tasks (front, row0, tiles, k0, slot)
+  int64_t sb_off, sb_cnt;             // backward-solve tasks (front, row0, tiles, k0, slot)
+  int64_t rf_off = 0, rf_cnt = 0;     // forward partial-sum reductions (front, row0, slot0, count)
+  int64_t rb_off = 0, rb_cnt = 0;     // backward partial-sum reductions
+  int32_t pf_cnt = 0, pb_cnt = 0;     // partial-sum slots of the level (64 rows each)
   int32_t max_f;
   // algorithmic bytes of one solve sweep of the level, per instance: factor
-  // bytes (once per RHS chunk) and vector bytes (per RHS)
+  // bytes (counted once per solve, whatever the RHS chunking: re-reads of a
+  // front by several chunks are not algorithmic) and vector bytes (per RHS)
   double fwd_mat = 0, fwd_vec = 0, bwd_mat = 0, bwd_vec = 0;
 };
 
@@ -149,6 +169,7 @@ struct DevMesh {
   DBuf<int64_t> fioff, ffvoff;       // inverted-form offset (prefix of 2 f s), prefix of f
   DBuf<int32_t> cmap;                // per front row: update-vector index of child slot 0 / 1, or -1
   int64_t isz = 0;                   // inverted-form entries per instance
+  int64_t part_rhs = 0;              // partial-sum entries per RHS (largest level)
   DBuf<int32_t> tasks;               // all task lists
   int64_t tri_off = 0, tri_cnt = 0;  // triangular inversion tasks (front, J, kind)
   int64_t wz_off = 0, wz_cnt = 0;    // off-diagonal inverted blocks (front, kind, row0, col0)
@@ -220,7 +241,8 @@ void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int3
                zd *inv, hipStream_t s);
 void lu_invert(DevMesh &dm, int nb, const zd *pool, zd *inv, hipStream_t s);
 // x = A^{-1} b, both in the new (nested-dissection) numbering, [nb][nrhs][n];
-// y (same shape) and w ([nb][nrhs][nw]) are work vectors
+// y (same shape) and w ([nb][nrhs][solve_ws(dm)]) are work vectors
+int64_t solve_ws(const DevMesh &dm);
 void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *sig, const zd *b, zd *y, zd *x,
               zd *w, hipStream_t s, Profiler *prof = nullptr);
 // permute vectors old <-> new numbering: dst[new] = src[perm[new]] or inverse
@@ -231,11 +253,13 @@ void launch_permute_vec(DevMesh &dm, int nb, int ncol, const zd *src, zd *dst, b
 // y[b][r] = sum_k A[b][r][k] x[b][col k] for CSR (n x n), ncol columns
 void launch_spmm(int nb, int64_t n, int ncol, const int64_t *rp, const int32_t *ci, const zd *vals,
                  int64_t vstride, const zd *x, int64_t xstride, zd *y, int64_t ystride,
-                 hipStream_t s);
-// C[b] (M x N, ldc) = op(A[b])^T B[b] with K rows; op = conj if conjA.  Split-K.
+                 hipStream_t s, KTime kt = {});
+// C[b] (M x N, ldc) = op(A[b])^T B[b] with K rows; op = conj if conjA.  Split-K
+// (partial products per K chunk in scratch, then a reduction); N <= 8 takes
+// the tall-skinny kernel, which is the one kt times.
 void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64_t sA,
                 const zd *B, int64_t ldb, int64_t sB, zd *C, int64_t ldc, int64_t sC,
-                bool conjA, zd *scratch, size_t scratch_elems, hipStream_t s);
+                bool conjA, zd *scratch, size_t scratch_elems, hipStream_t s, KTime kt = {});
 // C[b] (M x N) = alpha * A[b] (M x K) * B[b] (K x N) + beta*C[b], column-major
 void launch_gemm_nn(int nb, int64_t M, int N, int K, zd alpha, const zd *A, int64_t lda, int64_t sA,
                     const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,
@@ -251,7 +275,15 @@ void launch_axpy_h(int nb, int k, const zd *h1, const zd *h2, int64_t sh, zd *H,
 void launch_copy(int nb, int64_t n, const zd *src, int64_t ss, zd *dst, int64_t sd, hipStream_t s);
 // block Gram-Schmidt: W[:, :bq] -= V[:, :k] H[:k, :bq]
 void launch_sub_vh_blk(int nb, int64_t n, int k, int bq, const zd *V, int64_t ldv, int64_t sV, const zd *H,
-                       int64_t ldh, int64_t sH, zd *W, int64_t ldw, int64_t sW, hipStream_t s);
+                       int64_t ldh, int64_t sH, zd *W, int64_t ldw, int64_t sW, hipStream_t s, KTime kt = {});
+// Delayed block Gram-Schmidt (block size 8): W (n x nq, in place) = W L - V[:, :k] E
+void launch_combine(int nb, int64_t n, int k, int nq, const zd *V, int64_t ldv, int64_t sV, const zd *E,
+                    int64_t lde, int64_t sE, const zd *L, zd *W, int64_t ldw, int64_t sW, hipStream_t s,
+                    KTime kt = {});
+// ... and the coefficients of one step (fin: reorthogonalise the last block only)
+void launch_dcgs_coef(int nb, bool fin, int j, int c0, const zd *P, int64_t ldp, int64_t sP, const zd *G,
+                      const zd *A0, int64_t lda, int64_t sA, zd *H, int64_t ldh, int64_t sH, zd *E, int64_t lde,
+                      int64_t sE, zd *L, int32_t *bad, hipStream_t s);
 // Cholesky-QR of a bq-column block: G = R^H R, Rinv = R^-1, Racc = R * Rprev
 void launch_cholqr(int nb, int bq, const zd *G, zd *Rinv, const zd *Rprev, zd *Racc, int32_t *bad,
                    hipStream_t s);

@@ -19,6 +19,13 @@
 
 namespace emu {
 
+// One mutex per device: the "turn" that the Krylov cycles of concurrent
+// contexts on that device take for their GPU phases.
+std::mutex &gpu_turn(int device) {
+  static std::mutex turns[64];
+  return turns[device & 63];
+}
+
 typedef std::complex<double> cplx;
 
 // ------------------------------------------------------------------ LAPACK
@@ -196,30 +203,55 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
       }
       L.ea_cnt[sl] = ((int64_t)tasks.size() - L.ea_off[sl]) / 2;
     }
-    // solve tasks (front, r0, tiles per workgroup): 64-row tiles, 4 per
-    // workgroup when the front has at least 4, else 2 or 1 with the waves
-    // splitting each tile's columns
-    auto solve_tasks = [&](bool fwd) {
+    // solve tasks (front, r0, tiles per workgroup, k0, slot): 64-row tiles,
+    // 4 per workgroup when the front has at least 4, else 2 or 1 with the
+    // waves splitting each tile's columns (k0 = slot = -1).  The fronts at
+    // the top of the tree are few and long: a tile whose column range exceeds
+    // kSolveSplit columns is cut into tasks of kSolveSplit columns (from k0)
+    // that write partial sums to a slot, and a reduction task (front, r0,
+    // first slot, count) adds them, so these levels still fill the GPU.
+    auto solve_tasks = [&](bool fwd, std::vector<int32_t> &red, int32_t &nslot) {
+      nslot = 0;
       for (int32_t t : LP.fronts) {
-        const int32_t R = fwd ? sy.fronts[t].f : sy.fronts[t].s;
-        const int32_t T = (R + 63) / 64;
-        const int32_t tpw = T >= 4 ? 4 : (T >= 2 ? 2 : 1);
-        for (int32_t r0 = 0; r0 < R; r0 += 64 * tpw) {
-          tasks.push_back(t);
-          tasks.push_back(r0);
-          tasks.push_back(tpw);
+        const FrontInfo &F = sy.fronts[t];
+        const int32_t R = fwd ? F.f : F.s;
+        const int32_t kmax = fwd ? F.s : F.f;
+        if (kmax > kSolveSplit) {
+          for (int32_t r0 = 0; r0 < R; r0 += 64) {
+            const int32_t kb = fwd ? 0 : r0, ke = fwd ? std::min(F.s, r0 + 64) : F.f;
+            const int32_t nc = (ke - kb + kSolveSplit - 1) / kSolveSplit;
+            if (nc <= 1) {
+              tasks.insert(tasks.end(), {t, r0, 1, kb, -1});
+              continue;
+            }
+            red.insert(red.end(), {t, r0, nslot, nc});
+            for (int32_t c = 0; c < nc; ++c) tasks.insert(tasks.end(), {t, r0, 1, kb + c * kSolveSplit, nslot + c});
+            nslot += nc;
+          }
+        } else {
+          const int32_t T = (R + 63) / 64;
+          const int32_t tpw = T >= 4 ? 4 : (T >= 2 ? 2 : 1);
+          for (int32_t r0 = 0; r0 < R; r0 += 64 * tpw) tasks.insert(tasks.end(), {t, r0, tpw, -1, -1});
         }
       }
     };
+    std::vector<int32_t> redf, redb;
     L.sf_off = (int64_t)tasks.size();
-    solve_tasks(true);
-    L.sf_cnt = ((int64_t)tasks.size() - L.sf_off) / 3;
+    solve_tasks(true, redf, L.pf_cnt);
+    L.sf_cnt = ((int64_t)tasks.size() - L.sf_off) / 5;
     L.sb_off = (int64_t)tasks.size();
-    solve_tasks(false);
-    L.sb_cnt = ((int64_t)tasks.size() - L.sb_off) / 3;
+    solve_tasks(false, redb, L.pb_cnt);
+    L.sb_cnt = ((int64_t)tasks.size() - L.sb_off) / 5;
+    L.rf_off = (int64_t)tasks.size();
+    tasks.insert(tasks.end(), redf.begin(), redf.end());
+    L.rf_cnt = (int64_t)redf.size() / 4;
+    L.rb_off = (int64_t)tasks.size();
+    tasks.insert(tasks.end(), redb.begin(), redb.end());
+    L.rb_cnt = (int64_t)redb.size() / 4;
+    dm.part_rhs = std::max<int64_t>(dm.part_rhs, 64 * (int64_t)std::max(L.pf_cnt, L.pb_cnt));
     // algorithmic bytes of one solve sweep of this level (per instance):
     // triangle incl. diagonal + off-diagonal block of the inverted form once
-    // per RHS chunk; b/y/x/update vectors per RHS
+    // per solve; b/y/x/update vectors per RHS
     for (int32_t t : LP.fronts) {
       const FrontInfo &F = sy.fronts[t];
       double ss = F.s, uu = F.u, wc = 0;
@@ -381,7 +413,7 @@ void lu_solve_host(Ctx *c, DevMesh &dm, int nb, const double *mat1, int nrhs, co
   zd *bn = c->buf<zd>("lu_bn", (size_t)nb * nrhs * n);
   zd *yn = c->buf<zd>("lu_y", (size_t)nb * nrhs * n);
   zd *xn = c->buf<zd>("lu_x", (size_t)nb * nrhs * n);
-  zd *w = c->buf<zd>("lu_w", (size_t)nb * nrhs * std::max<int64_t>(sy.nw, 1));
+  zd *w = c->buf<zd>("lu_w", (size_t)nb * nrhs * solve_ws(dm));
   EMU_HIP(hipMemcpyAsync(bo, b, sizeof(zd) * nb * nrhs * n, hipMemcpyHostToDevice, c->stream));
   EMU_HIP(hipStreamSynchronize(c->stream));
   launch_permute_vec(dm, nb, nrhs, bo, bn, true, c->stream);
@@ -602,7 +634,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   zd *t1 = c->buf<zd>("ks_t1", (size_t)nb * bq * n);
   zd *ty = c->buf<zd>("ks_ty", (size_t)nb * bq * n);
   zd *wv = c->buf<zd>("ks_wv", (size_t)nb * bq * n);
-  zd *tw = c->buf<zd>("ks_w", (size_t)nb * bq * std::max<int64_t>(sy.nw, 1));
+  zd *tw = c->buf<zd>("ks_w", (size_t)nb * bq * solve_ws(dm));
   zd *Hd = c->buf<zd>("ks_H", (size_t)nb * sH);
   zd *H2 = c->buf<zd>("ks_H2", (size_t)nb * ld * bq);
   zd *G = c->buf<zd>("ks_G", (size_t)nb * bq * bq);
@@ -613,6 +645,15 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   size_t atb_elems = (size_t)nb * (m + bq) * std::max(bq, 32) * 128;
   zd *atbs = c->buf<zd>("ks_atb", atb_elems);
   zd *Qd = c->buf<zd>("ks_Q", (size_t)nb * m * m);
+  // delayed block Gram-Schmidt (block size 8): projections P = V^H [Z W],
+  // A0 = H S, combination E, L and the 16 x 16 Gram matrix of [Z W]
+  const bool delayed = bq == 8;
+  const int64_t ldp = m + bq, sP = ldp * 2 * bq, sA0 = ldp * bq;
+  zd *Pd = delayed ? c->buf<zd>("ks_P", (size_t)nb * sP) : nullptr;
+  zd *A0d = delayed ? c->buf<zd>("ks_A0", (size_t)nb * sA0) : nullptr;
+  zd *Ed = delayed ? c->buf<zd>("ks_E", (size_t)nb * sP) : nullptr;
+  zd *Ld = delayed ? c->buf<zd>("ks_L", (size_t)nb * 4 * bq * bq) : nullptr;
+  zd *G16 = delayed ? c->buf<zd>("ks_G16", (size_t)nb * 4 * bq * bq) : nullptr;
   EMU_HIP(hipMemsetAsync(bad, 0, sizeof(int32_t) * nb, s));
   // start block: deterministic pseudo-random, orthonormalised
   {
@@ -639,26 +680,48 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   std::vector<std::vector<cplx>> ritz(nb);
   Profiler *pf = c->prof.on ? &c->prof : nullptr;
   const double nbd = nb, nd = (double)n, nzd = (double)dm.host.nnz;
+  // The GPU phases of a cycle (restart combination + expansion) hold the
+  // device's turn; the host Schur phase releases it, so that concurrent
+  // sub-batches on other contexts alternate on the GPU instead of expanding
+  // together and then leaving the GPU idle during their Schur phases.
+  std::unique_lock<std::mutex> turn(gpu_turn(c->device), std::defer_lock);
   for (int it = 0; it <= itermax; ++it) {
+    if (!turn.owns_lock()) turn.lock();
     for (int j = k; j < m; j += bq) {
       zd *Xj = V + (int64_t)j * n;
       zd *Wn = V + (int64_t)(j + bq) * n;  // next block slot
       if (pf) pf->start_sweep();
-      int e0 = pf ? pf->begin(s) : -1;
-      launch_spmm(nb, n, bq, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, t1, (int64_t)bq * n, s);
-      if (pf) pf->end(s, e0, 4, nbd * (20.0 * nzd + 8.0 * (nd + 1) + 32.0 * nd * bq));
+      launch_spmm(nb, n, bq, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, t1, (int64_t)bq * n, s,
+                  pf ? pf->time(4, nbd * (20.0 * nzd + 8.0 * (nd + 1) + 32.0 * nd * bq)) : KTime{});
       lu_solve(dm, nb, bq, inv, sig, t1, ty, wv, tw, s, pf);
       launch_copy(nb, bq * n, wv, (int64_t)bq * n, Wn, sV, s);
+      if (delayed) {
+        // Xj is the provisional block Z; one pass over V[:, :j] for Z's
+        // reorthogonalisation and W's projection, then [Q_j, Zraw] in place
+        const int c0 = j == k ? 0 : j - bq;
+        if (j > 0) {
+          launch_atb(nb, n, j, 2 * bq, V, n, sV, Xj, n, sV, Pd, ldp, sP, true, atbs, atb_elems, s,
+                     pf ? pf->time(2, nbd * 16.0 * nd * (j + 2 * bq)) : KTime{});
+          launch_gemm_nn(nb, j, bq, j, zmk(1, 0), Hd, ld, sH, Pd, ldp, sP, zmk(0, 0), A0d, ldp, sA0, s);
+        }
+        launch_atb(nb, n, 2 * bq, 2 * bq, Xj, n, sV, Xj, n, sV, G16, 2 * bq, 4 * bq * bq, true, atbs, atb_elems, s);
+        launch_dcgs_coef(nb, false, j, c0, Pd, ldp, sP, G16, A0d, ldp, sA0, Hd, ld, sH, Ed, ldp, sP, Ld, bad, s);
+        launch_combine(nb, n, j, 2 * bq, V, n, sV, Ed, ldp, sP, Ld, Xj, n, sV, s,
+                       pf ? pf->time(3, nbd * 16.0 * nd * (j + 4 * bq)) : KTime{});
+        block_cholqr2(c, nb, n, bq, Wn, sV, G, Ri, R1, Rf, bad, atbs, atb_elems, s);
+        launch_block_copy(nb, bq, bq, Rf, bq, (int64_t)bq * bq, Hd + (int64_t)j * ld + j + bq, ld, sH, false, s);
+        res.ops += (int64_t)nb * bq;
+        res.steps += 1;
+        continue;
+      }
       // block CGS, two passes
       for (int pass = 0; pass < 2; ++pass) {
         zd *hc = pass == 0 ? Hd + (int64_t)j * ld : H2;
         int64_t hld = pass == 0 ? ld : ld, hs = pass == 0 ? sH : (int64_t)ld * bq;
-        int e1 = pf ? pf->begin(s) : -1;
-        launch_atb(nb, n, j + bq, bq, V, n, sV, Wn, n, sV, hc, hld, hs, true, atbs, atb_elems, s);
-        if (pf) pf->end(s, e1, 2, nbd * 16.0 * nd * (j + 2 * bq));
-        int e2 = pf ? pf->begin(s) : -1;
-        launch_sub_vh_blk(nb, n, j + bq, bq, V, n, sV, hc, hld, hs, Wn, n, sV, s);
-        if (pf) pf->end(s, e2, 3, nbd * 16.0 * nd * (j + 3 * bq));
+        launch_atb(nb, n, j + bq, bq, V, n, sV, Wn, n, sV, hc, hld, hs, true, atbs, atb_elems, s,
+                   pf ? pf->time(2, nbd * 16.0 * nd * (j + 2 * bq)) : KTime{});
+        launch_sub_vh_blk(nb, n, j + bq, bq, V, n, sV, hc, hld, hs, Wn, n, sV, s,
+                          pf ? pf->time(3, nbd * 16.0 * nd * (j + 3 * bq)) : KTime{});
       }
       launch_block_copy(nb, j + bq, bq, H2, ld, (int64_t)ld * bq, Hd + (int64_t)j * ld, ld, sH, true, s);
       block_cholqr2(c, nb, n, bq, Wn, sV, G, Ri, R1, Rf, bad, atbs, atb_elems, s);
@@ -666,9 +729,20 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
       res.ops += (int64_t)nb * bq;
       res.steps += 1;
     }
+    if (delayed) {
+      // reorthogonalise the last provisional block (the residual block of the
+      // Krylov-Schur relation) and correct row block m of H
+      launch_atb(nb, n, m, bq, V, n, sV, V + (int64_t)m * n, n, sV, Pd, ldp, sP, true, atbs, atb_elems, s);
+      launch_atb(nb, n, bq, bq, V + (int64_t)m * n, n, sV, V + (int64_t)m * n, n, sV, G16, bq, bq * bq, true, atbs,
+                 atb_elems, s);
+      launch_dcgs_coef(nb, true, m, m - bq, Pd, ldp, sP, G16, nullptr, ldp, sA0, Hd, ld, sH, Ed, ldp, sP, Ld, bad,
+                       s);
+      launch_combine(nb, n, m, bq, V, n, sV, Ed, ldp, sP, Ld, V + (int64_t)m * n, n, sV, s);
+    }
     tm.mark("krylov_expand");
     EMU_HIP(hipMemcpyAsync(Hh.data(), Hd, sizeof(zd) * Hh.size(), hipMemcpyDeviceToHost, s));
     EMU_HIP(hipStreamSynchronize(s));
+    turn.unlock();
     if (c->prof.on) c->prof.collect();
     bool last = (it == itermax);
     all_conv = true;
@@ -720,6 +794,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
       }
       S.nlock = S.nlock_next;
     }
+    turn.lock();
     EMU_HIP(hipMemcpyAsync(Qd, Qh.data(), sizeof(zd) * Qh.size(), hipMemcpyHostToDevice, s));
     EMU_HIP(hipMemcpyAsync(Hd, Hn.data(), sizeof(zd) * Hn.size(), hipMemcpyHostToDevice, s));
     EMU_HIP(hipStreamSynchronize(s));

@@ -1,5 +1,11 @@
 // Dense / vector kernels of the Krylov-Schur eigensolver and of the overlap
 // contractions, batched over instances (blockIdx.y / blockIdx.z = instance).
+#include <rocblas/rocblas.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+
 #include "engine.h"
 
 namespace emu {
@@ -55,9 +61,10 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, int ncol, const int64_t
 }
 
 void launch_spmm(int nb, int64_t n, int ncol, const int64_t *rp, const int32_t *ci, const zd *vals,
-                 int64_t vstride, const zd *x, int64_t xstride, zd *y, int64_t ystride, hipStream_t s) {
+                 int64_t vstride, const zd *x, int64_t xstride, zd *y, int64_t ystride, hipStream_t s, KTime kt) {
   dim3 g((unsigned)((n + 31) / 32), nb, (unsigned)((ncol + SM_C - 1) / SM_C));
-  hipLaunchKernelGGL(k_spmm, g, dim3(256), 0, s, n, ncol, rp, ci, vals, vstride, x, xstride, y, ystride);
+  hipExtLaunchKernelGGL(k_spmm, g, dim3(256), 0, s, kt.a, kt.b, 0, n, ncol, rp, ci, vals, vstride, x, xstride, y,
+                        ystride);
   EMU_LAUNCH_CHECK();
 }
 
@@ -111,113 +118,170 @@ __global__ __launch_bounds__(256) void k_atb_partial(int64_t K, int M, int N, in
     }
 }
 
-__global__ void k_atb_reduce(int M, int N, int nchunk, const zd *__restrict__ part, zd *__restrict__ C,
-                             int64_t ldc, int64_t sC) {
-  int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  int b = blockIdx.y;
-  if (idx >= (int64_t)M * N) return;
-  int m = idx % M, nn = idx / M;
+// ------------------------------------------ C = op(A)^T B for N <= 8 columns
+// The Gram-Schmidt projection h = V^H w: A (K x M) is the Krylov basis, read
+// once from HBM; B (K x NQ) is the new block.  A workgroup owns 64 columns of
+// A and one K chunk.  Per 32-row step it stages the 32 x 64 tile of A (read
+// along the rows, coalesced) and the 32 x NQ tile of B in LDS; lane c of each
+// wave then accumulates column c against all NQ columns of B over the rows of
+// its wave (rows w, w + 4, ...; the B reads are wave-uniform LDS broadcasts).
+// The next tile is fetched into registers while the current one is used.  At
+// the end the four waves' sums are combined in LDS and written as the chunk's
+// partial product; k_atb_reduce adds the chunks.
+constexpr int TT_M = 64, TT_K = 32;
+
+template <int NQ>
+__global__ __launch_bounds__(256) void k_atb_tall(int64_t K, int M, int64_t kchunk, const zd *__restrict__ A,
+                                                  int64_t lda, int64_t sA, const zd *__restrict__ B, int64_t ldb,
+                                                  int64_t sB, int conjA, zd *__restrict__ part) {
+  static_assert(NQ >= 1 && NQ <= 16, "tall kernel covers N <= 16");
+  const int m0 = blockIdx.x * TT_M;
+  const int chunk = blockIdx.y, nchunk = gridDim.y;
+  const int b = blockIdx.z;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const zd *Ab = A + b * sA;
+  const zd *Bb = B + b * sB;
+  __shared__ zd As[TT_K][TT_M + 1];
+  __shared__ zd Bs[TT_K][NQ];
+  const int64_t kb = chunk * kchunk, ke = min(K, kb + kchunk);
+  const int lr = tid & 31, lc = tid >> 5;  // staging: row lr, columns lc + 8 i
+  zd ra[8], rb = zmk(0, 0), rb2 = zmk(0, 0);
+  auto fetch = [&](int64_t k0) {
+    const int64_t k = k0 + lr;
+    const bool okk = k < ke;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int m = m0 + lc + 8 * i;
+      ra[i] = (okk && m < M) ? Ab[k + (int64_t)m * lda] : zmk(0, 0);
+    }
+    if (lc < NQ) rb = okk ? Bb[k + (int64_t)lc * ldb] : zmk(0, 0);
+    if (NQ > 8 && lc + 8 < NQ) rb2 = okk ? Bb[k + (int64_t)(lc + 8) * ldb] : zmk(0, 0);
+  };
+  zd acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = zmk(0, 0);
+  if (kb < ke) fetch(kb);
+  for (int64_t k0 = kb; k0 < ke; k0 += TT_K) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) As[lr][lc + 8 * i] = conjA ? zconj(ra[i]) : ra[i];
+    if (lc < NQ) Bs[lr][lc] = rb;
+    if (NQ > 8 && lc + 8 < NQ) Bs[lr][lc + 8] = rb2;
+    __syncthreads();
+    if (k0 + TT_K < ke) fetch(k0 + TT_K);
+#pragma unroll
+    for (int kk = wv; kk < TT_K; kk += 4) {
+      const zd a = As[kk][lane];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) acc[q] = zfma(a, Bs[kk][q], acc[q]);
+    }
+  }
+  __syncthreads();
+  // combine the four waves, eight columns of B at a time (LDS of the A tile
+  // reused: 4 x 64 x 8 <= 32 x 65)
+  zd *red = &As[0][0];
+  constexpr int QH = NQ < 8 ? NQ : 8;
+#pragma unroll
+  for (int h = 0; h < NQ; h += QH) {
+#pragma unroll
+    for (int q = 0; q < QH; ++q) red[(wv * TT_M + lane) * QH + q] = acc[h + q];
+    __syncthreads();
+    for (int o = tid; o < TT_M * QH; o += 256) {
+      const int c = o % TT_M, q = o / TT_M;
+      zd sum = red[c * QH + q];
+#pragma unroll
+      for (int w = 1; w < 4; ++w) sum = zadd(sum, red[(w * TT_M + c) * QH + q]);
+      if (m0 + c < M) part[(((int64_t)b * nchunk + chunk) * NQ + h + q) * M + m0 + c] = sum;
+    }
+    __syncthreads();
+  }
+}
+
+// C[b][m + nn ldc] = sum over chunks of part[b][chunk][nn][m]: 64 outputs per
+// workgroup, its four waves summing every fourth chunk, combined in LDS.
+__global__ __launch_bounds__(256) void k_atb_reduce(int M, int N, int nchunk, const zd *__restrict__ part,
+                                                    zd *__restrict__ C, int64_t ldc, int64_t sC) {
+  __shared__ zd red[4][64];
+  const int b = blockIdx.y;
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int64_t idx = blockIdx.x * 64ll + lane, tot = (int64_t)M * N;
   zd acc = zmk(0, 0);
-  for (int c = 0; c < nchunk; ++c) acc = zadd(acc, part[(((int64_t)b * nchunk + c) * N + nn) * M + m]);
-  C[b * sC + m + (int64_t)nn * ldc] = acc;
+  if (idx < tot)
+    for (int c = g; c < nchunk; c += 4) acc = zadd(acc, part[((int64_t)b * nchunk + c) * tot + idx]);
+  red[g][lane] = acc;
+  __syncthreads();
+  if (g == 0 && idx < tot) {
+    const int m = (int)(idx % M), nn = (int)(idx / M);
+    C[b * sC + m + (int64_t)nn * ldc] = zadd(zadd(red[0][lane], red[1][lane]), zadd(red[2][lane], red[3][lane]));
+  }
 }
 
 void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64_t sA, const zd *B,
                 int64_t ldb, int64_t sB, zd *C, int64_t ldc, int64_t sC, bool conjA, zd *scratch,
-                size_t scratch_elems, hipStream_t s) {
-  int tilesM = (M + AT - 1) / AT, tilesN = (N + AT - 1) / AT;
+                size_t scratch_elems, hipStream_t s, KTime kt) {
+  const bool tall = N == 1 || N == 2 || N == 4 || N == 8 || N == 16;
+  const int kstep = tall ? TT_K : AKC;
+  const int tiles = tall ? (M + TT_M - 1) / TT_M : ((M + AT - 1) / AT) * ((N + AT - 1) / AT);
   int64_t per = (int64_t)nb * M * N;
   int64_t maxchunk = std::max<int64_t>(1, (int64_t)(scratch_elems / std::max<int64_t>(per, 1)));
-  int64_t want = std::max<int64_t>(1, 2048 / std::max(1, tilesM * tilesN * nb));
-  int64_t nchunk = std::min<int64_t>(std::min<int64_t>(want, maxchunk), (K + AKC - 1) / AKC);
-  nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, 65535));
+  int64_t want = std::max<int64_t>(1, 2048 / std::max(1, tiles * nb));
+  int64_t nchunk = std::min<int64_t>(std::min<int64_t>(want, maxchunk), (K + kstep - 1) / kstep);
+  nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, tall ? 256 : 65535));
   int64_t kchunk = (K + nchunk - 1) / nchunk;
-  kchunk = ((kchunk + AKC - 1) / AKC) * AKC;
+  kchunk = ((kchunk + kstep - 1) / kstep) * kstep;
   nchunk = (K + kchunk - 1) / kchunk;
   if ((size_t)(nchunk * per) > scratch_elems) throw HipError("atb: scratch too small");
-  hipLaunchKernelGGL(k_atb_partial, dim3(tilesM * tilesN, (unsigned)nchunk, nb), dim3(256), 0, s, K, M,
-                     N, kchunk, A, lda, sA, B, ldb, sB, conjA ? 1 : 0, scratch, tilesM);
+  const int cj = conjA ? 1 : 0;
+  if (tall) {
+    dim3 g((unsigned)tiles, (unsigned)nchunk, nb), t(256);
+    switch (N) {
+      case 1: hipExtLaunchKernelGGL(k_atb_tall<1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+      case 2: hipExtLaunchKernelGGL(k_atb_tall<2>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+      case 4: hipExtLaunchKernelGGL(k_atb_tall<4>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+      case 8: hipExtLaunchKernelGGL(k_atb_tall<8>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+      default: hipExtLaunchKernelGGL(k_atb_tall<16>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+    }
+  } else {
+    hipLaunchKernelGGL(k_atb_partial, dim3(tiles, (unsigned)nchunk, nb), dim3(256), 0, s, K, M, N, kchunk, A, lda,
+                       sA, B, ldb, sB, cj, scratch, (M + AT - 1) / AT);
+  }
   EMU_LAUNCH_CHECK();
-  int64_t tot = (int64_t)M * N;
-  hipLaunchKernelGGL(k_atb_reduce, dim3((unsigned)((tot + 255) / 256), nb), dim3(256), 0, s, M, N,
-                     (int)nchunk, scratch, C, ldc, sC);
+  const int64_t tot = (int64_t)M * N;
+  hipLaunchKernelGGL(k_atb_reduce, dim3((unsigned)((tot + 63) / 64), nb), dim3(256), 0, s, M, N, (int)nchunk,
+                     scratch, C, ldc, sC);
   EMU_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------ C = alpha A B + beta C (NN)
-constexpr int GT = 64;
-constexpr int GK = 16;
-
-__global__ __launch_bounds__(256) void k_gemm_nn(int64_t M, int N, int K, zd alpha, const zd *__restrict__ A,
-                                                 int64_t lda, int64_t sA, const zd *__restrict__ B,
-                                                 int64_t ldb, int64_t sB, zd beta, zd *__restrict__ C,
-                                                 int64_t ldc, int64_t sC) {
-  const int64_t m0 = (int64_t)blockIdx.x * GT;
-  const int n0 = blockIdx.y * GT;
-  const int b = blockIdx.z;
-  const int tid = threadIdx.x;
-  const zd *Ab = A + b * sA;
-  const zd *Bb = B + b * sB;
-  zd *Cb = C + b * sC;
-  __shared__ zd As[GK][GT];
-  __shared__ zd Bs[GK][GT + 1];
-  const int tx = tid % 16, ty = tid / 16;
-  zd acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = zmk(0, 0);
-  for (int k0 = 0; k0 < K; k0 += GK) {
-    for (int idx = tid; idx < GK * GT; idx += 256) {
-      int i = idx % GT, kk = idx / GT;
-      int64_t m = m0 + i;
-      int k = k0 + kk;
-      As[kk][i] = (m < M && k < K) ? Ab[m + (int64_t)k * lda] : zmk(0, 0);
-    }
-    for (int idx = tid; idx < GK * GT; idx += 256) {
-      int kk = idx % GK, j = idx / GK;
-      int k = k0 + kk, nn = n0 + j;
-      Bs[kk][j] = (k < K && nn < N) ? Bb[k + (int64_t)nn * ldb] : zmk(0, 0);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < GK; ++kk) {
-      zd a[4], bb[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bb[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = zfma(a[i], bb[j], acc[i][j]);
-    }
-    __syncthreads();
-  }
-  const bool bz = (beta.x == 0.0 && beta.y == 0.0);
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    int nn = n0 + tx + 16 * j;
-    if (nn >= N) continue;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int64_t m = m0 + ty + 16 * i;
-      if (m < M) {
-        zd *p = &Cb[m + (int64_t)nn * ldc];
-        zd v = zmul(alpha, acc[i][j]);
-        *p = bz ? v : zfma(beta, *p, v);
-      }
-    }
-  }
+// Plain batched GEMMs (restart combination V Q, Ritz vectors, star products)
+// go to rocBLAS' strided-batched zgemm: one handle per stream, created on
+// first use and kept for the process, atomics off so that results do not
+// depend on the run.
+static rocblas_handle blas_handle(hipStream_t s) {
+  static std::mutex mu;
+  static std::map<hipStream_t, rocblas_handle> handles;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = handles.find(s);
+  if (it != handles.end()) return it->second;
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) throw HipError("rocblas_create_handle failed");
+  if (rocblas_set_stream(h, s) != rocblas_status_success ||
+      rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed) != rocblas_status_success)
+    throw HipError("rocblas handle set-up failed");
+  handles[s] = h;
+  return h;
 }
 
 void launch_gemm_nn(int nb, int64_t M, int N, int K, zd alpha, const zd *A, int64_t lda, int64_t sA,
                     const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,
                     hipStream_t s) {
-  if (M <= 0 || N <= 0) return;
-  dim3 g((unsigned)((M + GT - 1) / GT), (unsigned)((N + GT - 1) / GT), nb);
-  hipLaunchKernelGGL(k_gemm_nn, g, dim3(256), 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC);
-  EMU_LAUNCH_CHECK();
+  if (M <= 0 || N <= 0 || nb <= 0) return;
+  using rz = rocblas_double_complex;
+  const rocblas_status st = rocblas_zgemm_strided_batched_64(
+      blas_handle(s), rocblas_operation_none, rocblas_operation_none, M, N, K, reinterpret_cast<const rz *>(&alpha),
+      reinterpret_cast<const rz *>(A), lda, sA, reinterpret_cast<const rz *>(B), ldb, sB,
+      reinterpret_cast<const rz *>(&beta), reinterpret_cast<rz *>(C), ldc, sC, nb);
+  if (st != rocblas_status_success) throw HipError("rocblas_zgemm_strided_batched failed (status " + std::to_string((int)st) + ")");
 }
 
 // ------------------------------------------------------- w -= V[:, :k] h
@@ -354,76 +418,334 @@ __global__ __launch_bounds__(256) void k_sub_vh_blk(int64_t n, int k, const zd *
 }
 
 void launch_sub_vh_blk(int nb, int64_t n, int k, int bq, const zd *V, int64_t ldv, int64_t sV, const zd *H,
-                       int64_t ldh, int64_t sH, zd *W, int64_t ldw, int64_t sW, hipStream_t s) {
+                       int64_t ldh, int64_t sH, zd *W, int64_t ldw, int64_t sW, hipStream_t s, KTime kt) {
   dim3 g((unsigned)((n + 255) / 256), nb);
+  dim3 t(256);
   switch (bq) {
-    case 1: hipLaunchKernelGGL(k_sub_vh_blk<1>, g, dim3(256), 0, s, n, k, V, ldv, sV, H, ldh, sH, W, ldw, sW); break;
-    case 2: hipLaunchKernelGGL(k_sub_vh_blk<2>, g, dim3(256), 0, s, n, k, V, ldv, sV, H, ldh, sH, W, ldw, sW); break;
-    case 4: hipLaunchKernelGGL(k_sub_vh_blk<4>, g, dim3(256), 0, s, n, k, V, ldv, sV, H, ldh, sH, W, ldw, sW); break;
-    case 8: hipLaunchKernelGGL(k_sub_vh_blk<8>, g, dim3(256), 0, s, n, k, V, ldv, sV, H, ldh, sH, W, ldw, sW); break;
+    case 1: hipExtLaunchKernelGGL(k_sub_vh_blk<1>, g, t, 0, s, kt.a, kt.b, 0, n, k, V, ldv, sV, H, ldh, sH, W, ldw, sW); break;
+    case 2: hipExtLaunchKernelGGL(k_sub_vh_blk<2>, g, t, 0, s, kt.a, kt.b, 0, n, k, V, ldv, sV, H, ldh, sH, W, ldw, sW); break;
+    case 4: hipExtLaunchKernelGGL(k_sub_vh_blk<4>, g, t, 0, s, kt.a, kt.b, 0, n, k, V, ldv, sV, H, ldh, sH, W, ldw, sW); break;
+    case 8: hipExtLaunchKernelGGL(k_sub_vh_blk<8>, g, t, 0, s, kt.a, kt.b, 0, n, k, V, ldv, sV, H, ldh, sH, W, ldw, sW); break;
     default: throw HipError("sub_vh_blk: block size must be 1, 2, 4 or 8");
   }
   EMU_LAUNCH_CHECK();
 }
 
-// Cholesky of the bq x bq Gram matrix G = R^H R (R upper), its inverse, and
-// Racc = R * Rprev (Rprev = identity when null).  One thread per instance.
-// A non-positive pivot (rank-deficient block) is replaced by a tiny value
-// and counted in bad[b].
-__global__ void k_cholqr(int nb, int bq, const zd *__restrict__ G, zd *__restrict__ Rinv,
-                         const zd *__restrict__ Rprev, zd *__restrict__ Racc, int32_t *__restrict__ bad) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  const zd *g = G + (int64_t)b * bq * bq;
-  zd R[8][8], Ri[8][8];
-  for (int i = 0; i < bq; ++i)
-    for (int j = 0; j < bq; ++j) { R[i][j] = zmk(0, 0); Ri[i][j] = zmk(0, 0); }
-  double gmax = 0;
-  for (int i = 0; i < bq; ++i) gmax = fmax(gmax, g[i + i * bq].x);
-  for (int j = 0; j < bq; ++j) {
-    // R[i][j] for i < j, then R[j][j]
-    for (int i = 0; i < j; ++i) {
-      zd acc = g[i + j * bq];  // G(i, j)
-      for (int k = 0; k < i; ++k) acc = zsub(acc, zmul(zconj(R[k][i]), R[k][j]));
-      R[i][j] = zdiv(acc, R[i][i]);
+// ------------------------------- delayed block Gram-Schmidt (one pass per block)
+// Block Arnoldi where the second Gram-Schmidt pass of block j is delayed into
+// the projection of block j + 1 (the low-synchronisation "DCGS2" scheme of
+// Swirydowicz et al., Numer. Linear Algebra Appl. 28 (2021), here in block
+// form): the operator is applied to the provisional block Z (projected once),
+// and a single read of the basis Vt = V[:, :j] gives both Vt^H Z (Z's
+// reorthogonalisation) and Vt^H W (W = OP Z's first projection), so each
+// block costs two passes over the basis instead of four.  The Arnoldi
+// relation OP V[:, :j] = V[:, :j+8] H[:j+8, :j] is kept exact by correcting
+// row block j of H when Z is reorthogonalised and by removing OP Vt S (known
+// from H) from W.
+
+// W[b] (n x NQ, in place) = W[b] L[b] - V[b][:, :k] E[b] (L: NQ x NQ, column-
+// major; E: k x NQ, ld lde).  Thread per row; E staged through LDS.
+template <int NQ>
+__global__ __launch_bounds__(256) void k_combine(int64_t n, int k, const zd *__restrict__ V, int64_t ldv,
+                                                 int64_t sV, const zd *__restrict__ E, int64_t lde, int64_t sE,
+                                                 const zd *__restrict__ L, zd *__restrict__ W, int64_t ldw,
+                                                 int64_t sW) {
+  __shared__ zd es[32][NQ];
+  __shared__ zd ls[NQ][NQ];
+  const int b = blockIdx.y;
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const zd *Vb = V + b * sV;
+  const zd *Eb = E + b * sE;
+  for (int idx = threadIdx.x; idx < NQ * NQ; idx += blockDim.x) ls[idx % NQ][idx / NQ] = L[(int64_t)b * NQ * NQ + idx];
+  __syncthreads();
+  zd acc[NQ];
+  {
+    zd x[NQ];
+#pragma unroll
+    for (int p = 0; p < NQ; ++p) x[p] = (r < n) ? W[b * sW + p * ldw + r] : zmk(0, 0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      zd a = zmk(0, 0);
+#pragma unroll
+      for (int p = 0; p < NQ; ++p) a = zfma(x[p], ls[p][q], a);
+      acc[q] = a;
     }
-    double d = g[j + j * bq].x;
-    for (int k = 0; k < j; ++k) d -= R[k][j].x * R[k][j].x + R[k][j].y * R[k][j].y;
-    if (!(d > 1e-28 * gmax)) {
-      d = 1e-28 * gmax + 1e-300;
-      bad[b] += 1;
-    }
-    R[j][j] = zmk(sqrt(d), 0);
   }
-  // inverse of upper triangular R
-  for (int j = 0; j < bq; ++j) {
-    Ri[j][j] = zdiv(zmk(1, 0), R[j][j]);
-    for (int i = j - 1; i >= 0; --i) {
-      zd acc = zmk(0, 0);
-      for (int k = i + 1; k <= j; ++k) acc = zadd(acc, zmul(R[i][k], Ri[k][j]));
-      Ri[i][j] = zsub(zmk(0, 0), zdiv(acc, R[i][i]));
+  for (int c0 = 0; c0 < k; c0 += 32) {
+    const int cn = min(32, k - c0);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 32 * NQ; idx += blockDim.x) {
+      const int c = idx / NQ, q = idx - c * NQ;
+      es[c][q] = c < cn ? Eb[(c0 + c) + (int64_t)q * lde] : zmk(0, 0);
     }
-  }
-  zd *ri = Rinv + (int64_t)b * bq * bq;
-  zd *ra = Racc + (int64_t)b * bq * bq;
-  const zd *rp = Rprev ? Rprev + (int64_t)b * bq * bq : nullptr;
-  for (int i = 0; i < bq; ++i)
-    for (int j = 0; j < bq; ++j) {
-      ri[i + j * bq] = Ri[i][j];
-      zd acc = zmk(0, 0);
-      if (rp) {
-        for (int k = 0; k < bq; ++k) acc = zadd(acc, zmul(R[i][k], rp[k + j * bq]));
-      } else {
-        acc = R[i][j];
+    __syncthreads();
+    if (r < n)
+      for (int c = 0; c < cn; ++c) {
+        const zd v = Vb[r + (int64_t)(c0 + c) * ldv];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[q] = zfms(v, es[c][q], acc[q]);
       }
-      ra[i + j * bq] = acc;
+  }
+  if (r < n)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) W[b * sW + q * ldw + r] = acc[q];
+}
+
+void launch_combine(int nb, int64_t n, int k, int nq, const zd *V, int64_t ldv, int64_t sV, const zd *E,
+                    int64_t lde, int64_t sE, const zd *L, zd *W, int64_t ldw, int64_t sW, hipStream_t s, KTime kt) {
+  dim3 g((unsigned)((n + 255) / 256), nb), t(256);
+  switch (nq) {
+    case 8: hipExtLaunchKernelGGL(k_combine<8>, g, t, 0, s, kt.a, kt.b, 0, n, k, V, ldv, sV, E, lde, sE, L, W, ldw, sW); break;
+    case 16: hipExtLaunchKernelGGL(k_combine<16>, g, t, 0, s, kt.a, kt.b, 0, n, k, V, ldv, sV, E, lde, sE, L, W, ldw, sW); break;
+    default: throw HipError("combine: 8 or 16 columns");
+  }
+  EMU_LAUNCH_CHECK();
+}
+
+// Coefficients of one delayed step, one workgroup per instance (block size 8).
+//   P = Vt^H [Z W] (j x 16, ld ldp): S = P[:, :8], T = P[:, 8:]
+//   G = [Z W]^H [Z W] (16 x 16);  A0 = H[:j, :j] S (j x 8, ld lda), old H
+//   row block j of H (rows j..j+7) is nonzero in columns [c0, j)
+// Z's reorthogonalisation: C = Z^H Z - S^H S = R^H R, Q = (Z - Vt S) R^-1,
+// H[:j, c0:j] += S H[j:j+8, c0:j], H[j:j+8, c0:j] = R H[j:j+8, c0:j].
+// W's first projection against [Vt, Q] with OP Vt S = [Vt, Q] H S removed:
+//   u = H_old[j:j+8, :j] S, a = A0 + S u, b = R u,
+//   cV = (T - a) R^-1, cQ = R^-H (Z^H W - S^H T) R^-1 - b R^-1  (column j of H),
+//   D = b R^-1 + cQ, E = [S R^-1, a R^-1 + cV - S R^-1 D],
+//   L = [[R^-1, -R^-1 D], [0, R^-1]], so that [Q, Zraw] = [Z W] L - Vt E.
+// FIN (no W; the last provisional block of an expansion): only Z's
+// reorthogonalisation, E = S R^-1 (j x 8), L = R^-1.
+template <bool FIN>
+__global__ __launch_bounds__(256) void k_dcgs_coef(int j, int c0, const zd *__restrict__ P, int64_t ldp, int64_t sP,
+                                                   const zd *__restrict__ G, const zd *__restrict__ A0, int64_t lda,
+                                                   int64_t sA, zd *__restrict__ H, int64_t ldh, int64_t sH,
+                                                   zd *__restrict__ E, int64_t lde, int64_t sE, zd *__restrict__ L,
+                                                   int32_t *__restrict__ bad) {
+  constexpr int B = 8;
+  constexpr int NG = FIN ? B : 2 * B;
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const zd *Pb = P + b * sP;
+  const zd *Gb = G + (int64_t)b * NG * NG;
+  zd *Hb = H + b * sH;
+  zd *Eb = E + b * sE;
+  __shared__ zd SS[B][B], ST[B][B], U[B][B], R[B][B], Ri[B][B], Bm[B][B], CQ[B][B], Dm[B][B], T1[B][B];
+  __shared__ zd Gs[NG][NG];
+  for (int idx = tid; idx < NG * NG; idx += blockDim.x) Gs[idx % NG][idx / NG] = Gb[idx];
+  // 1. S^H S, S^H T, u = Hrow S (one output per thread)
+  if (tid < 3 * B * B) {
+    const int o = tid % (B * B), kind = tid / (B * B), p = o % B, q = o / B;
+    zd acc = zmk(0, 0);
+    if (kind == 0) {
+      for (int i = 0; i < j; ++i) acc = zfma(zconj(Pb[i + p * ldp]), Pb[i + q * ldp], acc);
+      SS[p][q] = acc;
+    } else if (kind == 1) {
+      if (!FIN)
+        for (int i = 0; i < j; ++i) acc = zfma(zconj(Pb[i + p * ldp]), Pb[i + (B + q) * ldp], acc);
+      ST[p][q] = acc;
+    } else {
+      if (!FIN)
+        for (int c = c0; c < j; ++c) acc = zfma(Hb[(j + p) + (int64_t)c * ldh], Pb[c + q * ldp], acc);
+      U[p][q] = acc;
     }
+  }
+  __syncthreads();
+  // 2. Cholesky of C = Z^H Z - S^H S and the inverse of R (one thread)
+  if (tid == 0) {
+    double gmax = 0;
+    for (int i = 0; i < B; ++i) gmax = fmax(gmax, Gs[i][i].x);
+    for (int c = 0; c < B; ++c) {
+      for (int i = 0; i < B; ++i) { R[i][c] = zmk(0, 0); Ri[i][c] = zmk(0, 0); }
+    }
+    for (int c = 0; c < B; ++c) {
+      for (int i = 0; i < c; ++i) {
+        zd acc = zsub(Gs[i][c], SS[i][c]);
+        for (int k = 0; k < i; ++k) acc = zsub(acc, zmul(zconj(R[k][i]), R[k][c]));
+        R[i][c] = zdiv(acc, R[i][i]);
+      }
+      double d = Gs[c][c].x - SS[c][c].x;
+      for (int k = 0; k < c; ++k) d -= R[k][c].x * R[k][c].x + R[k][c].y * R[k][c].y;
+      if (!(d > 1e-28 * gmax)) {
+        d = 1e-28 * gmax + 1e-300;
+        bad[b] += 1;
+      }
+      R[c][c] = zmk(sqrt(d), 0);
+    }
+    for (int c = 0; c < B; ++c) {
+      Ri[c][c] = zdiv(zmk(1, 0), R[c][c]);
+      for (int i = c - 1; i >= 0; --i) {
+        zd acc = zmk(0, 0);
+        for (int k = i + 1; k <= c; ++k) acc = zadd(acc, zmul(R[i][k], Ri[k][c]));
+        Ri[i][c] = zsub(zmk(0, 0), zdiv(acc, R[i][i]));
+      }
+    }
+  }
+  __syncthreads();
+  // 3. b = R u; T1 = Z^H W - S^H T; CQ = Ri^H T1 Ri - b Ri; D = Ri^H T1 Ri (64 threads, one entry each)
+  const int p8 = tid % B, q8 = (tid / B) % B;
+  if (!FIN) {
+    if (tid < B * B) {
+      zd acc = zmk(0, 0);
+      for (int k = p8; k < B; ++k) acc = zfma(R[p8][k], U[k][q8], acc);
+      Bm[p8][q8] = acc;
+      T1[p8][q8] = zsub(Gs[p8][B + q8], ST[p8][q8]);
+    }
+    __syncthreads();
+    if (tid < B * B) {  // SS reused: T1 Ri
+      zd acc = zmk(0, 0);
+      for (int k = 0; k <= q8; ++k) acc = zfma(T1[p8][k], Ri[k][q8], acc);
+      SS[p8][q8] = acc;
+    }
+    __syncthreads();
+    if (tid < B * B) {
+      zd acc = zmk(0, 0), bri = zmk(0, 0);
+      for (int k = 0; k <= p8; ++k) acc = zfma(zconj(Ri[k][p8]), SS[k][q8], acc);
+      for (int k = 0; k <= q8; ++k) bri = zfma(Bm[p8][k], Ri[k][q8], bri);
+      CQ[p8][q8] = zsub(acc, bri);
+      Dm[p8][q8] = acc;  // D = b R^-1 + cQ = R^-H T1 R^-1
+    }
+    __syncthreads();
+  }
+  // 4. per basis row i < j: a, cV (column j of H), E; and H[:j, c0:j] += S Hrow
+  for (int i = tid; i < j; i += blockDim.x) {
+    zd Si[B], e1[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) Si[q] = Pb[i + q * ldp];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      zd acc = zmk(0, 0);
+      for (int k = 0; k <= q; ++k) acc = zfma(Si[k], Ri[k][q], acc);
+      e1[q] = acc;
+      Eb[i + (int64_t)q * lde] = acc;
+    }
+    if (!FIN) {
+      zd a[B], cv[B];
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        zd acc = A0[b * sA + i + (int64_t)q * lda];
+        for (int k = 0; k < B; ++k) acc = zfma(Si[k], U[k][q], acc);
+        a[q] = acc;
+      }
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        zd acc = zmk(0, 0);
+        for (int k = 0; k <= q; ++k) acc = zfma(zsub(Pb[i + (B + k) * ldp], a[k]), Ri[k][q], acc);
+        cv[q] = acc;
+        Hb[i + (int64_t)(j + q) * ldh] = acc;
+      }
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        zd acc = cv[q];
+        for (int k = 0; k <= q; ++k) acc = zfma(a[k], Ri[k][q], acc);
+        for (int k = 0; k < B; ++k) acc = zfms(e1[k], Dm[k][q], acc);
+        Eb[i + (int64_t)(B + q) * lde] = acc;
+      }
+    }
+  }
+  const int w = j - c0;
+  for (int64_t idx = tid; idx < (int64_t)j * w; idx += blockDim.x) {
+    const int i = (int)(idx % j), c = c0 + (int)(idx / j);
+    zd acc = Hb[i + (int64_t)c * ldh];
+    for (int k = 0; k < B; ++k) acc = zfma(Pb[i + k * ldp], Hb[(j + k) + (int64_t)c * ldh], acc);
+    Hb[i + (int64_t)c * ldh] = acc;
+  }
+  __syncthreads();
+  // 5. Hrow <- R Hrow, column j's diagonal block, L
+  for (int c = c0 + tid; c < j; c += blockDim.x) {
+    zd h[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) h[k] = Hb[(j + k) + (int64_t)c * ldh];
+#pragma unroll
+    for (int p = 0; p < B; ++p) {
+      zd acc = zmk(0, 0);
+      for (int k = p; k < B; ++k) acc = zfma(R[p][k], h[k], acc);
+      Hb[(j + p) + (int64_t)c * ldh] = acc;
+    }
+  }
+  zd *Lb = L + (int64_t)b * NG * NG;
+  if (FIN) {
+    if (tid < B * B) Lb[p8 + q8 * B] = Ri[p8][q8];
+  } else {
+    if (tid < B * B) {
+      Hb[(j + p8) + (int64_t)(j + q8) * ldh] = CQ[p8][q8];
+      zd rd = zmk(0, 0);
+      for (int k = p8; k < B; ++k) rd = zfma(Ri[p8][k], Dm[k][q8], rd);
+      Lb[p8 + q8 * NG] = Ri[p8][q8];
+      Lb[p8 + (B + q8) * NG] = zsub(zmk(0, 0), rd);
+      Lb[(B + p8) + q8 * NG] = zmk(0, 0);
+      Lb[(B + p8) + (B + q8) * NG] = Ri[p8][q8];
+    }
+  }
+}
+
+void launch_dcgs_coef(int nb, bool fin, int j, int c0, const zd *P, int64_t ldp, int64_t sP, const zd *G,
+                      const zd *A0, int64_t lda, int64_t sA, zd *H, int64_t ldh, int64_t sH, zd *E, int64_t lde,
+                      int64_t sE, zd *L, int32_t *bad, hipStream_t s) {
+  if (fin)
+    hipLaunchKernelGGL(k_dcgs_coef<true>, dim3(nb), dim3(256), 0, s, j, c0, P, ldp, sP, G, A0, lda, sA, H, ldh, sH, E,
+                       lde, sE, L, bad);
+  else
+    hipLaunchKernelGGL(k_dcgs_coef<false>, dim3(nb), dim3(256), 0, s, j, c0, P, ldp, sP, G, A0, lda, sA, H, ldh, sH,
+                       E, lde, sE, L, bad);
+  EMU_LAUNCH_CHECK();
+}
+
+// Cholesky of the bq x bq Gram matrix G = R^H R (R upper), its inverse, and
+// Racc = R * Rprev (Rprev = identity when null).  One 64-thread workgroup per
+// instance: the matrices are loaded into LDS in parallel, one thread runs the
+// (tiny, sequential) factorisation there, and the products are written back
+// one entry per thread.  A non-positive pivot (rank-deficient block) is
+// replaced by a tiny value and counted in bad[b].
+__global__ __launch_bounds__(64) void k_cholqr(int bq, const zd *__restrict__ G, zd *__restrict__ Rinv,
+                                               const zd *__restrict__ Rprev, zd *__restrict__ Racc,
+                                               int32_t *__restrict__ bad) {
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int i = t % 8, j = t / 8;
+  const bool in = i < bq && j < bq;
+  const int64_t o = (int64_t)b * bq * bq + i + j * bq;
+  __shared__ zd g[8][8], R[8][8], Ri[8][8], rp[8][8];
+  g[i][j] = in ? G[o] : zmk(0, 0);
+  rp[i][j] = (in && Rprev) ? Rprev[o] : zmk(i == j ? 1.0 : 0.0, 0);
+  R[i][j] = zmk(0, 0);
+  Ri[i][j] = zmk(0, 0);
+  __syncthreads();
+  if (t == 0) {
+    double gmax = 0;
+    for (int k = 0; k < bq; ++k) gmax = fmax(gmax, g[k][k].x);
+    for (int c = 0; c < bq; ++c) {
+      for (int r = 0; r < c; ++r) {
+        zd acc = g[r][c];
+        for (int k = 0; k < r; ++k) acc = zsub(acc, zmul(zconj(R[k][r]), R[k][c]));
+        R[r][c] = zdiv(acc, R[r][r]);
+      }
+      double d = g[c][c].x;
+      for (int k = 0; k < c; ++k) d -= R[k][c].x * R[k][c].x + R[k][c].y * R[k][c].y;
+      if (!(d > 1e-28 * gmax)) {
+        d = 1e-28 * gmax + 1e-300;
+        bad[b] += 1;
+      }
+      R[c][c] = zmk(sqrt(d), 0);
+    }
+    for (int c = 0; c < bq; ++c) {
+      Ri[c][c] = zdiv(zmk(1, 0), R[c][c]);
+      for (int r = c - 1; r >= 0; --r) {
+        zd acc = zmk(0, 0);
+        for (int k = r + 1; k <= c; ++k) acc = zadd(acc, zmul(R[r][k], Ri[k][c]));
+        Ri[r][c] = zsub(zmk(0, 0), zdiv(acc, R[r][r]));
+      }
+    }
+  }
+  __syncthreads();
+  if (!in) return;
+  Rinv[o] = Ri[i][j];
+  zd acc = zmk(0, 0);
+  for (int k = i; k < bq; ++k) acc = zfma(R[i][k], rp[k][j], acc);
+  Racc[o] = acc;
 }
 
 void launch_cholqr(int nb, int bq, const zd *G, zd *Rinv, const zd *Rprev, zd *Racc, int32_t *bad,
                    hipStream_t s) {
   if (bq > 8) throw HipError("cholqr: block size above 8");
-  hipLaunchKernelGGL(k_cholqr, dim3((nb + 63) / 64), dim3(64), 0, s, nb, bq, G, Rinv, Rprev, Racc, bad);
+  hipLaunchKernelGGL(k_cholqr, dim3(nb), dim3(64), 0, s, bq, G, Rinv, Rprev, Racc, bad);
   EMU_LAUNCH_CHECK();
 }
 

@@ -488,8 +488,10 @@ template <int RC>
 __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, FrontArrays A,
                                              const zd *__restrict__ inv, int64_t isz, const int32_t *__restrict__ sig,
                                              int64_t npiv, int nrhs, int64_t n, int64_t nw,
-                                             const zd *__restrict__ bv, zd *__restrict__ y, zd *__restrict__ wbuf) {
-  const int t = tasks[3 * blockIdx.x], r0 = tasks[3 * blockIdx.x + 1], tpw = tasks[3 * blockIdx.x + 2];
+                                             const zd *__restrict__ bv, zd *__restrict__ y, zd *__restrict__ wbuf,
+                                             zd *__restrict__ part, int nslot) {
+  const int32_t *tk = tasks + 5 * blockIdx.x;
+  const int t = tk[0], r0 = tk[1], tpw = tk[2], kb = tk[3], slot = tk[4];
   const int b = blockIdx.y;
   const int q0 = blockIdx.z * RC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -501,14 +503,20 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
   const int row = r0 + 64 * tile + lane;
-  const int kend = min(s, r0 + 64 * tpw);             // workgroup's column range
-  const int kend_w = min(s, r0 + 64 * (tile + 1));    // lower-triangular rows need k <= row
+  // columns of this task: the workgroup's range, cut to [kb, kb + kSolveSplit)
+  // for a split task; lower-triangular rows need k <= row
+  int kbeg = 0, kend = min(s, r0 + 64 * tpw);
+  if (kb >= 0) {
+    kbeg = kb;
+    kend = min(kend, kb + kSolveSplit);
+  }
+  const int kend_w = min(kend, min(s, r0 + 64 * (tile + 1)));
   __shared__ zd xin[KC][RC];
   __shared__ zd red[3][64][RC];
   zd acc[RC];
 #pragma unroll
   for (int q = 0; q < RC; ++q) acc[q] = zmk(0, 0);
-  for (int kc = 0; kc < kend; kc += KC) {
+  for (int kc = kbeg; kc < kend; kc += KC) {
     for (int idx = tid; idx < KC * RC; idx += 256) {
       const int k = idx % KC, q = idx / KC;
       const int kk = kc + k;
@@ -545,7 +553,12 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
 #pragma unroll
         for (int q = 0; q < RC; ++q) acc[q] = zadd(acc[q], red[wave + j - 1][lane][q]);
   }
-  if (kp == 0 && row < f) {
+  if (kp != 0) return;
+  if (slot >= 0) {  // split task: partial sums of the tile, added by k_fwd_red
+    for (int q = 0; q < nq; ++q) part[((col0 + q) * nslot + slot) * 64 + lane] = acc[q];
+    return;
+  }
+  if (row < f) {
     for (int q = 0; q < nq; ++q) {
       zd v = acc[q];
       if (row < s) {
@@ -560,11 +573,43 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
   }
 }
 
+// Sum of the partial sums of a split forward tile (task: front, r0, first
+// slot, count; one lane per row), then the forward epilogue of k_fwd.
+template <int RC>
+__global__ __launch_bounds__(64) void k_fwd_red(const int32_t *__restrict__ tasks, FrontArrays A, int nrhs,
+                                                int64_t n, int64_t nw, const zd *__restrict__ part, int nslot,
+                                                zd *__restrict__ y, zd *__restrict__ wbuf) {
+  const int32_t *tk = tasks + 4 * blockIdx.x;
+  const int t = tk[0], r0 = tk[1], p0 = tk[2], np = tk[3];
+  const int b = blockIdx.y, q0 = blockIdx.z * RC, lane = threadIdx.x;
+  const int s = A.s[t], f = A.f[t], fb = A.b[t];
+  const int32_t *cm = A.cmap + 2 * A.fvoff[t];
+  const int nq = min(RC, nrhs - q0);
+  const int64_t col0 = (int64_t)b * nrhs + q0;
+  const int row = r0 + lane;
+  if (row >= f) return;
+  for (int q = 0; q < nq; ++q) {
+    const zd *pp = part + ((col0 + q) * nslot + p0) * 64 + lane;
+    zd v = pp[0];
+    for (int c = 1; c < np; ++c) v = zadd(v, pp[c * 64]);
+    if (row < s) {
+      y[(col0 + q) * n + fb + row] = v;
+    } else {
+      const int c0i = cm[2 * row], c1i = cm[2 * row + 1];
+      if (c0i >= 0) v = zadd(v, wbuf[(col0 + q) * nw + c0i]);
+      if (c1i >= 0) v = zadd(v, wbuf[(col0 + q) * nw + c1i]);
+      wbuf[(col0 + q) * nw + A.uoff[t] + (row - s)] = v;
+    }
+  }
+}
+
 template <int RC>
 __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, FrontArrays A,
                                              const zd *__restrict__ inv, int64_t isz, int nrhs, int64_t n,
-                                             const zd *__restrict__ y, zd *__restrict__ x) {
-  const int t = tasks[3 * blockIdx.x], r0 = tasks[3 * blockIdx.x + 1], tpw = tasks[3 * blockIdx.x + 2];
+                                             const zd *__restrict__ y, zd *__restrict__ x, zd *__restrict__ part,
+                                             int nslot) {
+  const int32_t *tk = tasks + 5 * blockIdx.x;
+  const int t = tk[0], r0 = tk[1], tpw = tk[2], kb = tk[3], slot = tk[4];
   const int b = blockIdx.y;
   const int q0 = blockIdx.z * RC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -574,25 +619,32 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, 
   const int32_t *ui = A.uidx + A.uoff[t];
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
-  const int row = r0 + 64 * tile + lane;
-  const int kbeg_w = r0 + 64 * tile;  // upper-triangular rows need k >= row
+  const int row = r0 + 64 * tile;
+  const int rowl = row + lane;
+  // columns: [r0, f) for the workgroup (upper-triangular rows need k >= row),
+  // cut to [kb, kb + kSolveSplit) for a split task
+  int kbeg = r0, kend = f;
+  if (kb >= 0) {
+    kbeg = kb;
+    kend = min(f, kb + kSolveSplit);
+  }
   __shared__ zd xin[KC][RC];
   __shared__ zd red[3][64][RC];
   zd acc[RC];
 #pragma unroll
   for (int q = 0; q < RC; ++q) acc[q] = zmk(0, 0);
-  for (int kc = r0; kc < f; kc += KC) {
+  for (int kc = kbeg; kc < kend; kc += KC) {
     for (int idx = tid; idx < KC * RC; idx += 256) {
       const int k = idx % KC, q = idx / KC;
       const int kk = kc + k;
       zd v = zmk(0, 0);
-      if (kk < f && q < nq) v = kk < s ? y[(col0 + q) * n + fb + kk] : x[(col0 + q) * n + ui[kk - s]];
+      if (kk < kend && q < nq) v = kk < s ? y[(col0 + q) * n + fb + kk] : x[(col0 + q) * n + ui[kk - s]];
       xin[k][q] = v;
     }
     __syncthreads();
-    const int klo = max(0, kbeg_w - kc), khi = min(KC, f - kc);
-    if (row < s && klo < khi) {
-      const zd *col = Bw + row + (int64_t)kc * s;
+    const int klo = max(0, row - kc), khi = min(KC, kend - kc);
+    if (rowl < s && klo < khi) {
+      const zd *col = Bw + rowl + (int64_t)kc * s;
 #pragma unroll 4
       for (int k = klo + kp; k < khi; k += ks) {
         const zd a = col[(int64_t)k * s];
@@ -612,8 +664,33 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, 
 #pragma unroll
         for (int q = 0; q < RC; ++q) acc[q] = zadd(acc[q], red[wave + j - 1][lane][q]);
   }
-  if (kp == 0 && row < s)
-    for (int q = 0; q < nq; ++q) x[(col0 + q) * n + fb + row] = acc[q];
+  if (kp != 0) return;
+  if (slot >= 0) {  // split task: partial sums, added by k_bwd_red
+    for (int q = 0; q < nq; ++q) part[((col0 + q) * nslot + slot) * 64 + lane] = acc[q];
+    return;
+  }
+  if (rowl < s)
+    for (int q = 0; q < nq; ++q) x[(col0 + q) * n + fb + rowl] = acc[q];
+}
+
+template <int RC>
+__global__ __launch_bounds__(64) void k_bwd_red(const int32_t *__restrict__ tasks, FrontArrays A, int nrhs,
+                                                int64_t n, const zd *__restrict__ part, int nslot,
+                                                zd *__restrict__ x) {
+  const int32_t *tk = tasks + 4 * blockIdx.x;
+  const int t = tk[0], r0 = tk[1], p0 = tk[2], np = tk[3];
+  const int b = blockIdx.y, q0 = blockIdx.z * RC, lane = threadIdx.x;
+  const int s = A.s[t], fb = A.b[t];
+  const int nq = min(RC, nrhs - q0);
+  const int64_t col0 = (int64_t)b * nrhs + q0;
+  const int row = r0 + lane;
+  if (row >= s) return;
+  for (int q = 0; q < nq; ++q) {
+    const zd *pp = part + ((col0 + q) * nslot + p0) * 64 + lane;
+    zd v = pp[0];
+    for (int c = 1; c < np; ++c) v = zadd(v, pp[c * 64]);
+    x[(col0 + q) * n + fb + row] = v;
+  }
 }
 
 // RHS columns per workgroup: 8 when the level has enough workgroups to fill
@@ -628,48 +705,58 @@ static int pick_rc(int nrhs, int64_t tasks, int nb) {
 
 template <int RC>
 static void launch_fwd(const LevelSched &L, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *inv,
-                       const int32_t *sig, const zd *bnew, zd *y, zd *w, hipStream_t s) {
+                       const int32_t *sig, const zd *bnew, zd *y, zd *w, zd *part, hipStream_t s, KTime kt) {
   const Symbolic &sy = dm.host.sym;
   const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
-  hipLaunchKernelGGL(k_fwd<RC>, dim3((unsigned)L.sf_cnt, nb, nz), dim3(256), 0, s, dm.tasks.p + L.sf_off, A, inv,
-                     dm.isz, sig, sy.npiv, nrhs, sy.n, std::max<int64_t>(sy.nw, 1), bnew, y, w);
+  const int64_t nw = std::max<int64_t>(sy.nw, 1);
+  hipExtLaunchKernelGGL(k_fwd<RC>, dim3((unsigned)L.sf_cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
+                        dm.tasks.p + L.sf_off, A, inv, dm.isz, sig, sy.npiv, nrhs, sy.n, nw, bnew, y, w, part,
+                        (int)L.pf_cnt);
+  if (L.rf_cnt > 0)
+    hipLaunchKernelGGL(k_fwd_red<RC>, dim3((unsigned)L.rf_cnt, nb, nz), dim3(64), 0, s, dm.tasks.p + L.rf_off, A,
+                       nrhs, sy.n, nw, part, (int)L.pf_cnt, y, w);
 }
 
 template <int RC>
 static void launch_bwd(const LevelSched &L, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *inv,
-                       const zd *y, zd *x, hipStream_t s) {
+                       const zd *y, zd *x, zd *part, hipStream_t s, KTime kt) {
   const Symbolic &sy = dm.host.sym;
   const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
-  hipLaunchKernelGGL(k_bwd<RC>, dim3((unsigned)L.sb_cnt, nb, nz), dim3(256), 0, s, dm.tasks.p + L.sb_off, A, inv,
-                     dm.isz, nrhs, sy.n, y, x);
+  hipExtLaunchKernelGGL(k_bwd<RC>, dim3((unsigned)L.sb_cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
+                        dm.tasks.p + L.sb_off, A, inv, dm.isz, nrhs, sy.n, y, x, part, (int)L.pb_cnt);
+  if (L.rb_cnt > 0)
+    hipLaunchKernelGGL(k_bwd_red<RC>, dim3((unsigned)L.rb_cnt, nb, nz), dim3(64), 0, s, dm.tasks.p + L.rb_off, A,
+                       nrhs, sy.n, part, (int)L.pb_cnt, x);
 }
+
+int64_t solve_ws(const DevMesh &dm) { return std::max<int64_t>(dm.host.sym.nw, 1) + dm.part_rhs; }
 
 void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *sig, const zd *bnew, zd *y, zd *x,
               zd *w, hipStream_t s, Profiler *prof) {
   FrontArrays A = fa(dm);
   const double vecs = (double)nb * nrhs;
+  // partial sums of split tasks live behind the update vectors in w
+  zd *part = w + (int64_t)nb * nrhs * std::max<int64_t>(dm.host.sym.nw, 1);
   for (const LevelSched &L : dm.sched) {
-    int ev = prof ? prof->begin(s) : -1;
     const int rc = pick_rc(nrhs, L.sf_cnt, nb);
+    KTime kt = prof ? prof->time(0, L.fwd_mat * nb + L.fwd_vec * vecs) : KTime{};
     switch (rc) {
-      case 8: launch_fwd<8>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, s); break;
-      case 4: launch_fwd<4>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, s); break;
-      case 2: launch_fwd<2>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, s); break;
-      default: launch_fwd<1>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, s); break;
+      case 8: launch_fwd<8>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
+      case 4: launch_fwd<4>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
+      case 2: launch_fwd<2>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
+      default: launch_fwd<1>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
     }
-    if (prof) prof->end(s, ev, 0, L.fwd_mat * nb * ((nrhs + rc - 1) / rc) + L.fwd_vec * vecs);
   }
   EMU_LAUNCH_CHECK();
   for (auto it = dm.sched.rbegin(); it != dm.sched.rend(); ++it) {
-    int ev = prof ? prof->begin(s) : -1;
     const int rc = pick_rc(nrhs, it->sb_cnt, nb);
+    KTime kt = prof ? prof->time(1, it->bwd_mat * nb + it->bwd_vec * vecs) : KTime{};
     switch (rc) {
-      case 8: launch_bwd<8>(*it, dm, A, nb, nrhs, inv, y, x, s); break;
-      case 4: launch_bwd<4>(*it, dm, A, nb, nrhs, inv, y, x, s); break;
-      case 2: launch_bwd<2>(*it, dm, A, nb, nrhs, inv, y, x, s); break;
-      default: launch_bwd<1>(*it, dm, A, nb, nrhs, inv, y, x, s); break;
+      case 8: launch_bwd<8>(*it, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
+      case 4: launch_bwd<4>(*it, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
+      case 2: launch_bwd<2>(*it, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
+      default: launch_bwd<1>(*it, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
     }
-    if (prof) prof->end(s, ev, 1, it->bwd_mat * nb * ((nrhs + rc - 1) / rc) + it->bwd_vec * vecs);
   }
   EMU_LAUNCH_CHECK();
 }
