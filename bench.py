@@ -204,8 +204,8 @@ def main():
     dom = max(prof, key=lambda k: prof[k][1])
     cnt, ms, byt = prof[dom]
     achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    kernel_name = {'solve_fwd': 'k_fwd', 'solve_bwd': 'k_bwd', 'gs_project': 'k_atb_partial',
-                   'gs_update': 'k_sub_vh_blk', 'spmv_M': 'k_spmm'}[dom]
+    kernel_name = {'solve_fwd': 'k_fwd', 'solve_bwd': 'k_bwd', 'gs_project': 'k_atb_tall<16>',
+                   'gs_update': 'k_combine<16>', 'spmv_M': 'k_spmm'}[dom]
     line = {
         'metric': 'wavelength-points/s at 8k-tri mesh; mixed vs FP64 GPU at <=1e-6 R/T/A error',
         'value': value, 'unit': 'wavelength-points/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,

@@ -273,17 +273,20 @@ void launch_norm_scale(int nb, int64_t n, const zd *w, int64_t sw, zd *out, int6
 void launch_axpy_h(int nb, int k, const zd *h1, const zd *h2, int64_t sh, zd *H, int64_t sH,
                    hipStream_t s);
 void launch_copy(int nb, int64_t n, const zd *src, int64_t ss, zd *dst, int64_t sd, hipStream_t s);
+// exchange two ranges of n elements (two batch slots' data)
+void launch_swap(int64_t n, zd *a, zd *b, hipStream_t s);
+void launch_swap_i32(int64_t n, int32_t *a, int32_t *b, hipStream_t s);
 // block Gram-Schmidt: W[:, :bq] -= V[:, :k] H[:k, :bq]
 void launch_sub_vh_blk(int nb, int64_t n, int k, int bq, const zd *V, int64_t ldv, int64_t sV, const zd *H,
                        int64_t ldh, int64_t sH, zd *W, int64_t ldw, int64_t sW, hipStream_t s, KTime kt = {});
 // Delayed block Gram-Schmidt (block size 8): W (n x nq, in place) = W L - V[:, :k] E
+// (E row-major, k x nq per instance)
 void launch_combine(int nb, int64_t n, int k, int nq, const zd *V, int64_t ldv, int64_t sV, const zd *E,
-                    int64_t lde, int64_t sE, const zd *L, zd *W, int64_t ldw, int64_t sW, hipStream_t s,
-                    KTime kt = {});
+                    int64_t sE, const zd *L, zd *W, int64_t ldw, int64_t sW, hipStream_t s, KTime kt = {});
 // ... and the coefficients of one step (fin: reorthogonalise the last block only)
-void launch_dcgs_coef(int nb, bool fin, int j, int c0, const zd *P, int64_t ldp, int64_t sP, const zd *G,
-                      const zd *A0, int64_t lda, int64_t sA, zd *H, int64_t ldh, int64_t sH, zd *E, int64_t lde,
-                      int64_t sE, zd *L, int32_t *bad, hipStream_t s);
+void launch_dcgs_coef(int nb, bool fin, int j, int c0, const zd *P, int64_t ldp, int64_t sP, const zd *A0,
+                      int64_t lda, int64_t sA, zd *H, int64_t ldh, int64_t sH, zd *E, int64_t sE, zd *L, int32_t *bad,
+                      hipStream_t s);
 // Cholesky-QR of a bq-column block: G = R^H R, Rinv = R^-1, Racc = R * Rprev
 void launch_cholqr(int nb, int bq, const zd *G, zd *Rinv, const zd *Rprev, zd *Racc, int32_t *bad,
                    hipStream_t s);

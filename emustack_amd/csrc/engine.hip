@@ -645,15 +645,14 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   size_t atb_elems = (size_t)nb * (m + bq) * std::max(bq, 32) * 128;
   zd *atbs = c->buf<zd>("ks_atb", atb_elems);
   zd *Qd = c->buf<zd>("ks_Q", (size_t)nb * m * m);
-  // delayed block Gram-Schmidt (block size 8): projections P = V^H [Z W],
-  // A0 = H S, combination E, L and the 16 x 16 Gram matrix of [Z W]
+  // delayed block Gram-Schmidt (block size 8): projections and Gram matrix
+  // P = [V Z W]^H [Z W], A0 = H S, combination E, L
   const bool delayed = bq == 8;
   const int64_t ldp = m + bq, sP = ldp * 2 * bq, sA0 = ldp * bq;
   zd *Pd = delayed ? c->buf<zd>("ks_P", (size_t)nb * sP) : nullptr;
   zd *A0d = delayed ? c->buf<zd>("ks_A0", (size_t)nb * sA0) : nullptr;
   zd *Ed = delayed ? c->buf<zd>("ks_E", (size_t)nb * sP) : nullptr;
   zd *Ld = delayed ? c->buf<zd>("ks_L", (size_t)nb * 4 * bq * bq) : nullptr;
-  zd *G16 = delayed ? c->buf<zd>("ks_G16", (size_t)nb * 4 * bq * bq) : nullptr;
   EMU_HIP(hipMemsetAsync(bad, 0, sizeof(int32_t) * nb, s));
   // start block: deterministic pseudo-random, orthonormalised
   {
@@ -675,11 +674,33 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   int keep = std::min(m - bq, nev + (m - nev) / 2);
   keep = m - ((m - keep + bq - 1) / bq) * bq;  // (m - keep) multiple of bq
   if (keep < nev) throw std::invalid_argument("Krylov subspace too small for num_BMs");
-  bool all_conv = false;
   std::vector<zd> Hh((size_t)nb * sH);
   std::vector<std::vector<cplx>> ritz(nb);
   Profiler *pf = c->prof.on ? &c->prof : nullptr;
-  const double nbd = nb, nd = (double)n, nzd = (double)dm.host.nnz;
+  const double nd = (double)n, nzd = (double)dm.host.nnz;
+  // Active instances occupy slots 0..nact-1; slot_inst maps a slot to its
+  // instance; converged instances are retired by swapping their slot with
+  // the last active one (device data and host state), logged in swaps
+  int nact = nb;
+  std::vector<int> slot_inst(nb);
+  std::iota(slot_inst.begin(), slot_inst.end(), 0);
+  std::vector<std::pair<int, int>> swaps;
+  zd *inv_m = const_cast<zd *>(inv);          // the caller's buffers: slots are swapped during the
+  int32_t *sig_m = const_cast<int32_t *>(sig);  // solve and restored in order before returning
+  zd *mv_m = const_cast<zd *>(mvals);
+  const int64_t nnzm = dm.host.nnz;
+  auto swap_caller = [&](int a1, int a2) {
+    launch_swap(dm.isz, inv_m + a1 * dm.isz, inv_m + a2 * dm.isz, s);
+    launch_swap_i32(sy.npiv, sig_m + (int64_t)a1 * sy.npiv, sig_m + (int64_t)a2 * sy.npiv, s);
+    launch_swap(nnzm, mv_m + a1 * nnzm, mv_m + a2 * nnzm, s);
+  };
+  auto swap_slots = [&](int a1, int a2) {
+    swap_caller(a1, a2);
+    launch_swap(sV, V + a1 * sV, V + a2 * sV, s);
+    launch_swap(sH, Hd + a1 * sH, Hd + a2 * sH, s);
+    launch_swap_i32(1, bad + a1, bad + a2, s);
+  };
+  zd *Wd = c->buf<zd>("ks_W", (size_t)m * nev);
   // The GPU phases of a cycle (restart combination + expansion) hold the
   // device's turn; the host Schur phase releases it, so that concurrent
   // sub-batches on other contexts alternate on the GPU instead of expanding
@@ -687,30 +708,29 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   std::unique_lock<std::mutex> turn(gpu_turn(c->device), std::defer_lock);
   for (int it = 0; it <= itermax; ++it) {
     if (!turn.owns_lock()) turn.lock();
+    const double nbd = nact;
     for (int j = k; j < m; j += bq) {
       zd *Xj = V + (int64_t)j * n;
       zd *Wn = V + (int64_t)(j + bq) * n;  // next block slot
       if (pf) pf->start_sweep();
-      launch_spmm(nb, n, bq, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, t1, (int64_t)bq * n, s,
-                  pf ? pf->time(4, nbd * (20.0 * nzd + 8.0 * (nd + 1) + 32.0 * nd * bq)) : KTime{});
+      const KTime kspmv = pf ? pf->time(4, nbd * (20.0 * nzd + 8.0 * (nd + 1) + 32.0 * nd * bq)) : KTime{};
+      launch_spmm(nact, n, bq, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, t1, (int64_t)bq * n, s,
+                  kspmv);
       lu_solve(dm, nb, bq, inv, sig, t1, ty, wv, tw, s, pf);
-      launch_copy(nb, bq * n, wv, (int64_t)bq * n, Wn, sV, s);
+      launch_copy(nact, bq * n, wv, (int64_t)bq * n, Wn, sV, s);
       if (delayed) {
         // Xj is the provisional block Z; one pass over V[:, :j] for Z's
         // reorthogonalisation and W's projection, then [Q_j, Zraw] in place
         const int c0 = j == k ? 0 : j - bq;
-        if (j > 0) {
-          launch_atb(nb, n, j, 2 * bq, V, n, sV, Xj, n, sV, Pd, ldp, sP, true, atbs, atb_elems, s,
-                     pf ? pf->time(2, nbd * 16.0 * nd * (j + 2 * bq)) : KTime{});
-          launch_gemm_nn(nb, j, bq, j, zmk(1, 0), Hd, ld, sH, Pd, ldp, sP, zmk(0, 0), A0d, ldp, sA0, s);
-        }
-        launch_atb(nb, n, 2 * bq, 2 * bq, Xj, n, sV, Xj, n, sV, G16, 2 * bq, 4 * bq * bq, true, atbs, atb_elems, s);
-        launch_dcgs_coef(nb, false, j, c0, Pd, ldp, sP, G16, A0d, ldp, sA0, Hd, ld, sH, Ed, ldp, sP, Ld, bad, s);
-        launch_combine(nb, n, j, 2 * bq, V, n, sV, Ed, ldp, sP, Ld, Xj, n, sV, s,
+        launch_atb(nact, n, j + 2 * bq, 2 * bq, V, n, sV, Xj, n, sV, Pd, ldp, sP, true, atbs, atb_elems, s,
+                   pf ? pf->time(2, nbd * 16.0 * nd * (j + 4 * bq)) : KTime{});
+        if (j > 0) launch_gemm_nn(nact, j, bq, j, zmk(1, 0), Hd, ld, sH, Pd, ldp, sP, zmk(0, 0), A0d, ldp, sA0, s);
+        launch_dcgs_coef(nact, false, j, c0, Pd, ldp, sP, A0d, ldp, sA0, Hd, ld, sH, Ed, sP, Ld, bad, s);
+        launch_combine(nact, n, j, 2 * bq, V, n, sV, Ed, sP, Ld, Xj, n, sV, s,
                        pf ? pf->time(3, nbd * 16.0 * nd * (j + 4 * bq)) : KTime{});
-        block_cholqr2(c, nb, n, bq, Wn, sV, G, Ri, R1, Rf, bad, atbs, atb_elems, s);
-        launch_block_copy(nb, bq, bq, Rf, bq, (int64_t)bq * bq, Hd + (int64_t)j * ld + j + bq, ld, sH, false, s);
-        res.ops += (int64_t)nb * bq;
+        block_cholqr2(c, nact, n, bq, Wn, sV, G, Ri, R1, Rf, bad, atbs, atb_elems, s);
+        launch_block_copy(nact, bq, bq, Rf, bq, (int64_t)bq * bq, Hd + (int64_t)j * ld + j + bq, ld, sH, false, s);
+        res.ops += (int64_t)nact * bq;
         res.steps += 1;
         continue;
       }
@@ -718,62 +738,82 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
       for (int pass = 0; pass < 2; ++pass) {
         zd *hc = pass == 0 ? Hd + (int64_t)j * ld : H2;
         int64_t hld = pass == 0 ? ld : ld, hs = pass == 0 ? sH : (int64_t)ld * bq;
-        launch_atb(nb, n, j + bq, bq, V, n, sV, Wn, n, sV, hc, hld, hs, true, atbs, atb_elems, s,
+        launch_atb(nact, n, j + bq, bq, V, n, sV, Wn, n, sV, hc, hld, hs, true, atbs, atb_elems, s,
                    pf ? pf->time(2, nbd * 16.0 * nd * (j + 2 * bq)) : KTime{});
-        launch_sub_vh_blk(nb, n, j + bq, bq, V, n, sV, hc, hld, hs, Wn, n, sV, s,
+        launch_sub_vh_blk(nact, n, j + bq, bq, V, n, sV, hc, hld, hs, Wn, n, sV, s,
                           pf ? pf->time(3, nbd * 16.0 * nd * (j + 3 * bq)) : KTime{});
       }
-      launch_block_copy(nb, j + bq, bq, H2, ld, (int64_t)ld * bq, Hd + (int64_t)j * ld, ld, sH, true, s);
-      block_cholqr2(c, nb, n, bq, Wn, sV, G, Ri, R1, Rf, bad, atbs, atb_elems, s);
-      launch_block_copy(nb, bq, bq, Rf, bq, (int64_t)bq * bq, Hd + (int64_t)j * ld + j + bq, ld, sH, false, s);
-      res.ops += (int64_t)nb * bq;
+      launch_block_copy(nact, j + bq, bq, H2, ld, (int64_t)ld * bq, Hd + (int64_t)j * ld, ld, sH, true, s);
+      block_cholqr2(c, nact, n, bq, Wn, sV, G, Ri, R1, Rf, bad, atbs, atb_elems, s);
+      launch_block_copy(nact, bq, bq, Rf, bq, (int64_t)bq * bq, Hd + (int64_t)j * ld + j + bq, ld, sH, false, s);
+      res.ops += (int64_t)nact * bq;
       res.steps += 1;
     }
     if (delayed) {
       // reorthogonalise the last provisional block (the residual block of the
       // Krylov-Schur relation) and correct row block m of H
-      launch_atb(nb, n, m, bq, V, n, sV, V + (int64_t)m * n, n, sV, Pd, ldp, sP, true, atbs, atb_elems, s);
-      launch_atb(nb, n, bq, bq, V + (int64_t)m * n, n, sV, V + (int64_t)m * n, n, sV, G16, bq, bq * bq, true, atbs,
-                 atb_elems, s);
-      launch_dcgs_coef(nb, true, m, m - bq, Pd, ldp, sP, G16, nullptr, ldp, sA0, Hd, ld, sH, Ed, ldp, sP, Ld, bad,
-                       s);
-      launch_combine(nb, n, m, bq, V, n, sV, Ed, ldp, sP, Ld, V + (int64_t)m * n, n, sV, s);
+      launch_atb(nact, n, m + bq, bq, V, n, sV, V + (int64_t)m * n, n, sV, Pd, ldp, sP, true, atbs, atb_elems, s);
+      launch_dcgs_coef(nact, true, m, m - bq, Pd, ldp, sP, nullptr, ldp, sA0, Hd, ld, sH, Ed, sP, Ld, bad, s);
+      launch_combine(nact, n, m, bq, V, n, sV, Ed, sP, Ld, V + (int64_t)m * n, n, sV, s);
     }
     tm.mark("krylov_expand");
-    EMU_HIP(hipMemcpyAsync(Hh.data(), Hd, sizeof(zd) * Hh.size(), hipMemcpyDeviceToHost, s));
+    EMU_HIP(hipMemcpyAsync(Hh.data(), Hd, sizeof(zd) * (size_t)nact * sH, hipMemcpyDeviceToHost, s));
     EMU_HIP(hipStreamSynchronize(s));
     turn.unlock();
     if (c->prof.on) c->prof.collect();
-    bool last = (it == itermax);
-    all_conv = true;
+    const bool last = (it == itermax);
     std::vector<std::thread> th;
-    std::vector<std::string> errs(nb);
-    std::vector<char> convflag(nb, 0);
-    for (int b = 0; b < nb; ++b) {
+    std::vector<std::string> errs(nact);
+    std::vector<char> convflag(nact, 0);
+    for (int b = 0; b < nact; ++b) {
       th.emplace_back([&, b]() {
         try {
           KSState &S = st[b];
           for (int64_t i = 0; i < sH; ++i) S.H[i] = cplx(Hh[(size_t)b * sH + i].x, Hh[(size_t)b * sH + i].y);
           // Ritz vectors of the basis are formed once the pairs pass the test
-          // (kept for the final combination) or on the last allowed restart
-          convflag[b] = ks_project(S, m, bq, nev, keep, tol, &ritz[b], nullptr, last) ? 1 : 0;
+          convflag[b] = ks_project(S, m, bq, nev, keep, tol, &ritz[b], nullptr, false) ? 1 : 0;
         } catch (std::exception &e) {
           errs[b] = e.what();
         }
       });
     }
     for (auto &t : th) t.join();
-    for (int b = 0; b < nb; ++b) {
+    for (int b = 0; b < nact; ++b)
       if (!errs[b].empty()) throw std::runtime_error(errs[b]);
-      if (!convflag[b]) all_conv = false;
-    }
     res.restarts = it;
     tm.mark("host_schur");
-    if (all_conv || last) break;
+    turn.lock();
+    // Retire the converged instances: their Ritz vectors X = V[:, :m] W are
+    // formed now, and the slot is filled with the last active instance, so
+    // that the next cycles run on the active instances only.
+    for (int b = nact - 1; b >= 0; --b) {
+      if (!convflag[b]) continue;
+      const int o = slot_inst[b];
+      const std::vector<cplx> &W = ritz[b];
+      if (W.size() != (size_t)m * nev) throw std::logic_error("Ritz vectors missing after convergence");
+      std::vector<zd> Wh((size_t)m * nev);
+      for (size_t i = 0; i < W.size(); ++i) Wh[i] = zmk(W[i].real(), W[i].imag());
+      for (int i = 0; i < nev; ++i) res.mu[(size_t)o * nev + i] = st[b].theta[i];
+      EMU_HIP(hipMemcpyAsync(Wd, Wh.data(), sizeof(zd) * Wh.size(), hipMemcpyHostToDevice, s));
+      launch_gemm_nn(1, n, nev, m, zmk(1, 0), V + (int64_t)b * sV, n, sV, Wd, m, (int64_t)m * nev, zmk(0, 0),
+                     Xout + (int64_t)o * n * nev, n, (int64_t)n * nev, s);
+      EMU_HIP(hipStreamSynchronize(s));  // Wh is reused
+      const int e = nact - 1;
+      if (b != e) {
+        swap_slots(b, e);
+        std::swap(st[b], st[e]);
+        std::swap(ritz[b], ritz[e]);
+        std::swap(slot_inst[b], slot_inst[e]);
+        swaps.push_back({b, e});
+      }
+      --nact;
+    }
+    if (nact == 0) break;
+    if (last) break;
     // restart: V2[:, :keep] = V[:, :m] Q[:, :keep]; V2[:, keep:keep+bq] = V[:, m:m+bq]
-    std::vector<zd> Qh((size_t)nb * m * keep);
-    std::vector<zd> Hn((size_t)nb * sH, zmk(0, 0));
-    for (int b = 0; b < nb; ++b) {
+    std::vector<zd> Qh((size_t)nact * m * keep);
+    std::vector<zd> Hn((size_t)nact * sH, zmk(0, 0));
+    for (int b = 0; b < nact; ++b) {
       KSState &S = st[b];
       for (int j = 0; j < keep; ++j)
         for (int i = 0; i < m; ++i) {
@@ -794,33 +834,23 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
       }
       S.nlock = S.nlock_next;
     }
-    turn.lock();
     EMU_HIP(hipMemcpyAsync(Qd, Qh.data(), sizeof(zd) * Qh.size(), hipMemcpyHostToDevice, s));
     EMU_HIP(hipMemcpyAsync(Hd, Hn.data(), sizeof(zd) * Hn.size(), hipMemcpyHostToDevice, s));
     EMU_HIP(hipStreamSynchronize(s));
-    launch_gemm_nn(nb, n, keep, m, zmk(1, 0), V, n, sV, Qd, m, (int64_t)m * keep, zmk(0, 0), V2, n, sV, s);
-    launch_copy(nb, (int64_t)bq * n, V + (int64_t)m * n, sV, V2 + (int64_t)keep * n, sV, s);
+    launch_gemm_nn(nact, n, keep, m, zmk(1, 0), V, n, sV, Qd, m, (int64_t)m * keep, zmk(0, 0), V2, n, sV, s);
+    launch_copy(nact, (int64_t)bq * n, V + (int64_t)m * n, sV, V2 + (int64_t)keep * n, sV, s);
     std::swap(V, V2);
     k = keep;
     tm.mark("restart");
   }
-  if (!all_conv) throw NoConvergence("Krylov-Schur: eigenpairs did not converge within itermax restarts");
+  if (turn.owns_lock()) turn.unlock();
+  // the caller's per-instance arrays back in their original order
+  for (auto it = swaps.rbegin(); it != swaps.rend(); ++it) swap_caller(it->first, it->second);
+  EMU_HIP(hipStreamSynchronize(s));
+  if (nact > 0) throw NoConvergence("Krylov-Schur: eigenpairs did not converge within itermax restarts");
   std::vector<int32_t> badh(nb);
   EMU_HIP(hipMemcpy(badh.data(), bad, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
   for (int b = 0; b < nb; ++b) res.rank_fixes += badh[b];
-  // Ritz vectors X = V[:, :m] W, W = Q[:, :nev] Y
-  std::vector<zd> Wh((size_t)nb * m * nev);
-  for (int b = 0; b < nb; ++b) {
-    const std::vector<cplx> &W = ritz[b];
-    if (W.size() != (size_t)m * nev) throw std::logic_error("Ritz vectors missing after convergence");
-    for (size_t i = 0; i < W.size(); ++i) Wh[(size_t)b * m * nev + i] = zmk(W[i].real(), W[i].imag());
-    for (int i = 0; i < nev; ++i) res.mu[(size_t)b * nev + i] = st[b].theta[i];
-  }
-  zd *Wd = c->buf<zd>("ks_W", (size_t)nb * m * nev);
-  EMU_HIP(hipMemcpyAsync(Wd, Wh.data(), sizeof(zd) * Wh.size(), hipMemcpyHostToDevice, s));
-  EMU_HIP(hipStreamSynchronize(s));
-  launch_gemm_nn(nb, n, nev, m, zmk(1, 0), V, n, sV, Wd, m, (int64_t)m * nev, zmk(0, 0), Xout, n, (int64_t)n * nev, s);
-  EMU_HIP(hipStreamSynchronize(s));
   tm.mark("ritz_vectors");
   return res;
 }

@@ -39,12 +39,16 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, int ncol, const int64_t
   for (int c = 0; c < SM_C; ++c) acc[c] = zmk(0, 0);
   if (r < n) {
     const int64_t k0 = rp[r], k1 = rp[r + 1];
+    // the column loads are unconditional (clamped column index, unused
+    // columns multiply zero) so that all of them are in flight together
     for (int64_t k = k0 + lane8; k < k1; k += 8) {
       const zd a = v[k];
       const int64_t col = ci[k];
+      zd xv[SM_C];
 #pragma unroll
-      for (int c = 0; c < SM_C; ++c)
-        if (c < nc) acc[c] = zfma(a, xb[(int64_t)c * n + col], acc[c]);
+      for (int c = 0; c < SM_C; ++c) xv[c] = xb[(int64_t)min(c, nc - 1) * n + col];
+#pragma unroll
+      for (int c = 0; c < SM_C; ++c) acc[c] = zfma(c < nc ? a : zmk(0, 0), xv[c], acc[c]);
     }
   }
 #pragma unroll
@@ -67,6 +71,7 @@ void launch_spmm(int nb, int64_t n, int ncol, const int64_t *rp, const int32_t *
                         ystride);
   EMU_LAUNCH_CHECK();
 }
+
 
 // ----------------------------------------------- C = op(A)^T B, split over K
 constexpr int AT = 32;     // output tile
@@ -128,74 +133,126 @@ __global__ __launch_bounds__(256) void k_atb_partial(int64_t K, int M, int N, in
 // The next tile is fetched into registers while the current one is used.  At
 // the end the four waves' sums are combined in LDS and written as the chunk's
 // partial product; k_atb_reduce adds the chunks.
-constexpr int TT_M = 64, TT_K = 32;
-
-template <int NQ>
+// CPT columns of A per thread: a tile is 64 CPT columns x 32 / CPT rows (the
+// same 2048 elements); with CPT = 2 every B value read from LDS feeds two
+// columns, halving the LDS traffic per FMA for N = 16.
+template <int NQ, int CPT>
 __global__ __launch_bounds__(256) void k_atb_tall(int64_t K, int M, int64_t kchunk, const zd *__restrict__ A,
                                                   int64_t lda, int64_t sA, const zd *__restrict__ B, int64_t ldb,
                                                   int64_t sB, int conjA, zd *__restrict__ part) {
   static_assert(NQ >= 1 && NQ <= 16, "tall kernel covers N <= 16");
-  const int m0 = blockIdx.x * TT_M;
+  constexpr int TM = 64 * CPT, TK = 32 / CPT, CG = 256 / TK;  // CG column groups in the staging map
+  const int m0 = blockIdx.x * TM;
   const int chunk = blockIdx.y, nchunk = gridDim.y;
   const int b = blockIdx.z;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const zd *Ab = A + b * sA;
   const zd *Bb = B + b * sB;
-  __shared__ zd As[TT_K][TT_M + 1];
-  __shared__ zd Bs[TT_K][NQ];
+  __shared__ zd As[TK][TM + 1];
+  __shared__ zd Bs[TK][NQ];
   const int64_t kb = chunk * kchunk, ke = min(K, kb + kchunk);
-  const int lr = tid & 31, lc = tid >> 5;  // staging: row lr, columns lc + 8 i
-  zd ra[8], rb = zmk(0, 0), rb2 = zmk(0, 0);
+  const int lr = tid % TK, lc = tid / TK;  // staging: row lr, columns lc + CG i
+  zd ra[8], rb[(TK * NQ + 255) / 256];
+  // Loads are unconditional from clamped (in-range) addresses, and the
+  // out-of-range values are zeroed afterwards: a load inside a branch makes
+  // the compiler wait for it before the branch merges, which serialises the
+  // tile's loads instead of keeping them all in flight.
   auto fetch = [&](int64_t k0) {
     const int64_t k = k0 + lr;
     const bool okk = k < ke;
+    const int64_t kc = okk ? k : ke - 1;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const int m = m0 + lc + 8 * i;
-      ra[i] = (okk && m < M) ? Ab[k + (int64_t)m * lda] : zmk(0, 0);
+      const int m = m0 + lc + CG * i;
+      const zd v = Ab[kc + (int64_t)min(m, M - 1) * lda];
+      ra[i] = (okk && m < M) ? v : zmk(0, 0);
     }
-    if (lc < NQ) rb = okk ? Bb[k + (int64_t)lc * ldb] : zmk(0, 0);
-    if (NQ > 8 && lc + 8 < NQ) rb2 = okk ? Bb[k + (int64_t)(lc + 8) * ldb] : zmk(0, 0);
+#pragma unroll
+    for (int i = 0; i < (TK * NQ + 255) / 256; ++i) {
+      const int idx = tid + 256 * i, r = idx % TK, q = idx / TK;
+      const bool ok = q < NQ && k0 + r < ke;
+      const zd v = Bb[(ok ? k0 + r : ke - 1) + (int64_t)min(q, NQ - 1) * ldb];
+      rb[i] = ok ? v : zmk(0, 0);
+    }
   };
-  zd acc[NQ];
+  zd acc[CPT][NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) acc[q] = zmk(0, 0);
+  for (int c = 0; c < CPT; ++c)
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) acc[c][q] = zmk(0, 0);
   if (kb < ke) fetch(kb);
-  for (int64_t k0 = kb; k0 < ke; k0 += TT_K) {
+  for (int64_t k0 = kb; k0 < ke; k0 += TK) {
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 8; ++i) As[lr][lc + 8 * i] = conjA ? zconj(ra[i]) : ra[i];
-    if (lc < NQ) Bs[lr][lc] = rb;
-    if (NQ > 8 && lc + 8 < NQ) Bs[lr][lc + 8] = rb2;
+    for (int i = 0; i < 8; ++i) As[lr][lc + CG * i] = conjA ? zconj(ra[i]) : ra[i];
+#pragma unroll
+    for (int i = 0; i < (TK * NQ + 255) / 256; ++i) {
+      const int idx = tid + 256 * i;
+      if (idx < TK * NQ) Bs[idx % TK][idx / TK] = rb[i];
+    }
     __syncthreads();
-    if (k0 + TT_K < ke) fetch(k0 + TT_K);
+    if (k0 + TK < ke) fetch(k0 + TK);
 #pragma unroll
-    for (int kk = wv; kk < TT_K; kk += 4) {
-      const zd a = As[kk][lane];
+    for (int kk = wv; kk < TK; kk += 4) {
+      zd a[CPT];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) acc[q] = zfma(a, Bs[kk][q], acc[q]);
+      for (int c = 0; c < CPT; ++c) a[c] = As[kk][lane + 64 * c];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const zd bq = Bs[kk][q];
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) acc[c][q] = zfma(a[c], bq, acc[c][q]);
+      }
     }
   }
   __syncthreads();
-  // combine the four waves, eight columns of B at a time (LDS of the A tile
-  // reused: 4 x 64 x 8 <= 32 x 65)
+  // combine the four waves, 64 columns and eight columns of B at a time (LDS
+  // of the A tile reused: 4 x 64 x 8 <= TK x (TM + 1))
   zd *red = &As[0][0];
   constexpr int QH = NQ < 8 ? NQ : 8;
 #pragma unroll
-  for (int h = 0; h < NQ; h += QH) {
+  for (int c = 0; c < CPT; ++c)
 #pragma unroll
-    for (int q = 0; q < QH; ++q) red[(wv * TT_M + lane) * QH + q] = acc[h + q];
-    __syncthreads();
-    for (int o = tid; o < TT_M * QH; o += 256) {
-      const int c = o % TT_M, q = o / TT_M;
-      zd sum = red[c * QH + q];
+    for (int h = 0; h < NQ; h += QH) {
 #pragma unroll
-      for (int w = 1; w < 4; ++w) sum = zadd(sum, red[(w * TT_M + c) * QH + q]);
-      if (m0 + c < M) part[(((int64_t)b * nchunk + chunk) * NQ + h + q) * M + m0 + c] = sum;
+      for (int q = 0; q < QH; ++q) red[(wv * 64 + lane) * QH + q] = acc[c][h + q];
+      __syncthreads();
+      for (int o = tid; o < 64 * QH; o += 256) {
+        const int cc = o % 64, q = o / 64;
+        zd sum = red[cc * QH + q];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) sum = zadd(sum, red[(w * 64 + cc) * QH + q]);
+        const int m = m0 + 64 * c + cc;
+        if (m < M) part[(((int64_t)b * nchunk + chunk) * NQ + h + q) * M + m] = sum;
+      }
+      __syncthreads();
     }
-    __syncthreads();
+}
+
+// Exchange two equal ranges (the data of two batch slots).
+template <typename T>
+__global__ __launch_bounds__(256) void k_swap(int64_t n, T *__restrict__ a, T *__restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const T t = a[i];
+    a[i] = b[i];
+    b[i] = t;
   }
 }
+
+void launch_swap(int64_t n, zd *a, zd *b, hipStream_t s) {
+  if (n <= 0 || a == b) return;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_swap<zd>, dim3(g), dim3(256), 0, s, n, a, b);
+  EMU_LAUNCH_CHECK();
+}
+
+void launch_swap_i32(int64_t n, int32_t *a, int32_t *b, hipStream_t s) {
+  if (n <= 0 || a == b) return;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_swap<int32_t>, dim3(g), dim3(256), 0, s, n, a, b);
+  EMU_LAUNCH_CHECK();
+}
+
 
 // C[b][m + nn ldc] = sum over chunks of part[b][chunk][nn][m]: 64 outputs per
 // workgroup, its four waves summing every fourth chunk, combined in LDS.
@@ -220,8 +277,9 @@ void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64
                 int64_t ldb, int64_t sB, zd *C, int64_t ldc, int64_t sC, bool conjA, zd *scratch,
                 size_t scratch_elems, hipStream_t s, KTime kt) {
   const bool tall = N == 1 || N == 2 || N == 4 || N == 8 || N == 16;
-  const int kstep = tall ? TT_K : AKC;
-  const int tiles = tall ? (M + TT_M - 1) / TT_M : ((M + AT - 1) / AT) * ((N + AT - 1) / AT);
+  const int cpt = N == 16 ? 2 : 1;
+  const int kstep = tall ? 32 / cpt : AKC;
+  const int tiles = tall ? (M + 64 * cpt - 1) / (64 * cpt) : ((M + AT - 1) / AT) * ((N + AT - 1) / AT);
   int64_t per = (int64_t)nb * M * N;
   int64_t maxchunk = std::max<int64_t>(1, (int64_t)(scratch_elems / std::max<int64_t>(per, 1)));
   int64_t want = std::max<int64_t>(1, 2048 / std::max(1, tiles * nb));
@@ -235,11 +293,11 @@ void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64
   if (tall) {
     dim3 g((unsigned)tiles, (unsigned)nchunk, nb), t(256);
     switch (N) {
-      case 1: hipExtLaunchKernelGGL(k_atb_tall<1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
-      case 2: hipExtLaunchKernelGGL(k_atb_tall<2>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
-      case 4: hipExtLaunchKernelGGL(k_atb_tall<4>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
-      case 8: hipExtLaunchKernelGGL(k_atb_tall<8>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
-      default: hipExtLaunchKernelGGL(k_atb_tall<16>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+      case 1: hipExtLaunchKernelGGL(k_atb_tall<1, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+      case 2: hipExtLaunchKernelGGL(k_atb_tall<2, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+      case 4: hipExtLaunchKernelGGL(k_atb_tall<4, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+      case 8: hipExtLaunchKernelGGL(k_atb_tall<8, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+      default: hipExtLaunchKernelGGL(k_atb_tall<16, 2>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
     }
   } else {
     hipLaunchKernelGGL(k_atb_partial, dim3(tiles, (unsigned)nchunk, nb), dim3(256), 0, s, K, M, N, kchunk, A, lda,
@@ -251,6 +309,7 @@ void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64
                      scratch, C, ldc, sC);
   EMU_LAUNCH_CHECK();
 }
+
 
 // ------------------------------------------------ C = alpha A B + beta C (NN)
 // Plain batched GEMMs (restart combination V Q, Ritz vectors, star products)
@@ -444,17 +503,20 @@ void launch_sub_vh_blk(int nb, int64_t n, int k, int bq, const zd *V, int64_t ld
 // from H) from W.
 
 // W[b] (n x NQ, in place) = W[b] L[b] - V[b][:, :k] E[b] (L: NQ x NQ, column-
-// major; E: k x NQ, ld lde).  Thread per row; E staged through LDS.
+// major; E: k x NQ, row-major so that the NQ coefficients of basis column c
+// are contiguous).  Thread per row.  The coefficients of a column are the
+// same for every lane, so they are read with scalar loads straight into
+// SGPRs and used as FMA operands: no LDS traffic in the main loop.
 template <int NQ>
 __global__ __launch_bounds__(256) void k_combine(int64_t n, int k, const zd *__restrict__ V, int64_t ldv,
-                                                 int64_t sV, const zd *__restrict__ E, int64_t lde, int64_t sE,
+                                                 int64_t sV, const zd *__restrict__ E, int64_t sE,
                                                  const zd *__restrict__ L, zd *__restrict__ W, int64_t ldw,
                                                  int64_t sW) {
-  __shared__ zd es[32][NQ];
   __shared__ zd ls[NQ][NQ];
   const int b = blockIdx.y;
   const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const zd *Vb = V + b * sV;
+  const int64_t rr = r < n ? r : n - 1;  // clamped row: loads stay in bounds, the store is skipped
+  const zd *Vb = V + b * sV + rr;
   const zd *Eb = E + b * sE;
   for (int idx = threadIdx.x; idx < NQ * NQ; idx += blockDim.x) ls[idx % NQ][idx / NQ] = L[(int64_t)b * NQ * NQ + idx];
   __syncthreads();
@@ -462,7 +524,7 @@ __global__ __launch_bounds__(256) void k_combine(int64_t n, int k, const zd *__r
   {
     zd x[NQ];
 #pragma unroll
-    for (int p = 0; p < NQ; ++p) x[p] = (r < n) ? W[b * sW + p * ldw + r] : zmk(0, 0);
+    for (int p = 0; p < NQ; ++p) x[p] = W[b * sW + p * ldw + rr];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       zd a = zmk(0, 0);
@@ -471,20 +533,12 @@ __global__ __launch_bounds__(256) void k_combine(int64_t n, int k, const zd *__r
       acc[q] = a;
     }
   }
-  for (int c0 = 0; c0 < k; c0 += 32) {
-    const int cn = min(32, k - c0);
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < 32 * NQ; idx += blockDim.x) {
-      const int c = idx / NQ, q = idx - c * NQ;
-      es[c][q] = c < cn ? Eb[(c0 + c) + (int64_t)q * lde] : zmk(0, 0);
-    }
-    __syncthreads();
-    if (r < n)
-      for (int c = 0; c < cn; ++c) {
-        const zd v = Vb[r + (int64_t)(c0 + c) * ldv];
+#pragma unroll 2
+  for (int c = 0; c < k; ++c) {
+    const zd v = Vb[(int64_t)c * ldv];
+    const zd *ec = Eb + (int64_t)c * NQ;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) acc[q] = zfms(v, es[c][q], acc[q]);
-      }
+    for (int q = 0; q < NQ; ++q) acc[q] = zfms(v, ec[q], acc[q]);
   }
   if (r < n)
 #pragma unroll
@@ -492,19 +546,19 @@ __global__ __launch_bounds__(256) void k_combine(int64_t n, int k, const zd *__r
 }
 
 void launch_combine(int nb, int64_t n, int k, int nq, const zd *V, int64_t ldv, int64_t sV, const zd *E,
-                    int64_t lde, int64_t sE, const zd *L, zd *W, int64_t ldw, int64_t sW, hipStream_t s, KTime kt) {
+                    int64_t sE, const zd *L, zd *W, int64_t ldw, int64_t sW, hipStream_t s, KTime kt) {
   dim3 g((unsigned)((n + 255) / 256), nb), t(256);
   switch (nq) {
-    case 8: hipExtLaunchKernelGGL(k_combine<8>, g, t, 0, s, kt.a, kt.b, 0, n, k, V, ldv, sV, E, lde, sE, L, W, ldw, sW); break;
-    case 16: hipExtLaunchKernelGGL(k_combine<16>, g, t, 0, s, kt.a, kt.b, 0, n, k, V, ldv, sV, E, lde, sE, L, W, ldw, sW); break;
+    case 8: hipExtLaunchKernelGGL(k_combine<8>, g, t, 0, s, kt.a, kt.b, 0, n, k, V, ldv, sV, E, sE, L, W, ldw, sW); break;
+    case 16: hipExtLaunchKernelGGL(k_combine<16>, g, t, 0, s, kt.a, kt.b, 0, n, k, V, ldv, sV, E, sE, L, W, ldw, sW); break;
     default: throw HipError("combine: 8 or 16 columns");
   }
   EMU_LAUNCH_CHECK();
 }
 
 // Coefficients of one delayed step, one workgroup per instance (block size 8).
-//   P = Vt^H [Z W] (j x 16, ld ldp): S = P[:, :8], T = P[:, 8:]
-//   G = [Z W]^H [Z W] (16 x 16);  A0 = H[:j, :j] S (j x 8, ld lda), old H
+//   P = [Vt Z W]^H [Z W] ((j + 16) x 16, ld ldp): S = P[:j, :8], T = P[:j, 8:],
+//   G = P[j:j+16, :] = [Z W]^H [Z W];  A0 = H[:j, :j] S (j x 8, ld lda), old H
 //   row block j of H (rows j..j+7) is nonzero in columns [c0, j)
 // Z's reorthogonalisation: C = Z^H Z - S^H S = R^H R, Q = (Z - Vt S) R^-1,
 // H[:j, c0:j] += S H[j:j+8, c0:j], H[j:j+8, c0:j] = R H[j:j+8, c0:j].
@@ -513,24 +567,23 @@ void launch_combine(int nb, int64_t n, int k, int nq, const zd *V, int64_t ldv, 
 //   cV = (T - a) R^-1, cQ = R^-H (Z^H W - S^H T) R^-1 - b R^-1  (column j of H),
 //   D = b R^-1 + cQ, E = [S R^-1, a R^-1 + cV - S R^-1 D],
 //   L = [[R^-1, -R^-1 D], [0, R^-1]], so that [Q, Zraw] = [Z W] L - Vt E.
-// FIN (no W; the last provisional block of an expansion): only Z's
-// reorthogonalisation, E = S R^-1 (j x 8), L = R^-1.
+// FIN (no W; the last provisional block of an expansion; P = [Vt Z]^H Z):
+// only Z's reorthogonalisation, E = S R^-1 (j x 8), L = R^-1.
 template <bool FIN>
 __global__ __launch_bounds__(256) void k_dcgs_coef(int j, int c0, const zd *__restrict__ P, int64_t ldp, int64_t sP,
-                                                   const zd *__restrict__ G, const zd *__restrict__ A0, int64_t lda,
+                                                   const zd *__restrict__ A0, int64_t lda,
                                                    int64_t sA, zd *__restrict__ H, int64_t ldh, int64_t sH,
-                                                   zd *__restrict__ E, int64_t lde, int64_t sE, zd *__restrict__ L,
+                                                   zd *__restrict__ E, int64_t sE, zd *__restrict__ L,
                                                    int32_t *__restrict__ bad) {
   constexpr int B = 8;
   constexpr int NG = FIN ? B : 2 * B;
   const int b = blockIdx.x, tid = threadIdx.x;
   const zd *Pb = P + b * sP;
-  const zd *Gb = G + (int64_t)b * NG * NG;
   zd *Hb = H + b * sH;
   zd *Eb = E + b * sE;
   __shared__ zd SS[B][B], ST[B][B], U[B][B], R[B][B], Ri[B][B], Bm[B][B], CQ[B][B], Dm[B][B], T1[B][B];
   __shared__ zd Gs[NG][NG];
-  for (int idx = tid; idx < NG * NG; idx += blockDim.x) Gs[idx % NG][idx / NG] = Gb[idx];
+  for (int idx = tid; idx < NG * NG; idx += blockDim.x) Gs[idx % NG][idx / NG] = Pb[(j + idx % NG) + (idx / NG) * ldp];
   // 1. S^H S, S^H T, u = Hrow S (one output per thread)
   if (tid < 3 * B * B) {
     const int o = tid % (B * B), kind = tid / (B * B), p = o % B, q = o / B;
@@ -605,40 +658,37 @@ __global__ __launch_bounds__(256) void k_dcgs_coef(int j, int c0, const zd *__re
     }
     __syncthreads();
   }
-  // 4. per basis row i < j: a, cV (column j of H), E; and H[:j, c0:j] += S Hrow
-  for (int i = tid; i < j; i += blockDim.x) {
-    zd Si[B], e1[B];
-#pragma unroll
-    for (int q = 0; q < B; ++q) Si[q] = Pb[i + q * ldp];
-#pragma unroll
-    for (int q = 0; q < B; ++q) {
-      zd acc = zmk(0, 0);
-      for (int k = 0; k <= q; ++k) acc = zfma(Si[k], Ri[k][q], acc);
-      e1[q] = acc;
-      Eb[i + (int64_t)q * lde] = acc;
-    }
-    if (!FIN) {
-      zd a[B], cv[B];
-#pragma unroll
-      for (int q = 0; q < B; ++q) {
-        zd acc = A0[b * sA + i + (int64_t)q * lda];
-        for (int k = 0; k < B; ++k) acc = zfma(Si[k], U[k][q], acc);
-        a[q] = acc;
+  // 4. per basis row i < j: a, cV (column j of H), E.  32 rows at a time, one
+  // thread per (row, column); the row vectors a and S R^-1 go through LDS.
+  __shared__ zd sa[32][B], se[32][B];
+  {
+    const int rr = tid / B, q = tid % B;
+    for (int i0 = 0; i0 < j; i0 += 32) {
+      const int i = i0 + rr;
+      const bool ok = i < j;
+      zd e1 = zmk(0, 0), av = zmk(0, 0);
+      if (ok) {
+        for (int k = 0; k <= q; ++k) e1 = zfma(Pb[i + k * ldp], Ri[k][q], e1);
+        Eb[(int64_t)i * NG + q] = e1;
+        if (!FIN) {
+          av = A0[b * sA + i + (int64_t)q * lda];
+          for (int k = 0; k < B; ++k) av = zfma(Pb[i + k * ldp], U[k][q], av);
+        }
       }
-#pragma unroll
-      for (int q = 0; q < B; ++q) {
-        zd acc = zmk(0, 0);
-        for (int k = 0; k <= q; ++k) acc = zfma(zsub(Pb[i + (B + k) * ldp], a[k]), Ri[k][q], acc);
-        cv[q] = acc;
-        Hb[i + (int64_t)(j + q) * ldh] = acc;
+      se[rr][q] = e1;
+      sa[rr][q] = av;
+      __syncthreads();
+      if (!FIN && ok) {
+        zd cv = zmk(0, 0), e2 = zmk(0, 0);
+        for (int k = 0; k <= q; ++k) {
+          cv = zfma(zsub(Pb[i + (B + k) * ldp], sa[rr][k]), Ri[k][q], cv);
+          e2 = zfma(sa[rr][k], Ri[k][q], e2);
+        }
+        for (int k = 0; k < B; ++k) e2 = zfms(se[rr][k], Dm[k][q], e2);
+        Hb[i + (int64_t)(j + q) * ldh] = cv;
+        Eb[(int64_t)i * NG + B + q] = zadd(e2, cv);
       }
-#pragma unroll
-      for (int q = 0; q < B; ++q) {
-        zd acc = cv[q];
-        for (int k = 0; k <= q; ++k) acc = zfma(a[k], Ri[k][q], acc);
-        for (int k = 0; k < B; ++k) acc = zfms(e1[k], Dm[k][q], acc);
-        Eb[i + (int64_t)(B + q) * lde] = acc;
-      }
+      __syncthreads();
     }
   }
   const int w = j - c0;
@@ -657,6 +707,7 @@ __global__ __launch_bounds__(256) void k_dcgs_coef(int j, int c0, const zd *__re
 #pragma unroll
     for (int p = 0; p < B; ++p) {
       zd acc = zmk(0, 0);
+#pragma unroll
       for (int k = p; k < B; ++k) acc = zfma(R[p][k], h[k], acc);
       Hb[(j + p) + (int64_t)c * ldh] = acc;
     }
@@ -677,15 +728,15 @@ __global__ __launch_bounds__(256) void k_dcgs_coef(int j, int c0, const zd *__re
   }
 }
 
-void launch_dcgs_coef(int nb, bool fin, int j, int c0, const zd *P, int64_t ldp, int64_t sP, const zd *G,
-                      const zd *A0, int64_t lda, int64_t sA, zd *H, int64_t ldh, int64_t sH, zd *E, int64_t lde,
-                      int64_t sE, zd *L, int32_t *bad, hipStream_t s) {
+void launch_dcgs_coef(int nb, bool fin, int j, int c0, const zd *P, int64_t ldp, int64_t sP, const zd *A0,
+                      int64_t lda, int64_t sA, zd *H, int64_t ldh, int64_t sH, zd *E, int64_t sE, zd *L, int32_t *bad,
+                      hipStream_t s) {
   if (fin)
-    hipLaunchKernelGGL(k_dcgs_coef<true>, dim3(nb), dim3(256), 0, s, j, c0, P, ldp, sP, G, A0, lda, sA, H, ldh, sH, E,
-                       lde, sE, L, bad);
+    hipLaunchKernelGGL(k_dcgs_coef<true>, dim3(nb), dim3(256), 0, s, j, c0, P, ldp, sP, A0, lda, sA, H, ldh, sH, E, sE,
+                       L, bad);
   else
-    hipLaunchKernelGGL(k_dcgs_coef<false>, dim3(nb), dim3(256), 0, s, j, c0, P, ldp, sP, G, A0, lda, sA, H, ldh, sH,
-                       E, lde, sE, L, bad);
+    hipLaunchKernelGGL(k_dcgs_coef<false>, dim3(nb), dim3(256), 0, s, j, c0, P, ldp, sP, A0, lda, sA, H, ldh, sH, E,
+                       sE, L, bad);
   EMU_LAUNCH_CHECK();
 }
 

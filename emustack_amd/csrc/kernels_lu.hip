@@ -516,29 +516,47 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
   zd acc[RC];
 #pragma unroll
   for (int q = 0; q < RC; ++q) acc[q] = zmk(0, 0);
+  // All loads below are unconditional from clamped (in-range) addresses with
+  // the unused values zeroed afterwards, so that each thread keeps its loads
+  // in flight together instead of waiting on each one inside a branch.
   for (int kc = kbeg; kc < kend; kc += KC) {
-    for (int idx = tid; idx < KC * RC; idx += 256) {
-      const int k = idx % KC, q = idx / KC;
+    {
+      // input row k of the chunk (pivot gather + child update entries), four
+      // threads per row sharing its RC columns
+      const int k = tid & (KC - 1), qg = tid / KC;
       const int kk = kc + k;
-      zd v = zmk(0, 0);
-      if (kk < kend && q < nq) {
-        const int j = sg[kk];
-        v = bv[(col0 + q) * n + fb + j];
-        const int c0i = cm[2 * j], c1i = cm[2 * j + 1];
-        if (c0i >= 0) v = zadd(v, wbuf[(col0 + q) * nw + c0i]);
-        if (c1i >= 0) v = zadd(v, wbuf[(col0 + q) * nw + c1i]);
+      const bool okk = kk < kend;
+      const int j = sg[okk ? kk : kbeg];
+      const int c0i = cm[2 * j], c1i = cm[2 * j + 1];
+#pragma unroll
+      for (int u = 0; u < (RC + 3) / 4; ++u) {
+        const int qq = qg + 4 * u;
+        if (qq < RC) {
+          const int64_t cq = col0 + min(qq, nq - 1);
+          zd v = bv[cq * n + fb + j];
+          const zd w0 = wbuf[cq * nw + max(c0i, 0)], w1 = wbuf[cq * nw + max(c1i, 0)];
+          if (c0i >= 0) v = zadd(v, w0);
+          if (c1i >= 0) v = zadd(v, w1);
+          xin[k][qq] = (okk && qq < nq) ? v : zmk(0, 0);
+        }
       }
-      xin[k][q] = v;
     }
     __syncthreads();
     const int kl = min(KC, kend_w - kc);
     if (row < f && kl > 0) {
       const zd *col = Fw + row + (int64_t)kc * f;
-#pragma unroll 4
-      for (int k = kp; k < kl; k += ks) {
-        const zd a = col[(int64_t)k * f];
+      const int nk = (kl - kp + ks - 1) / ks;  // this wave's columns: kp, kp + ks, ...
+      for (int t0 = 0; t0 < nk; t0 += 8) {
+        zd a[8];
 #pragma unroll
-        for (int q = 0; q < RC; ++q) acc[q] = zfma(a, xin[k][q], acc[q]);
+        for (int u = 0; u < 8; ++u) a[u] = col[(int64_t)(kp + min(t0 + u, nk - 1) * ks) * f];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (t0 + u < nk) {
+            const int k = kp + (t0 + u) * ks;
+#pragma unroll
+            for (int q = 0; q < RC; ++q) acc[q] = zfma(a[u], xin[k][q], acc[q]);
+          }
       }
     }
     __syncthreads();
@@ -555,11 +573,15 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
   }
   if (kp != 0) return;
   if (slot >= 0) {  // split task: partial sums of the tile, added by k_fwd_red
-    for (int q = 0; q < nq; ++q) part[((col0 + q) * nslot + slot) * 64 + lane] = acc[q];
+#pragma unroll
+    for (int q = 0; q < RC; ++q)
+      if (q < nq) part[((col0 + q) * nslot + slot) * 64 + lane] = acc[q];
     return;
   }
   if (row < f) {
-    for (int q = 0; q < nq; ++q) {
+#pragma unroll
+    for (int q = 0; q < RC; ++q) {
+      if (q >= nq) break;
       zd v = acc[q];
       if (row < s) {
         y[(col0 + q) * n + fb + row] = v;
@@ -634,22 +656,40 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, 
 #pragma unroll
   for (int q = 0; q < RC; ++q) acc[q] = zmk(0, 0);
   for (int kc = kbeg; kc < kend; kc += KC) {
-    for (int idx = tid; idx < KC * RC; idx += 256) {
-      const int k = idx % KC, q = idx / KC;
+    {
+      // input row k of the chunk: y of this front or x of an ancestor
+      const int k = tid & (KC - 1), qg = tid / KC;
       const int kk = kc + k;
-      zd v = zmk(0, 0);
-      if (kk < kend && q < nq) v = kk < s ? y[(col0 + q) * n + fb + kk] : x[(col0 + q) * n + ui[kk - s]];
-      xin[k][q] = v;
+      const bool okk = kk < kend;
+      const int kkc = okk ? kk : kbeg;
+      const zd *src = kkc < s ? y : x;
+      const int64_t off = kkc < s ? (int64_t)fb + kkc : (int64_t)ui[kkc - s];
+#pragma unroll
+      for (int u = 0; u < (RC + 3) / 4; ++u) {
+        const int qq = qg + 4 * u;
+        if (qq < RC) {
+          const zd v = src[(col0 + min(qq, nq - 1)) * n + off];
+          xin[k][qq] = (okk && qq < nq) ? v : zmk(0, 0);
+        }
+      }
     }
     __syncthreads();
     const int klo = max(0, row - kc), khi = min(KC, kend - kc);
     if (rowl < s && klo < khi) {
       const zd *col = Bw + rowl + (int64_t)kc * s;
-#pragma unroll 4
-      for (int k = klo + kp; k < khi; k += ks) {
-        const zd a = col[(int64_t)k * s];
+      const int kst = klo + kp;
+      const int nk = (khi - kst + ks - 1) / ks;
+      for (int t0 = 0; t0 < nk; t0 += 8) {
+        zd a[8];
 #pragma unroll
-        for (int q = 0; q < RC; ++q) acc[q] = zfma(a, xin[k][q], acc[q]);
+        for (int u = 0; u < 8; ++u) a[u] = col[(int64_t)(kst + min(t0 + u, nk - 1) * ks) * s];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (t0 + u < nk) {
+            const int k = kst + (t0 + u) * ks;
+#pragma unroll
+            for (int q = 0; q < RC; ++q) acc[q] = zfma(a[u], xin[k][q], acc[q]);
+          }
       }
     }
     __syncthreads();
@@ -666,11 +706,15 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, 
   }
   if (kp != 0) return;
   if (slot >= 0) {  // split task: partial sums, added by k_bwd_red
-    for (int q = 0; q < nq; ++q) part[((col0 + q) * nslot + slot) * 64 + lane] = acc[q];
+#pragma unroll
+    for (int q = 0; q < RC; ++q)
+      if (q < nq) part[((col0 + q) * nslot + slot) * 64 + lane] = acc[q];
     return;
   }
   if (rowl < s)
-    for (int q = 0; q < nq; ++q) x[(col0 + q) * n + fb + rowl] = acc[q];
+#pragma unroll
+    for (int q = 0; q < RC; ++q)
+      if (q < nq) x[(col0 + q) * n + fb + rowl] = acc[q];
 }
 
 template <int RC>
