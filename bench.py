@@ -38,6 +38,16 @@ NUM_BMS = 200
 ORDER = 4
 NWL = 512
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per dispatch of the kernels that can be the roofline kernel, from
+# two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this workload at these
+# defaults, calibrated against a plain 2 GiB copy; the runs and their
+# commands are summarised in profiles/kernels_stacked_gratings_8k.md.
+TRAFFIC_PER_DISPATCH = {
+    'k_atb_tall<16, 2>': 3802547283,
+    'k_combine<16>': 3656499385,
+    'k_fwd': 367811805,
+    'k_bwd': 368452440,
+}
 
 
 def inclusion_indices():
@@ -204,7 +214,7 @@ def main():
     dom = max(prof, key=lambda k: prof[k][1])
     cnt, ms, byt = prof[dom]
     achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    kernel_name = {'solve_fwd': 'k_fwd', 'solve_bwd': 'k_bwd', 'gs_project': 'k_atb_tall<16>',
+    kernel_name = {'solve_fwd': 'k_fwd', 'solve_bwd': 'k_bwd', 'gs_project': 'k_atb_tall<16, 2>',
                    'gs_update': 'k_combine<16>', 'spmv_M': 'k_spmm'}[dom]
     line = {
         'metric': 'wavelength-points/s at 8k-tri mesh; mixed vs FP64 GPU at <=1e-6 R/T/A error',
@@ -217,7 +227,7 @@ def main():
                    'batch_wavelengths': args.batch, 'groups': args.groups, 'krylov_dim': args.ncv or None, 'precision': 'fp64 (complex128) everywhere',
                    'op_applications_per_instance': ops / (npts * 6.0)},
         'roofline': {'bound': 'hbm', 'kernel': kernel_name, 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': TRAFFIC_PER_DISPATCH.get(kernel_name),
                      'launches_timed': int(cnt), 'avg_launch_us': 1e3 * ms / max(cnt, 1)},
         'kernel_profile': {k: {'launches': int(v[0]), 'ms': v[1], 'GB': v[2] / 1e9} for k, v in prof.items()},
         'rta_sample': results[:2],
