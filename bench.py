@@ -82,12 +82,11 @@ def run_step(ns, film, sub, sup, wls, groups=1, ncv=0):
     """All three layers for the wavelength batch `wls` (`groups` concurrent
     sub-batches per layer, Krylov dimension ncv), then the cascade."""
     from emustack_amd import objects, Stack
-    from emustack_amd.batch import calc_modes_many, calc_scat_many
+    from emustack_amd.batch import calc_modes_stack, calc_scat_many
     lights = [objects.Light(wl, max_order_PWs=ORDER, theta=0.0, phi=0.0) for wl in wls]
-    sims, ops = [], 0
-    for s in ns:
-        sims.append(calc_modes_many(s, lights, num_BMs=NUM_BMS, groups=groups, ncv=ncv))
-        ops += int(calc_modes_many.last_stats[0])
+    # the three layers' sub-batches share one pool of host threads
+    sims = calc_modes_stack(ns, lights, num_BMs=NUM_BMS, groups=groups, ncv=ncv)
+    ops = int(calc_modes_stack.last_stats[0])
     stacks = [Stack((sub.calc_modes(light), sims[0][i], film.calc_modes(light), sims[1][i],
                      film.calc_modes(light), sims[2][i], sup.calc_modes(light))) for i, light in enumerate(lights)]
     calc_scat_many(stacks, pol='TE')  # batched star-product cascade on the GPU

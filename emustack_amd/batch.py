@@ -15,7 +15,7 @@ import numpy as np
 from . import EMUstack, paths
 from .mode_calcs import Simmo
 
-__all__ = ['calc_modes_many', 'calc_scat_many']
+__all__ = ['calc_modes_many', 'calc_modes_stack', 'calc_scat_many']
 
 _pools = {}
 _pool_lock = threading.Lock()
@@ -51,32 +51,27 @@ def _solve(structure, sims, inputs, nbm, order, P, tol, itermax, ncv):
     return beta, [np.asmatrix(j) for j in J], [np.asmatrix(j) for j in Jd], stats
 
 
-def calc_modes_many(structure, lights, num_BMs=None, tol=0.0, itermax=30, groups=1, ncv=0):
-    """Simmo objects for every light in `lights` (same max_order_PWs),
-    computed together on the GPU (`groups` concurrent sub-batches; ncv the
-    Krylov dimension, 0 for the reference's rule)."""
+def _prepare(structure, lights, num_BMs):
     if structure.periodicity not in ('1D_array', '2D_array'):
         raise ValueError("NanoStruct layer must have periodicity of either '1D_array' or '2D_array'.")
     sims = [Simmo(structure, l) for l in lights]
     orders = {l.max_order_PWs for l in lights}
     if len(orders) != 1:
         raise ValueError("all lights of a batch need the same max_order_PWs")
-    order = orders.pop()
     inputs = [s.fem_inputs(num_BMs) for s in sims]
     nbms = {x[2] for x in inputs}
     if len(nbms) != 1:
         raise ValueError("all lights of a batch need the same num_BMs")
-    nbm = nbms.pop()
-    P = inputs[0][1]
-    groups = max(1, min(int(groups), len(sims)))
-    cuts = np.linspace(0, len(sims), groups + 1).round().astype(int)
-    parts = [(cuts[g], cuts[g + 1]) for g in range(groups)]
-    if groups == 1:
-        results = [_solve(structure, sims, inputs, nbm, order, P, tol, itermax, ncv)]
-    else:
-        futs = [_pool(groups).submit(_solve, structure, sims[a:b], inputs[a:b], nbm, order, P, tol, itermax, ncv)
-                for a, b in parts]
-        results = [f.result() for f in futs]
+    return sims, inputs, nbms.pop(), orders.pop(), inputs[0][1]
+
+
+def _parts(n, groups):
+    groups = max(1, min(int(groups), n))
+    cuts = np.linspace(0, n, groups + 1).round().astype(int)
+    return [(cuts[g], cuts[g + 1]) for g in range(groups)]
+
+
+def _fill(sims, nbm, parts, results):
     stats = np.zeros(8, np.int64)
     for (a, b), (beta, J, Jd, st) in zip(parts, results):
         stats[0] += st[0]
@@ -88,10 +83,51 @@ def calc_modes_many(structure, lights, num_BMs=None, tol=0.0, itermax=30, groups
             s.J = J[i]
             s.J_dag = Jd[i]
             s.sol1 = s.mode_pol = s.table_nod = s.type_el = s.x_arr = None
-    calc_modes_many.last_stats = stats
+    return stats
+
+
+def calc_modes_many(structure, lights, num_BMs=None, tol=0.0, itermax=30, groups=1, ncv=0):
+    """Simmo objects for every light in `lights` (same max_order_PWs),
+    computed together on the GPU (`groups` concurrent sub-batches; ncv the
+    Krylov dimension, 0 for the reference's rule)."""
+    sims, inputs, nbm, order, P = _prepare(structure, lights, num_BMs)
+    parts = _parts(len(sims), groups)
+    if len(parts) == 1:
+        results = [_solve(structure, sims, inputs, nbm, order, P, tol, itermax, ncv)]
+    else:
+        futs = [_pool(len(parts)).submit(_solve, structure, sims[a:b], inputs[a:b], nbm, order, P, tol, itermax, ncv)
+                for a, b in parts]
+        results = [f.result() for f in futs]
+    calc_modes_many.last_stats = _fill(sims, nbm, parts, results)
     return sims
 
 
+def calc_modes_stack(structures, lights, num_BMs=None, tol=0.0, itermax=30, groups=2, ncv=0, workers=None):
+    """calc_modes_many for several NanoStruct layers at once: every (layer,
+    sub-batch) job goes to one pool of `workers` host threads (default
+    groups + 1), each with its own context, so that one job's host Schur
+    steps overlap another's GPU work and the GPU does not idle between
+    layers.  Returns one list of Simmo objects per structure."""
+    preps = [_prepare(st, lights, num_BMs) for st in structures]
+    pool = _pool(int(workers) if workers else groups + 1)
+    jobs = []
+    for st, (sims, inputs, nbm, order, P) in zip(structures, preps):
+        parts = _parts(len(sims), groups)
+        futs = [pool.submit(_solve, st, sims[a:b], inputs[a:b], nbm, order, P, tol, itermax, ncv) for a, b in parts]
+        jobs.append((parts, futs))
+    stats = np.zeros(8, np.int64)
+    out = []
+    for (sims, inputs, nbm, order, P), (parts, futs) in zip(preps, jobs):
+        st = _fill(sims, nbm, parts, [f.result() for f in futs])
+        stats[0] += st[0]
+        stats[1] = max(stats[1], st[1])
+        stats[2:] += st[2:]
+        out.append(sims)
+    calc_modes_stack.last_stats = stats
+    return out
+
+
+calc_modes_stack.last_stats = None
 calc_modes_many.last_stats = None
 
 

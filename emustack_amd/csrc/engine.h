@@ -115,8 +115,14 @@ struct Profiler {
   }
 };
 
-// Column span of one split solve task (see the solve task lists).
-constexpr int kSolveSplit = 256;
+// Solve tasks of one level and direction: (front, row0, tiles, k0, slot)
+// entries, partial-sum reductions (front, row0, first slot, count), the
+// number of 64-row partial-sum slots and the column span of split tasks.
+struct SolveTasks {
+  int64_t off = 0, cnt = 0;
+  int64_t roff = 0, rcnt = 0;
+  int32_t nslot = 0, span = 0;
+};
 
 // Kernel task lists of one factorisation / solve level.
 struct StepPlan {
@@ -129,11 +135,9 @@ struct LevelSched {
   std::vector<StepPlan> steps;
   int64_t fronts_off, fronts_cnt;     // fronts of the level
   int64_t ea_off[2], ea_cnt[2];       // extend-add tasks (child front, col0) by slot
-  int64_t sf_off, sf_cnt;             // forward-solve tasks (front, row0, tiles, k0, slot)
-  int64_t sb_off, sb_cnt;             // backward-solve tasks (front, row0, tiles, k0, slot)
-  int64_t rf_off = 0, rf_cnt = 0;     // forward partial-sum reductions (front, row0, slot0, count)
-  int64_t rb_off = 0, rb_cnt = 0;     // backward partial-sum reductions
-  int32_t pf_cnt = 0, pb_cnt = 0;     // partial-sum slots of the level (64 rows each)
+  // forward / backward solve task lists, [0] without column splitting and
+  // [1] with the long tiles split (chosen per solve by the grid size)
+  SolveTasks fwd[2], bwd[2];
   int32_t max_f;
   // algorithmic bytes of one solve sweep of the level, per instance: factor
   // bytes (counted once per solve, whatever the RHS chunking: re-reads of a
@@ -254,12 +258,17 @@ void launch_permute_vec(DevMesh &dm, int nb, int ncol, const zd *src, zd *dst, b
 void launch_spmm(int nb, int64_t n, int ncol, const int64_t *rp, const int32_t *ci, const zd *vals,
                  int64_t vstride, const zd *x, int64_t xstride, zd *y, int64_t ystride,
                  hipStream_t s, KTime kt = {});
-// C[b] (M x N, ldc) = op(A[b])^T B[b] with K rows; op = conj if conjA.  Split-K
-// (partial products per K chunk in scratch, then a reduction); N <= 8 takes
-// the tall-skinny kernel, which is the one kt times.
+// C[b] (M x N, ldc) = op(A[b])^T B[b] with K rows; op = conj if conjA, split
+// over K (partial products per K chunk in scratch, then a reduction).  N in
+// {1, 2, 4, 8, 16}: the tall-skinny kernel (the one kt times); 8 x 8 Gram
+// matrices of a block: a dedicated kernel; other N: 32 x 32 output tiles.
 void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64_t sA,
                 const zd *B, int64_t ldb, int64_t sB, zd *C, int64_t ldc, int64_t sC,
                 bool conjA, zd *scratch, size_t scratch_elems, hipStream_t s, KTime kt = {});
+// C[b] = alpha op(A[b]) op(B[b]) + beta C[b], op 'N', 'T' or 'C', column-major (rocBLAS)
+void launch_gemm(int nb, char opA, char opB, int64_t M, int64_t N, int64_t K, zd alpha, const zd *A, int64_t lda,
+                 int64_t sA, const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,
+                 hipStream_t s);
 // C[b] (M x N) = alpha * A[b] (M x K) * B[b] (K x N) + beta*C[b], column-major
 void launch_gemm_nn(int nb, int64_t M, int N, int K, zd alpha, const zd *A, int64_t lda, int64_t sA,
                     const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,

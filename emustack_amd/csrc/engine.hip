@@ -20,7 +20,9 @@
 namespace emu {
 
 // One mutex per device: the "turn" that the Krylov cycles of concurrent
-// contexts on that device take for their GPU phases.
+// contexts on that device take for their GPU phases.  (Measured on the
+// stacked-gratings bench: letting two sub-batches expand at once, or three of
+// four, is no faster than strict alternation.)
 std::mutex &gpu_turn(int device) {
   static std::mutex turns[64];
   return turns[device & 63];
@@ -205,28 +207,42 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
     }
     // solve tasks (front, r0, tiles per workgroup, k0, slot): 64-row tiles,
     // 4 per workgroup when the front has at least 4, else 2 or 1 with the
-    // waves splitting each tile's columns (k0 = slot = -1).  The fronts at
-    // the top of the tree are few and long: a tile whose column range exceeds
-    // kSolveSplit columns is cut into tasks of kSolveSplit columns (from k0)
-    // that write partial sums to a slot, and a reduction task (front, r0,
-    // first slot, count) adds them, so these levels still fill the GPU.
-    auto solve_tasks = [&](bool fwd, std::vector<int32_t> &red, int32_t &nslot) {
-      nslot = 0;
+    // waves splitting each tile's columns (k0 = slot = -1).  In the split
+    // variant a tile whose column range exceeds the level's span is cut into
+    // tasks of `span` columns (from k0) that write partial sums to a slot,
+    // and a reduction task (front, r0, first slot, count) adds them.  The
+    // span is 64 columns on narrow levels (fewer than 256 unsplit tasks: the
+    // few long fronts at the top of the tree, whose tasks would otherwise be
+    // long serial column loops on a handful of CUs) and 256 elsewhere.  The
+    // split variant serves the Krylov solves (at most 8 right-hand sides),
+    // the plain one solves with many right-hand sides.
+    auto solve_tasks = [&](bool fwd, bool split, SolveTasks &st) {
+      int64_t unsplit = 0;
+      for (int32_t t : LP.fronts) {
+        const int32_t R = fwd ? sy.fronts[t].f : sy.fronts[t].s;
+        const int32_t T = (R + 63) / 64;
+        const int32_t tpw = T >= 4 ? 4 : (T >= 2 ? 2 : 1);
+        unsplit += (T + tpw - 1) / tpw;
+      }
+      st.span = !split ? INT32_MAX : (unsplit < 256 ? 64 : 256);
+      st.nslot = 0;
+      std::vector<int32_t> red;
+      st.off = (int64_t)tasks.size();
       for (int32_t t : LP.fronts) {
         const FrontInfo &F = sy.fronts[t];
         const int32_t R = fwd ? F.f : F.s;
         const int32_t kmax = fwd ? F.s : F.f;
-        if (kmax > kSolveSplit) {
+        if (kmax > st.span) {
           for (int32_t r0 = 0; r0 < R; r0 += 64) {
             const int32_t kb = fwd ? 0 : r0, ke = fwd ? std::min(F.s, r0 + 64) : F.f;
-            const int32_t nc = (ke - kb + kSolveSplit - 1) / kSolveSplit;
+            const int32_t nc = (ke - kb + st.span - 1) / st.span;
             if (nc <= 1) {
               tasks.insert(tasks.end(), {t, r0, 1, kb, -1});
               continue;
             }
-            red.insert(red.end(), {t, r0, nslot, nc});
-            for (int32_t c = 0; c < nc; ++c) tasks.insert(tasks.end(), {t, r0, 1, kb + c * kSolveSplit, nslot + c});
-            nslot += nc;
+            red.insert(red.end(), {t, r0, st.nslot, nc});
+            for (int32_t c = 0; c < nc; ++c) tasks.insert(tasks.end(), {t, r0, 1, kb + c * st.span, st.nslot + c});
+            st.nslot += nc;
           }
         } else {
           const int32_t T = (R + 63) / 64;
@@ -234,21 +250,16 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
           for (int32_t r0 = 0; r0 < R; r0 += 64 * tpw) tasks.insert(tasks.end(), {t, r0, tpw, -1, -1});
         }
       }
+      st.cnt = ((int64_t)tasks.size() - st.off) / 5;
+      st.roff = (int64_t)tasks.size();
+      tasks.insert(tasks.end(), red.begin(), red.end());
+      st.rcnt = (int64_t)red.size() / 4;
+      dm.part_rhs = std::max<int64_t>(dm.part_rhs, 64 * (int64_t)st.nslot);
     };
-    std::vector<int32_t> redf, redb;
-    L.sf_off = (int64_t)tasks.size();
-    solve_tasks(true, redf, L.pf_cnt);
-    L.sf_cnt = ((int64_t)tasks.size() - L.sf_off) / 5;
-    L.sb_off = (int64_t)tasks.size();
-    solve_tasks(false, redb, L.pb_cnt);
-    L.sb_cnt = ((int64_t)tasks.size() - L.sb_off) / 5;
-    L.rf_off = (int64_t)tasks.size();
-    tasks.insert(tasks.end(), redf.begin(), redf.end());
-    L.rf_cnt = (int64_t)redf.size() / 4;
-    L.rb_off = (int64_t)tasks.size();
-    tasks.insert(tasks.end(), redb.begin(), redb.end());
-    L.rb_cnt = (int64_t)redb.size() / 4;
-    dm.part_rhs = std::max<int64_t>(dm.part_rhs, 64 * (int64_t)std::max(L.pf_cnt, L.pb_cnt));
+    for (int v = 0; v < 2; ++v) {
+      solve_tasks(true, v == 1, L.fwd[v]);
+      solve_tasks(false, v == 1, L.bwd[v]);
+    }
     // algorithmic bytes of one solve sweep of this level (per instance):
     // triangle incl. diagonal + off-diagonal block of the inverted form once
     // per solve; b/y/x/update vectors per RHS

@@ -96,13 +96,17 @@ __global__ __launch_bounds__(256) void k_atb_partial(int64_t K, int M, int N, in
   zd acc[2][2] = {{zmk(0, 0), zmk(0, 0)}, {zmk(0, 0), zmk(0, 0)}};
   const int64_t kb = chunk * kchunk, ke = min(K, kb + kchunk);
   for (int64_t k0 = kb; k0 < ke; k0 += AKC) {
-    for (int idx = tid; idx < AKC * AT; idx += 256) {
-      int kk = idx % AKC, mm = idx / AKC;
-      int64_t k = k0 + kk;
-      zd a = (k < ke && m0 + mm < M) ? Ab[k + (int64_t)(m0 + mm) * lda] : zmk(0, 0);
-      As[kk][mm] = conjA ? zconj(a) : a;
-      zd bb = (k < ke && n0 + mm < N) ? Bb[k + (int64_t)(n0 + mm) * ldb] : zmk(0, 0);
-      Bs[kk][mm] = bb;
+    // unconditional loads from clamped addresses (a load inside a branch
+    // is waited for before the branch merges)
+#pragma unroll
+    for (int i = 0; i < AKC * AT / 256; ++i) {
+      const int idx = tid + 256 * i, kk = idx % AKC, mm = idx / AKC;
+      const int64_t k = k0 + kk;
+      const int64_t kc = k < ke ? k : ke - 1;
+      const zd a = Ab[kc + (int64_t)min(m0 + mm, M - 1) * lda];
+      const zd bb = Bb[kc + (int64_t)min(n0 + mm, N - 1) * ldb];
+      As[kk][mm] = (k < ke && m0 + mm < M) ? (conjA ? zconj(a) : a) : zmk(0, 0);
+      Bs[kk][mm] = (k < ke && n0 + mm < N) ? bb : zmk(0, 0);
     }
     __syncthreads();
 #pragma unroll 8
@@ -254,6 +258,53 @@ void launch_swap_i32(int64_t n, int32_t *a, int32_t *b, hipStream_t s) {
 }
 
 
+// Gram matrix W^H W of an 8-column block (K x 8, ld ldw), partial sums per K
+// chunk (layout of k_atb_tall's partials).  A 256-row tile of W is staged in
+// LDS; thread (o, g) owns output o = (p, q) and sums rows g, g + 4, ... of the
+// tile, so every lane does useful work (the general tall kernel would keep
+// 56 of its 64 columns idle).
+constexpr int GR_TK = 256;
+
+__global__ __launch_bounds__(256) void k_gram8(int64_t K, int64_t kchunk, const zd *__restrict__ W, int64_t ldw,
+                                               int64_t sW, zd *__restrict__ part) {
+  constexpr int NQ = 8;
+  __shared__ zd Ws[GR_TK][NQ + 1];
+  const int chunk = blockIdx.x, nchunk = gridDim.x, b = blockIdx.y, tid = threadIdx.x;
+  const int o = tid % (NQ * NQ), g = tid / (NQ * NQ), p = o % NQ, q = o / NQ;
+  const zd *Wb = W + b * sW;
+  const int64_t kb = chunk * kchunk, ke = min(K, kb + kchunk);
+  zd rw[NQ];
+  auto fetch = [&](int64_t k0) {
+    const int64_t k = k0 + tid;
+    const bool ok = k < ke;
+    const int64_t kc = ok ? k : ke - 1;
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+      const zd v = Wb[kc + (int64_t)c * ldw];
+      rw[c] = ok ? v : zmk(0, 0);
+    }
+  };
+  zd acc = zmk(0, 0);
+  if (kb < ke) fetch(kb);
+  for (int64_t k0 = kb; k0 < ke; k0 += GR_TK) {
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) Ws[tid][c] = rw[c];
+    __syncthreads();
+    if (k0 + GR_TK < ke) fetch(k0 + GR_TK);
+#pragma unroll 8
+    for (int kk = g; kk < GR_TK; kk += 4) acc = zfma(zconj(Ws[kk][p]), Ws[kk][q], acc);
+  }
+  __syncthreads();
+  zd *red = &Ws[0][0];
+  red[tid] = acc;
+  __syncthreads();
+  if (tid < NQ * NQ) {
+    const zd v = zadd(zadd(red[tid], red[tid + 64]), zadd(red[tid + 128], red[tid + 192]));
+    part[(((int64_t)b * nchunk + chunk) * NQ + q) * NQ + p] = v;
+  }
+}
+
 // C[b][m + nn ldc] = sum over chunks of part[b][chunk][nn][m]: 64 outputs per
 // workgroup, its four waves summing every fourth chunk, combined in LDS.
 __global__ __launch_bounds__(256) void k_atb_reduce(int M, int N, int nchunk, const zd *__restrict__ part,
@@ -276,32 +327,61 @@ __global__ __launch_bounds__(256) void k_atb_reduce(int M, int N, int nchunk, co
 void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64_t sA, const zd *B,
                 int64_t ldb, int64_t sB, zd *C, int64_t ldc, int64_t sC, bool conjA, zd *scratch,
                 size_t scratch_elems, hipStream_t s, KTime kt) {
+  if (M <= 0 || N <= 0 || nb <= 0) return;
   const bool tall = N == 1 || N == 2 || N == 4 || N == 8 || N == 16;
+  if (!tall) {  // wide products (mode overlaps, J contractions): 32 x 32 output tiles, split over K
+    const int tiles = ((M + AT - 1) / AT) * ((N + AT - 1) / AT);
+    const int64_t per = (int64_t)nb * M * N;
+    const int64_t maxchunk = std::max<int64_t>(1, (int64_t)(scratch_elems / std::max<int64_t>(per, 1)));
+    const int64_t want = std::max<int64_t>(1, 2048 / std::max(1, tiles * nb));
+    int64_t nchunk = std::min<int64_t>(std::min<int64_t>(want, maxchunk), (K + AKC - 1) / AKC);
+    nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, 65535));
+    int64_t kchunk = (K + nchunk - 1) / nchunk;
+    kchunk = ((kchunk + AKC - 1) / AKC) * AKC;
+    nchunk = (K + kchunk - 1) / kchunk;
+    if ((size_t)(nchunk * per) > scratch_elems) throw HipError("atb: scratch too small");
+    hipLaunchKernelGGL(k_atb_partial, dim3(tiles, (unsigned)nchunk, nb), dim3(256), 0, s, K, M, N, kchunk, A, lda, sA,
+                       B, ldb, sB, conjA ? 1 : 0, scratch, (M + AT - 1) / AT);
+    EMU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_atb_reduce, dim3((unsigned)((per / nb + 63) / 64), nb), dim3(256), 0, s, M, N, (int)nchunk,
+                       scratch, C, ldc, sC);
+    EMU_LAUNCH_CHECK();
+    return;
+  }
+  if (M == 8 && N == 8 && conjA && A == B && lda == ldb && sA == sB) {  // Gram matrix of a block
+    const int64_t per = (int64_t)nb * 64;
+    int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(1024 / std::max(nb, 1), (K + GR_TK - 1) / GR_TK));
+    nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, (int64_t)(scratch_elems / per)));
+    int64_t kchunk = (K + nchunk - 1) / nchunk;
+    kchunk = ((kchunk + GR_TK - 1) / GR_TK) * GR_TK;
+    nchunk = (K + kchunk - 1) / kchunk;
+    hipExtLaunchKernelGGL(k_gram8, dim3((unsigned)nchunk, nb), dim3(256), 0, s, kt.a, kt.b, 0, K, kchunk, A, lda, sA,
+                          scratch);
+    EMU_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_atb_reduce, dim3(1, nb), dim3(256), 0, s, 8, 8, (int)nchunk, scratch, C, ldc, sC);
+    EMU_LAUNCH_CHECK();
+    return;
+  }
   const int cpt = N == 16 ? 2 : 1;
-  const int kstep = tall ? 32 / cpt : AKC;
-  const int tiles = tall ? (M + 64 * cpt - 1) / (64 * cpt) : ((M + AT - 1) / AT) * ((N + AT - 1) / AT);
+  const int kstep = 32 / cpt;
+  const int tiles = (M + 64 * cpt - 1) / (64 * cpt);
   int64_t per = (int64_t)nb * M * N;
   int64_t maxchunk = std::max<int64_t>(1, (int64_t)(scratch_elems / std::max<int64_t>(per, 1)));
   int64_t want = std::max<int64_t>(1, 2048 / std::max(1, tiles * nb));
   int64_t nchunk = std::min<int64_t>(std::min<int64_t>(want, maxchunk), (K + kstep - 1) / kstep);
-  nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, tall ? 256 : 65535));
+  nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, 256));
   int64_t kchunk = (K + nchunk - 1) / nchunk;
   kchunk = ((kchunk + kstep - 1) / kstep) * kstep;
   nchunk = (K + kchunk - 1) / kchunk;
   if ((size_t)(nchunk * per) > scratch_elems) throw HipError("atb: scratch too small");
   const int cj = conjA ? 1 : 0;
-  if (tall) {
-    dim3 g((unsigned)tiles, (unsigned)nchunk, nb), t(256);
-    switch (N) {
-      case 1: hipExtLaunchKernelGGL(k_atb_tall<1, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
-      case 2: hipExtLaunchKernelGGL(k_atb_tall<2, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
-      case 4: hipExtLaunchKernelGGL(k_atb_tall<4, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
-      case 8: hipExtLaunchKernelGGL(k_atb_tall<8, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
-      default: hipExtLaunchKernelGGL(k_atb_tall<16, 2>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
-    }
-  } else {
-    hipLaunchKernelGGL(k_atb_partial, dim3(tiles, (unsigned)nchunk, nb), dim3(256), 0, s, K, M, N, kchunk, A, lda,
-                       sA, B, ldb, sB, cj, scratch, (M + AT - 1) / AT);
+  dim3 g((unsigned)tiles, (unsigned)nchunk, nb), t(256);
+  switch (N) {
+    case 1: hipExtLaunchKernelGGL(k_atb_tall<1, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+    case 2: hipExtLaunchKernelGGL(k_atb_tall<2, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+    case 4: hipExtLaunchKernelGGL(k_atb_tall<4, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+    case 8: hipExtLaunchKernelGGL(k_atb_tall<8, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+    default: hipExtLaunchKernelGGL(k_atb_tall<16, 2>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
   }
   EMU_LAUNCH_CHECK();
   const int64_t tot = (int64_t)M * N;
@@ -331,16 +411,26 @@ static rocblas_handle blas_handle(hipStream_t s) {
   return h;
 }
 
-void launch_gemm_nn(int nb, int64_t M, int N, int K, zd alpha, const zd *A, int64_t lda, int64_t sA,
-                    const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,
-                    hipStream_t s) {
+void launch_gemm(int nb, char opA, char opB, int64_t M, int64_t N, int64_t K, zd alpha, const zd *A, int64_t lda,
+                 int64_t sA, const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,
+                 hipStream_t s) {
   if (M <= 0 || N <= 0 || nb <= 0) return;
+  auto op = [](char c) {
+    return c == 'C' ? rocblas_operation_conjugate_transpose
+                    : (c == 'T' ? rocblas_operation_transpose : rocblas_operation_none);
+  };
   using rz = rocblas_double_complex;
   const rocblas_status st = rocblas_zgemm_strided_batched_64(
-      blas_handle(s), rocblas_operation_none, rocblas_operation_none, M, N, K, reinterpret_cast<const rz *>(&alpha),
+      blas_handle(s), op(opA), op(opB), M, N, K, reinterpret_cast<const rz *>(&alpha),
       reinterpret_cast<const rz *>(A), lda, sA, reinterpret_cast<const rz *>(B), ldb, sB,
       reinterpret_cast<const rz *>(&beta), reinterpret_cast<rz *>(C), ldc, sC, nb);
   if (st != rocblas_status_success) throw HipError("rocblas_zgemm_strided_batched failed (status " + std::to_string((int)st) + ")");
+}
+
+void launch_gemm_nn(int nb, int64_t M, int N, int K, zd alpha, const zd *A, int64_t lda, int64_t sA,
+                    const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,
+                    hipStream_t s) {
+  launch_gemm(nb, 'N', 'N', M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, s);
 }
 
 // ------------------------------------------------------- w -= V[:, :k] h
@@ -589,15 +679,20 @@ __global__ __launch_bounds__(256) void k_dcgs_coef(int j, int c0, const zd *__re
     const int o = tid % (B * B), kind = tid / (B * B), p = o % B, q = o / B;
     zd acc = zmk(0, 0);
     if (kind == 0) {
+#pragma unroll 8
       for (int i = 0; i < j; ++i) acc = zfma(zconj(Pb[i + p * ldp]), Pb[i + q * ldp], acc);
       SS[p][q] = acc;
     } else if (kind == 1) {
-      if (!FIN)
+      if (!FIN) {
+#pragma unroll 8
         for (int i = 0; i < j; ++i) acc = zfma(zconj(Pb[i + p * ldp]), Pb[i + (B + q) * ldp], acc);
+      }
       ST[p][q] = acc;
     } else {
-      if (!FIN)
+      if (!FIN) {
+#pragma unroll 8
         for (int c = c0; c < j; ++c) acc = zfma(Hb[(j + p) + (int64_t)c * ldh], Pb[c + q * ldp], acc);
+      }
       U[p][q] = acc;
     }
   }

@@ -78,9 +78,10 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ tasks
     for (int i = tid; i < s; i += 256) sg[i] = i;
     __syncthreads();
   }
-  __shared__ double redv[256];
-  __shared__ int redi[256];
+  __shared__ double redv[4];
+  __shared__ int redi[4];
   __shared__ zd urow[NB];
+  const int lane = tid & 63, wave = tid >> 6;
   for (int jj = 0; jj < w; ++jj) {
     const int k = k0 + jj;
     double best = -1.0;
@@ -89,21 +90,31 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ tasks
       double a = zabs1(F[r + (int64_t)k * f]);
       if (a > best) { best = a; bi = r; }
     }
-    redv[tid] = best;
-    redi[tid] = bi;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if (tid < st) {
-        double a = redv[tid + st];
-        int ai = redi[tid + st];
-        if (a > redv[tid] || (a == redv[tid] && ai < redi[tid])) {
-          redv[tid] = a;
-          redi[tid] = ai;
-        }
+    // pivot search: largest |.|_1, smallest row on ties (wave shuffles, then
+    // the four waves' winners through LDS: two barriers per column)
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+      const double a = __shfl_xor(best, m, 64);
+      const int ai = __shfl_xor(bi, m, 64);
+      if (a > best || (a == best && ai < bi)) {
+        best = a;
+        bi = ai;
       }
-      __syncthreads();
     }
-    const int p = redi[0];
+    if (lane == 0) {
+      redv[wave] = best;
+      redi[wave] = bi;
+    }
+    __syncthreads();
+    int p = redi[0];
+    {
+      double pb = redv[0];
+      for (int q = 1; q < 4; ++q)
+        if (redv[q] > pb || (redv[q] == pb && redi[q] < p)) {
+          pb = redv[q];
+          p = redi[q];
+        }
+    }
     if (p != k)
       for (int c = tid; c < f; c += 256) {
         zd a = F[k + (int64_t)c * f];
@@ -131,9 +142,16 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ tasks
     for (int r = k + 1 + tid; r < f; r += 256) {
       zd l = zmul(F[r + (int64_t)k * f], dinv);
       F[r + (int64_t)k * f] = l;
-      for (int c = 0; c < nrest; ++c) {
-        zd *pr = &F[r + (int64_t)(k + 1 + c) * f];
-        *pr = zfms(l, urow[c], *pr);
+      // rank-1 update of the row's remaining panel columns, eight at a time
+      // (all eight loads in flight before the updates)
+      zd *pr = &F[r + (int64_t)(k + 1) * f];
+      for (int c0 = 0; c0 < nrest; c0 += 8) {
+        zd v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = pr[(int64_t)min(c0 + u, nrest - 1) * f];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (c0 + u < nrest) pr[(int64_t)(c0 + u) * f] = zfms(l, urow[c0 + u], v[u]);
       }
     }
     __syncthreads();
@@ -159,7 +177,10 @@ __global__ __launch_bounds__(64) void k_trsm(const int32_t *__restrict__ tasks, 
   zd *col = F + (int64_t)c * f + k0;
   zd x[NB];
 #pragma unroll
-  for (int i = 0; i < NB; ++i) x[i] = i < w ? col[i] : zmk(0, 0);
+  for (int i = 0; i < NB; ++i) {
+    const zd v = col[min(i, w - 1)];
+    x[i] = i < w ? v : zmk(0, 0);
+  }
 #pragma unroll
   for (int i = 1; i < NB; ++i) {
     if (i < w) {
@@ -185,15 +206,20 @@ __global__ __launch_bounds__(256) void k_gemm_update(const int32_t *__restrict__
   const int w = min(NB, s - k0);
   __shared__ zd As[NB][TILE];
   __shared__ zd Bs[NB][TILE];
-  for (int idx = tid; idx < NB * TILE; idx += 256) {
-    int i = idx % TILE, kk = idx / TILE;
-    int r = r0 + i;
-    As[kk][i] = (kk < w && r < f) ? F[r + (int64_t)(k0 + kk) * f] : zmk(0, 0);
+  // staging: unconditional loads from clamped addresses (a load inside a
+  // branch is waited for before the branch merges, which would serialise
+  // the tile's loads), out-of-range entries zeroed afterwards
+#pragma unroll
+  for (int u = 0; u < NB * TILE / 256; ++u) {
+    const int idx = tid + 256 * u, i = idx % TILE, kk = idx / TILE, r = r0 + i;
+    const zd v = F[min(r, f - 1) + (int64_t)(k0 + min(kk, w - 1)) * f];
+    As[kk][i] = (kk < w && r < f) ? v : zmk(0, 0);
   }
-  for (int idx = tid; idx < NB * TILE; idx += 256) {
-    int kk = idx % NB, j = idx / NB;
-    int c = c0 + j;
-    Bs[kk][j] = (kk < w && c < f) ? F[(k0 + kk) + (int64_t)c * f] : zmk(0, 0);
+#pragma unroll
+  for (int u = 0; u < NB * TILE / 256; ++u) {
+    const int idx = tid + 256 * u, kk = idx % NB, j = idx / NB, c = c0 + j;
+    const zd v = F[(k0 + min(kk, w - 1)) + (int64_t)min(c, f - 1) * f];
+    Bs[kk][j] = (kk < w && c < f) ? v : zmk(0, 0);
   }
   __syncthreads();
   const int tx = tid % 16, ty = tid / 16;
@@ -340,10 +366,13 @@ __global__ __launch_bounds__(256) void k_triinv(const int32_t *__restrict__ task
     const int Kb = kind ? I + 1 : J, Ke = kind ? J + 1 : I;
     for (int K = Kb; K < Ke; ++K) {
       const int k0 = K * TB, wk = min(TB, s - k0);
-      for (int idx = tid; idx < TB * TB; idx += 256) {
-        int i = idx % TB, kk = idx / TB;
-        Ts[i][kk] = (i < wi && kk < wk) ? F[(i0 + i) + (int64_t)(k0 + kk) * f] : zmk(0, 0);
-        Xs[i][kk] = (i < wk && kk < wj) ? O[(k0 + i) + (int64_t)(j0 + kk) * ldo] : zmk(0, 0);
+#pragma unroll
+      for (int u = 0; u < TB * TB / 256; ++u) {
+        const int idx = tid + 256 * u, i = idx % TB, kk = idx / TB;
+        const zd a = F[(i0 + min(i, wi - 1)) + (int64_t)(k0 + min(kk, wk - 1)) * f];
+        const zd x = O[(k0 + min(i, wk - 1)) + (int64_t)(j0 + min(kk, wj - 1)) * ldo];
+        Ts[i][kk] = (i < wi && kk < wk) ? a : zmk(0, 0);
+        Xs[i][kk] = (i < wk && kk < wj) ? x : zmk(0, 0);
       }
       __syncthreads();
 #pragma unroll 4
@@ -354,9 +383,11 @@ __global__ __launch_bounds__(256) void k_triinv(const int32_t *__restrict__ task
       }
       __syncthreads();
     }
-    for (int idx = tid; idx < TB * TB; idx += 256) {
-      int i = idx % TB, j = idx / TB;
-      Ts[i][j] = (i < wi && j < wi) ? F[(i0 + i) + (int64_t)(i0 + j) * f] : zmk(0, 0);
+#pragma unroll
+    for (int u = 0; u < TB * TB / 256; ++u) {
+      const int idx = tid + 256 * u, i = idx % TB, j = idx / TB;
+      const zd a = F[(i0 + min(i, wi - 1)) + (int64_t)(i0 + min(j, wi - 1)) * f];
+      Ts[i][j] = (i < wi && j < wi) ? a : zmk(0, 0);
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m) Xs[r][cg + 8 * m] = acc[m];
@@ -411,19 +442,21 @@ __global__ __launch_bounds__(256) void k_wz(const int32_t *__restrict__ tasks, F
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = zmk(0, 0);
   for (int k0 = kbeg; k0 < s; k0 += NB) {
-    for (int idx = tid; idx < NB * TILE; idx += 256) {
-      int i = idx % TILE, kk = idx / TILE;
-      int rr = r0 + i, k = k0 + kk;
-      zd a = zmk(0, 0);
-      if (rr < M && k < s) a = kind ? Obw[rr + (int64_t)k * s] : F[(s + rr) + (int64_t)k * f];
-      As[kk][i] = a;
+#pragma unroll
+    for (int u = 0; u < NB * TILE / 256; ++u) {
+      const int idx = tid + 256 * u, i = idx % TILE, kk = idx / TILE;
+      const int rr = r0 + i, k = k0 + kk;
+      const int rc = min(rr, M - 1), kc = min(k, s - 1);
+      const zd a = kind ? Obw[rc + (int64_t)kc * s] : F[(s + rc) + (int64_t)kc * f];
+      As[kk][i] = (rr < M && k < s) ? a : zmk(0, 0);
     }
-    for (int idx = tid; idx < NB * TILE; idx += 256) {
-      int kk = idx % NB, j = idx / NB;
-      int cc = c0 + j, k = k0 + kk;
-      zd v = zmk(0, 0);
-      if (cc < N && k < s) v = kind ? F[k + (int64_t)(s + cc) * f] : Ofw[k + (int64_t)cc * f];
-      Bs[kk][j] = v;
+#pragma unroll
+    for (int u = 0; u < NB * TILE / 256; ++u) {
+      const int idx = tid + 256 * u, kk = idx % NB, j = idx / NB;
+      const int cc = c0 + j, k = k0 + kk;
+      const int ccc = min(cc, N - 1), kc = min(k, s - 1);
+      const zd v = kind ? F[kc + (int64_t)(s + ccc) * f] : Ofw[kc + (int64_t)ccc * f];
+      Bs[kk][j] = (cc < N && k < s) ? v : zmk(0, 0);
     }
     __syncthreads();
 #pragma unroll 4
@@ -489,7 +522,7 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
                                              const zd *__restrict__ inv, int64_t isz, const int32_t *__restrict__ sig,
                                              int64_t npiv, int nrhs, int64_t n, int64_t nw,
                                              const zd *__restrict__ bv, zd *__restrict__ y, zd *__restrict__ wbuf,
-                                             zd *__restrict__ part, int nslot) {
+                                             zd *__restrict__ part, int nslot, int span) {
   const int32_t *tk = tasks + 5 * blockIdx.x;
   const int t = tk[0], r0 = tk[1], tpw = tk[2], kb = tk[3], slot = tk[4];
   const int b = blockIdx.y;
@@ -503,12 +536,12 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
   const int row = r0 + 64 * tile + lane;
-  // columns of this task: the workgroup's range, cut to [kb, kb + kSolveSplit)
+  // columns of this task: the workgroup's range, cut to [kb, kb + span)
   // for a split task; lower-triangular rows need k <= row
   int kbeg = 0, kend = min(s, r0 + 64 * tpw);
   if (kb >= 0) {
     kbeg = kb;
-    kend = min(kend, kb + kSolveSplit);
+    kend = min(kend, kb + span);
   }
   const int kend_w = min(kend, min(s, r0 + 64 * (tile + 1)));
   __shared__ zd xin[KC][RC];
@@ -629,7 +662,7 @@ template <int RC>
 __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, FrontArrays A,
                                              const zd *__restrict__ inv, int64_t isz, int nrhs, int64_t n,
                                              const zd *__restrict__ y, zd *__restrict__ x, zd *__restrict__ part,
-                                             int nslot) {
+                                             int nslot, int span) {
   const int32_t *tk = tasks + 5 * blockIdx.x;
   const int t = tk[0], r0 = tk[1], tpw = tk[2], kb = tk[3], slot = tk[4];
   const int b = blockIdx.y;
@@ -644,11 +677,11 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, 
   const int row = r0 + 64 * tile;
   const int rowl = row + lane;
   // columns: [r0, f) for the workgroup (upper-triangular rows need k >= row),
-  // cut to [kb, kb + kSolveSplit) for a split task
+  // cut to [kb, kb + span) for a split task
   int kbeg = r0, kend = f;
   if (kb >= 0) {
     kbeg = kb;
-    kend = min(f, kb + kSolveSplit);
+    kend = min(f, kb + span);
   }
   __shared__ zd xin[KC][RC];
   __shared__ zd red[3][64][RC];
@@ -748,32 +781,40 @@ static int pick_rc(int nrhs, int64_t tasks, int nb) {
 }
 
 template <int RC>
-static void launch_fwd(const LevelSched &L, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *inv,
+static void launch_fwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *inv,
                        const int32_t *sig, const zd *bnew, zd *y, zd *w, zd *part, hipStream_t s, KTime kt) {
   const Symbolic &sy = dm.host.sym;
   const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
   const int64_t nw = std::max<int64_t>(sy.nw, 1);
-  hipExtLaunchKernelGGL(k_fwd<RC>, dim3((unsigned)L.sf_cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
-                        dm.tasks.p + L.sf_off, A, inv, dm.isz, sig, sy.npiv, nrhs, sy.n, nw, bnew, y, w, part,
-                        (int)L.pf_cnt);
-  if (L.rf_cnt > 0)
-    hipLaunchKernelGGL(k_fwd_red<RC>, dim3((unsigned)L.rf_cnt, nb, nz), dim3(64), 0, s, dm.tasks.p + L.rf_off, A,
-                       nrhs, sy.n, nw, part, (int)L.pf_cnt, y, w);
+  hipExtLaunchKernelGGL(k_fwd<RC>, dim3((unsigned)T.cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
+                        dm.tasks.p + T.off, A, inv, dm.isz, sig, sy.npiv, nrhs, sy.n, nw, bnew, y, w, part,
+                        (int)T.nslot, (int)std::min<int64_t>(T.span, INT32_MAX));
+  if (T.rcnt > 0)
+    hipLaunchKernelGGL(k_fwd_red<RC>, dim3((unsigned)T.rcnt, nb, nz), dim3(64), 0, s, dm.tasks.p + T.roff, A, nrhs,
+                       sy.n, nw, part, (int)T.nslot, y, w);
 }
 
 template <int RC>
-static void launch_bwd(const LevelSched &L, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *inv,
+static void launch_bwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *inv,
                        const zd *y, zd *x, zd *part, hipStream_t s, KTime kt) {
   const Symbolic &sy = dm.host.sym;
   const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
-  hipExtLaunchKernelGGL(k_bwd<RC>, dim3((unsigned)L.sb_cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
-                        dm.tasks.p + L.sb_off, A, inv, dm.isz, nrhs, sy.n, y, x, part, (int)L.pb_cnt);
-  if (L.rb_cnt > 0)
-    hipLaunchKernelGGL(k_bwd_red<RC>, dim3((unsigned)L.rb_cnt, nb, nz), dim3(64), 0, s, dm.tasks.p + L.rb_off, A,
-                       nrhs, sy.n, part, (int)L.pb_cnt, x);
+  hipExtLaunchKernelGGL(k_bwd<RC>, dim3((unsigned)T.cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
+                        dm.tasks.p + T.off, A, inv, dm.isz, nrhs, sy.n, y, x, part, (int)T.nslot,
+                        (int)std::min<int64_t>(T.span, INT32_MAX));
+  if (T.rcnt > 0)
+    hipLaunchKernelGGL(k_bwd_red<RC>, dim3((unsigned)T.rcnt, nb, nz), dim3(64), 0, s, dm.tasks.p + T.roff, A, nrhs,
+                       sy.n, part, (int)T.nslot, x);
 }
 
 int64_t solve_ws(const DevMesh &dm) { return std::max<int64_t>(dm.host.sym.nw, 1) + dm.part_rhs; }
+
+// The split task lists for the Krylov solves (one block of at most 8
+// right-hand sides: the grid has no RHS dimension to fill the GPU with); the
+// plain ones for many right-hand sides (dense solves of the cascade), where
+// RHS chunks already give a large grid and partial sums would only add
+// traffic.
+static const SolveTasks &pick_tasks(const SolveTasks *v, int nrhs) { return nrhs <= 8 ? v[1] : v[0]; }
 
 void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *sig, const zd *bnew, zd *y, zd *x,
               zd *w, hipStream_t s, Profiler *prof) {
@@ -782,24 +823,26 @@ void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *sig, 
   // partial sums of split tasks live behind the update vectors in w
   zd *part = w + (int64_t)nb * nrhs * std::max<int64_t>(dm.host.sym.nw, 1);
   for (const LevelSched &L : dm.sched) {
-    const int rc = pick_rc(nrhs, L.sf_cnt, nb);
+    const SolveTasks &T = pick_tasks(L.fwd, nrhs);
+    const int rc = pick_rc(nrhs, T.cnt, nb);
     KTime kt = prof ? prof->time(0, L.fwd_mat * nb + L.fwd_vec * vecs) : KTime{};
     switch (rc) {
-      case 8: launch_fwd<8>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
-      case 4: launch_fwd<4>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
-      case 2: launch_fwd<2>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
-      default: launch_fwd<1>(L, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
+      case 8: launch_fwd<8>(T, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
+      case 4: launch_fwd<4>(T, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
+      case 2: launch_fwd<2>(T, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
+      default: launch_fwd<1>(T, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
     }
   }
   EMU_LAUNCH_CHECK();
   for (auto it = dm.sched.rbegin(); it != dm.sched.rend(); ++it) {
-    const int rc = pick_rc(nrhs, it->sb_cnt, nb);
+    const SolveTasks &T = pick_tasks(it->bwd, nrhs);
+    const int rc = pick_rc(nrhs, T.cnt, nb);
     KTime kt = prof ? prof->time(1, it->bwd_mat * nb + it->bwd_vec * vecs) : KTime{};
     switch (rc) {
-      case 8: launch_bwd<8>(*it, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
-      case 4: launch_bwd<4>(*it, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
-      case 2: launch_bwd<2>(*it, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
-      default: launch_bwd<1>(*it, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
+      case 8: launch_bwd<8>(T, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
+      case 4: launch_bwd<4>(T, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
+      case 2: launch_bwd<2>(T, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
+      default: launch_bwd<1>(T, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
     }
   }
   EMU_LAUNCH_CHECK();

@@ -172,3 +172,24 @@ def test_dropin_api_case2():
     for fn, v in (('Absorptance_stack0001.txt', s.a_list[-1]), ('Reflectance_stack0001.txt', s.r_list[-1]),
                   ('Transmittance_stack0001.txt', s.t_list[-1])):
         np.testing.assert_allclose(float(v), load_txt('case_2', fn)[1], rtol=1e-6, atol=1e-6, err_msg=fn)
+
+
+def test_stack_batch_equals_per_layer(gpu_ctx):
+    """batch.calc_modes_stack (all layers' sub-batches on one thread pool, one
+    context per thread) gives what calc_modes_many gives layer by layer."""
+    from emustack_amd import objects, materials
+    from emustack_amd.batch import calc_modes_many, calc_modes_stack
+    import json
+    d = json.load(open(os.path.join(GOLDEN, 'materials.json')))
+    lights = [objects.Light(wl, max_order_PWs=1, theta=0.0, phi=0.0) for wl in (500.0, 1000.0)]
+    structs = [objects.NanoStruct('2D_array', 600, 120, height_nm=h, background=materials.Air, loss=True,
+                                  inclusion_a=materials.Material(complex(*d['Si_c']['500.0'])) if h < 300 else
+                                  materials.Material(3.0 + 0.01j), make_mesh_now=False, mesh_file=MESH_600)
+               for h in (200.0, 400.0)]
+    many = [calc_modes_many(s, lights, num_BMs=20, groups=2) for s in structs]
+    stack = calc_modes_stack(structs, lights, num_BMs=20, groups=2)
+    for a, b in zip(many, stack):
+        for sa, sb in zip(a, b):
+            np.testing.assert_allclose(np.asarray(sb.k_z), np.asarray(sa.k_z), rtol=1e-9)
+            P1, P2 = np.asarray(sa.J) @ np.asarray(sa.J_dag), np.asarray(sb.J) @ np.asarray(sb.J_dag)
+            np.testing.assert_allclose(P2, P1, rtol=0, atol=1e-8 * np.abs(P1).max())
