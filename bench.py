@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # defaults, calibrated against a plain 2 GiB copy; the runs and their
 # commands are summarised in profiles/kernels_stacked_gratings_8k.md.
 TRAFFIC_PER_DISPATCH = {
-    'k_atb_tall<16, 2>': 3802547283,
+    'k_atb_mfma16<true>': 3802547283,
     'k_combine<16>': 3656499385,
     'k_fwd': 367811805,
     'k_bwd': 368452440,
@@ -213,7 +213,7 @@ def main():
     dom = max(prof, key=lambda k: prof[k][1])
     cnt, ms, byt = prof[dom]
     achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    kernel_name = {'solve_fwd': 'k_fwd', 'solve_bwd': 'k_bwd', 'gs_project': 'k_atb_tall<16, 2>',
+    kernel_name = {'solve_fwd': 'k_fwd', 'solve_bwd': 'k_bwd', 'gs_project': 'k_atb_mfma16<true>',
                    'gs_update': 'k_combine<16>', 'spmv_M': 'k_spmm'}[dom]
     line = {
         'metric': 'wavelength-points/s at 8k-tri mesh; mixed vs FP64 GPU at <=1e-6 R/T/A error',

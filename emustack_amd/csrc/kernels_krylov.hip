@@ -258,6 +258,91 @@ void launch_swap_i32(int64_t n, int32_t *a, int32_t *b, hipStream_t s) {
 }
 
 
+// The 16-column projection on the matrix cores: op(A)^T B with B of 16
+// columns, as four real v_mfma_f64_16x16x4f64 per 16 x 16 x 4 complex block
+// (real and imaginary parts).  A workgroup stages a 32-row x 64-column tile
+// of A and the 32 x 16 tile of B in LDS (as k_atb_tall); wave w multiplies
+// rows 8w .. 8w + 7 of the tile (two k-steps of 4) into four 16 x 16 output
+// tiles and writes its own partial product (four partials per K chunk, added
+// by k_atb_reduce).  Fragment layouts (FP64 16x16x4): A[i][k] in lane
+// i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i % 4),
+// register i / 4.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <bool CONJ>
+__global__ __launch_bounds__(256) void k_atb_mfma16(int64_t K, int M, int64_t kchunk, const zd *__restrict__ A,
+                                                    int64_t lda, int64_t sA, const zd *__restrict__ B, int64_t ldb,
+                                                    int64_t sB, zd *__restrict__ part) {
+  constexpr int TM = 64, TK = 32, NQ = 16;
+  const int m0 = blockIdx.x * TM;
+  const int chunk = blockIdx.y, nchunk = gridDim.y;
+  const int b = blockIdx.z;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const zd *Ab = A + b * sA;
+  const zd *Bb = B + b * sB;
+  __shared__ zd As[TK][TM + 1];
+  __shared__ zd Bs[TK][NQ + 1];
+  const int64_t kb = chunk * kchunk, ke = min(K, kb + kchunk);
+  const int lr = tid & 31, lc = tid >> 5;  // staging: row lr, columns lc + 8 i
+  zd ra[8], rb[2];
+  // unconditional loads from clamped addresses (see k_atb_tall)
+  auto fetch = [&](int64_t k0) {
+    const int64_t k = k0 + lr;
+    const bool okk = k < ke;
+    const int64_t kc = okk ? k : ke - 1;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int m = m0 + lc + 8 * i;
+      const zd v = Ab[kc + (int64_t)min(m, M - 1) * lda];
+      ra[i] = (okk && m < M) ? v : zmk(0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const zd v = Bb[kc + (int64_t)(lc + 8 * i) * ldb];
+      rb[i] = okk ? v : zmk(0, 0);
+    }
+  };
+  v4d pr[4], pi[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    pr[t] = (v4d){0, 0, 0, 0};
+    pi[t] = (v4d){0, 0, 0, 0};
+  }
+  if (kb < ke) fetch(kb);
+  const int fi = lane & 15, fk = lane >> 4;
+  for (int64_t k0 = kb; k0 < ke; k0 += TK) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) As[lr][lc + 8 * i] = ra[i];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) Bs[lr][lc + 8 * i] = rb[i];
+    __syncthreads();
+    if (k0 + TK < ke) fetch(k0 + TK);
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+      const int kk = wv * 8 + ks * 4 + fk;
+      const zd xb = Bs[kk][fi];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const zd va = As[kk][16 * t + fi];
+        // CONJ: Re += ar br + ai bi, Im += ar bi - ai br; else Re += ar br - ai bi, Im += ar bi + ai br
+        pr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va.x, xb.x, pr[t], 0, 0, 0);
+        pr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(CONJ ? va.y : -va.y, xb.y, pr[t], 0, 0, 0);
+        pi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va.x, xb.y, pi[t], 0, 0, 0);
+        pi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(CONJ ? -va.y : va.y, xb.x, pi[t], 0, 0, 0);
+      }
+    }
+  }
+  const int64_t pc = ((int64_t)b * nchunk + chunk) * 4 + wv;  // this wave's partial
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + 16 * t + fk + 4 * r;
+      if (m < M) part[(pc * NQ + fi) * M + m] = zmk(pr[t][r], pi[t][r]);
+    }
+}
+
 // Gram matrix W^H W of an 8-column block (K x 8, ld ldw), partial sums per K
 // chunk (layout of k_atb_tall's partials).  A 256-row tile of W is staged in
 // LDS; thread (o, g) owns output o = (p, q) and sums rows g, g + 4, ... of the
@@ -362,7 +447,30 @@ void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64
     EMU_LAUNCH_CHECK();
     return;
   }
-  const int cpt = N == 16 ? 2 : 1;
+  if (N == 16) {  // matrix cores (four partials per K chunk, one per wave)
+    const int tiles = (M + 63) / 64;
+    const int64_t per = (int64_t)nb * M * N * 4;
+    const int64_t maxchunk = std::max<int64_t>(1, (int64_t)(scratch_elems / std::max<int64_t>(per, 1)));
+    int64_t nchunk = std::min<int64_t>(std::min<int64_t>(std::max<int64_t>(1, 2048 / std::max(1, tiles * nb)),
+                                                         maxchunk), (K + 31) / 32);
+    nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, 256));
+    int64_t kchunk = (K + nchunk - 1) / nchunk;
+    kchunk = ((kchunk + 31) / 32) * 32;
+    nchunk = (K + kchunk - 1) / kchunk;
+    if ((size_t)(nchunk * per) > scratch_elems) throw HipError("atb: scratch too small");
+    dim3 g((unsigned)tiles, (unsigned)nchunk, nb), t(256);
+    if (conjA)
+      hipExtLaunchKernelGGL(k_atb_mfma16<true>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, scratch);
+    else
+      hipExtLaunchKernelGGL(k_atb_mfma16<false>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, scratch);
+    EMU_LAUNCH_CHECK();
+    const int64_t tot = (int64_t)M * N;
+    hipLaunchKernelGGL(k_atb_reduce, dim3((unsigned)((tot + 63) / 64), nb), dim3(256), 0, s, M, N, (int)(nchunk * 4),
+                       scratch, C, ldc, sC);
+    EMU_LAUNCH_CHECK();
+    return;
+  }
+  const int cpt = 1;
   const int kstep = 32 / cpt;
   const int tiles = (M + 64 * cpt - 1) / (64 * cpt);
   int64_t per = (int64_t)nb * M * N;
@@ -380,8 +488,7 @@ void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64
     case 1: hipExtLaunchKernelGGL(k_atb_tall<1, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
     case 2: hipExtLaunchKernelGGL(k_atb_tall<2, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
     case 4: hipExtLaunchKernelGGL(k_atb_tall<4, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
-    case 8: hipExtLaunchKernelGGL(k_atb_tall<8, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
-    default: hipExtLaunchKernelGGL(k_atb_tall<16, 2>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
+    default: hipExtLaunchKernelGGL(k_atb_tall<8, 1>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, cj, scratch); break;
   }
   EMU_LAUNCH_CHECK();
   const int64_t tot = (int64_t)M * N;
