@@ -432,8 +432,9 @@ void solve_dense(Ctx *c, int nb, int n, const zd *M, const zd *R, int r, zd *X) 
   int32_t *piv = c->buf<int32_t>("ds_piv", (size_t)nb * n);
   int32_t *pfl = c->buf<int32_t>("ds_pfl", (size_t)nb * n);
   int32_t *sig = c->buf<int32_t>("ds_sig", (size_t)nb * n);
+  int32_t *gx = c->buf<int32_t>("ds_gx", (size_t)nb * n * 4);
   zd *inv = c->buf<zd>("ds_inv", (size_t)nb * dm.isz);
-  lu_factor(dm, nb, M, pool, piv, pfl, sig, inv, s);
+  lu_factor(dm, nb, M, pool, piv, pfl, sig, gx, inv, s);
   const zd *rhs = R;
   if (!R) {
     zd *I = c->buf<zd>("ds_eye", (size_t)nb * n * n);
@@ -443,7 +444,7 @@ void solve_dense(Ctx *c, int nb, int n, const zd *M, const zd *R, int r, zd *X) 
   }
   zd *y = c->buf<zd>("ds_y", (size_t)nb * n * r);
   zd *w = c->buf<zd>("ds_w", (size_t)nb * r * solve_ws(dm));
-  lu_solve(dm, nb, r, inv, sig, rhs, y, X, w, s);
+  lu_solve(dm, nb, r, inv, gx, rhs, y, X, w, s);
 }
 
 }  // namespace emu

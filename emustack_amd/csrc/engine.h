@@ -115,9 +115,20 @@ struct Profiler {
   }
 };
 
-// Solve tasks of one level and direction: (front, row0, tiles, k0, slot)
-// entries, partial-sum reductions (front, row0, first slot, count), the
-// number of 64-row partial-sum slots and the column span of split tasks.
+// One solve task, 64 bytes, read by a single scalar load per workgroup: the
+// tile (front t, first row r0, tiles per workgroup tpw; first column kb and
+// partial-sum slot of a split task, else -1) with the front's sizes and
+// offsets, so that no further lookup precedes the workgroup's first loads.
+struct SolveTask {
+  int32_t t, r0, tpw, kb, slot, s, f, fb;
+  int64_t ioff, pivoff, fvoff, uoff;
+};
+static_assert(sizeof(SolveTask) == 64, "SolveTask is 16 words");
+
+// Solve tasks of one level and direction: SolveTask records (off: word
+// offset into the task array, a multiple of 16), partial-sum reductions
+// (front, row0, first slot, count), the number of 64-row partial-sum slots
+// and the column span of split tasks.
 struct SolveTasks {
   int64_t off = 0, cnt = 0;
   int64_t roff = 0, rcnt = 0;
@@ -240,14 +251,15 @@ void launch_permute_vals(DevMesh &dm, int nb, const zd *src, zd *dst, hipStream_
 // ---------------- LU kernels (kernels_lu.hip) ----------------
 // numeric factorisation into pool (piv: LAPACK-style row swaps, pflag: zero
 // pivots perturbed, sig: pivot-order gather), then the inverted-factor form
-// inv [nb][isz] that the solves read
+// inv [nb][isz] and the forward gather records gx [nb][npiv][4] (per pivot:
+// right-hand-side index, the two child update entries) that the solves read
 void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int32_t *pflag, int32_t *sig,
-               zd *inv, hipStream_t s);
+               int32_t *gx, zd *inv, hipStream_t s);
 void lu_invert(DevMesh &dm, int nb, const zd *pool, zd *inv, hipStream_t s);
 // x = A^{-1} b, both in the new (nested-dissection) numbering, [nb][nrhs][n];
 // y (same shape) and w ([nb][nrhs][solve_ws(dm)]) are work vectors
 int64_t solve_ws(const DevMesh &dm);
-void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *sig, const zd *b, zd *y, zd *x,
+void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *gx, const zd *b, zd *y, zd *x,
               zd *w, hipStream_t s, Profiler *prof = nullptr);
 // permute vectors old <-> new numbering: dst[new] = src[perm[new]] or inverse
 void launch_permute_vec(DevMesh &dm, int nb, int ncol, const zd *src, zd *dst, bool to_new,

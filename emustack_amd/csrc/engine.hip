@@ -159,6 +159,20 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
   dm.pmap.upload(sy.pmap, s);
   dm.nnz2front.upload(sy.nnz2front, s);
   dm.max_f = sy.max_f;
+  // inverted-factor form: per front the offset of its forward / backward
+  // blocks (2 f s entries) and of its rows in the child update-vector map
+  std::vector<int64_t> io(nf), fv(nf);
+  {
+    int64_t acc = 0, accf = 0;
+    for (int32_t i = 0; i < nf; ++i) {
+      const FrontInfo &F = sy.fronts[i];
+      io[i] = acc;
+      fv[i] = accf;
+      acc += 2 * (int64_t)F.f * F.s;
+      accf += F.f;
+    }
+    dm.isz = std::max<int64_t>(acc, 1);
+  }
   // launch schedule: deepest level first
   std::vector<int32_t> tasks;
   dm.sched.clear();
@@ -216,6 +230,13 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
     // long serial column loops on a handful of CUs) and 256 elsewhere.  The
     // split variant serves the Krylov solves (at most 8 right-hand sides),
     // the plain one solves with many right-hand sides.
+    auto push_task = [&](int32_t t, int32_t r0, int32_t tpw, int32_t kb, int32_t slot) {
+      const FrontInfo &F = sy.fronts[t];
+      const SolveTask T{t, r0, tpw, kb, slot, F.s, F.f, F.b, io[t], F.pivoff, fv[t], F.uoff};
+      int32_t w[16];
+      std::memcpy(w, &T, sizeof(T));
+      tasks.insert(tasks.end(), w, w + 16);
+    };
     auto solve_tasks = [&](bool fwd, bool split, SolveTasks &st) {
       int64_t unsplit = 0;
       for (int32_t t : LP.fronts) {
@@ -227,6 +248,7 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
       st.span = !split ? INT32_MAX : (unsplit < 256 ? 64 : 256);
       st.nslot = 0;
       std::vector<int32_t> red;
+      while (tasks.size() % 16) tasks.push_back(-1);  // 64-byte aligned records
       st.off = (int64_t)tasks.size();
       for (int32_t t : LP.fronts) {
         const FrontInfo &F = sy.fronts[t];
@@ -237,20 +259,20 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
             const int32_t kb = fwd ? 0 : r0, ke = fwd ? std::min(F.s, r0 + 64) : F.f;
             const int32_t nc = (ke - kb + st.span - 1) / st.span;
             if (nc <= 1) {
-              tasks.insert(tasks.end(), {t, r0, 1, kb, -1});
+              push_task(t, r0, 1, kb, -1);
               continue;
             }
             red.insert(red.end(), {t, r0, st.nslot, nc});
-            for (int32_t c = 0; c < nc; ++c) tasks.insert(tasks.end(), {t, r0, 1, kb + c * st.span, st.nslot + c});
+            for (int32_t c = 0; c < nc; ++c) push_task(t, r0, 1, kb + c * st.span, st.nslot + c);
             st.nslot += nc;
           }
         } else {
           const int32_t T = (R + 63) / 64;
           const int32_t tpw = T >= 4 ? 4 : (T >= 2 ? 2 : 1);
-          for (int32_t r0 = 0; r0 < R; r0 += 64 * tpw) tasks.insert(tasks.end(), {t, r0, tpw, -1, -1});
+          for (int32_t r0 = 0; r0 < R; r0 += 64 * tpw) push_task(t, r0, tpw, -1, -1);
         }
       }
-      st.cnt = ((int64_t)tasks.size() - st.off) / 5;
+      st.cnt = ((int64_t)tasks.size() - st.off) / 16;
       st.roff = (int64_t)tasks.size();
       tasks.insert(tasks.end(), red.begin(), red.end());
       st.rcnt = (int64_t)red.size() / 4;
@@ -276,16 +298,7 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
   }
   // inverted-factor form: offsets, child update-vector map, task lists
   {
-    std::vector<int64_t> io(nf), fv(nf);
-    int64_t acc = 0, accf = 0;
-    for (int32_t i = 0; i < nf; ++i) {
-      const FrontInfo &F = sy.fronts[i];
-      io[i] = acc;
-      fv[i] = accf;
-      acc += 2 * (int64_t)F.f * F.s;
-      accf += F.f;
-    }
-    dm.isz = std::max<int64_t>(acc, 1);
+    const int64_t accf = nf ? fv[nf - 1] + sy.fronts[nf - 1].f : 0;
     std::vector<int32_t> cm(2 * std::max<int64_t>(accf, 1), -1);
     for (int32_t i = 0; i < nf; ++i) {
       const FrontInfo &F = sy.fronts[i];
@@ -418,8 +431,9 @@ void lu_solve_host(Ctx *c, DevMesh &dm, int nb, const double *mat1, int nrhs, co
   int32_t *piv = c->buf<int32_t>("piv", (size_t)nb * sy.npiv);
   int32_t *pfl = c->buf<int32_t>("pflag", (size_t)nb * sy.npiv);
   int32_t *sig = c->buf<int32_t>("sig", (size_t)nb * sy.npiv);
+  int32_t *gx = c->buf<int32_t>("gx", (size_t)nb * sy.npiv * 4);
   zd *inv = c->buf<zd>("inv", (size_t)nb * dm.isz);
-  lu_factor(dm, nb, m1, pool, piv, pfl, sig, inv, c->stream);
+  lu_factor(dm, nb, m1, pool, piv, pfl, sig, gx, inv, c->stream);
   zd *bo = c->buf<zd>("lu_b", (size_t)nb * nrhs * n);
   zd *bn = c->buf<zd>("lu_bn", (size_t)nb * nrhs * n);
   zd *yn = c->buf<zd>("lu_y", (size_t)nb * nrhs * n);
@@ -428,7 +442,7 @@ void lu_solve_host(Ctx *c, DevMesh &dm, int nb, const double *mat1, int nrhs, co
   EMU_HIP(hipMemcpyAsync(bo, b, sizeof(zd) * nb * nrhs * n, hipMemcpyHostToDevice, c->stream));
   EMU_HIP(hipStreamSynchronize(c->stream));
   launch_permute_vec(dm, nb, nrhs, bo, bn, true, c->stream);
-  lu_solve(dm, nb, nrhs, inv, sig, bn, yn, xn, w, c->stream);
+  lu_solve(dm, nb, nrhs, inv, gx, bn, yn, xn, w, c->stream);
   launch_permute_vec(dm, nb, nrhs, xn, bo, false, c->stream);
   EMU_HIP(hipMemcpyAsync(x, bo, sizeof(zd) * nb * nrhs * n, hipMemcpyDeviceToHost, c->stream));
   std::vector<int32_t> pf((size_t)nb * sy.npiv);
@@ -633,7 +647,7 @@ static void block_cholqr2(Ctx *c, int nb, int64_t n, int bq, zd *W, int64_t sV, 
 }
 
 static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int bq, int itermax, double tol,
-                              const zd *inv, const int32_t *sig, const zd *mvals, zd *Xout, StageTimer &tm) {
+                              const zd *inv, const int32_t *gx, const zd *mvals, zd *Xout, StageTimer &tm) {
   hipStream_t s = c->stream;
   const Symbolic &sy = dm.host.sym;
   const int64_t n = dm.host.neq;
@@ -697,12 +711,12 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   std::iota(slot_inst.begin(), slot_inst.end(), 0);
   std::vector<std::pair<int, int>> swaps;
   zd *inv_m = const_cast<zd *>(inv);          // the caller's buffers: slots are swapped during the
-  int32_t *sig_m = const_cast<int32_t *>(sig);  // solve and restored in order before returning
+  int32_t *gx_m = const_cast<int32_t *>(gx);  // solve and restored in order before returning
   zd *mv_m = const_cast<zd *>(mvals);
   const int64_t nnzm = dm.host.nnz;
   auto swap_caller = [&](int a1, int a2) {
     launch_swap(dm.isz, inv_m + a1 * dm.isz, inv_m + a2 * dm.isz, s);
-    launch_swap_i32(sy.npiv, sig_m + (int64_t)a1 * sy.npiv, sig_m + (int64_t)a2 * sy.npiv, s);
+    launch_swap_i32(4 * sy.npiv, gx_m + 4 * a1 * sy.npiv, gx_m + 4 * a2 * sy.npiv, s);
     launch_swap(nnzm, mv_m + a1 * nnzm, mv_m + a2 * nnzm, s);
   };
   auto swap_slots = [&](int a1, int a2) {
@@ -727,7 +741,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
       const KTime kspmv = pf ? pf->time(4, nbd * (20.0 * nzd + 8.0 * (nd + 1) + 32.0 * nd * bq)) : KTime{};
       launch_spmm(nact, n, bq, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, t1, (int64_t)bq * n, s,
                   kspmv);
-      lu_solve(dm, nb, bq, inv, sig, t1, ty, wv, tw, s, pf);
+      lu_solve(dm, nact, bq, inv, gx, t1, ty, wv, tw, s, pf);
       launch_copy(nact, bq * n, wv, (int64_t)bq * n, Wn, sV, s);
       if (delayed) {
         // Xj is the provisional block Z; one pass over V[:, :j] for Z's
@@ -1023,10 +1037,11 @@ static SolveOut factor_solve_sort(Ctx *c, DevMesh &dm, const ModesArgs &a, int n
   int32_t *piv = c->buf<int32_t>("piv", (size_t)nb * sy.npiv);
   o.pflag = c->buf<int32_t>("pflag", (size_t)nb * sy.npiv);
   int32_t *sig = c->buf<int32_t>("sig", (size_t)nb * sy.npiv);
+  int32_t *gx = c->buf<int32_t>("gx", (size_t)nb * sy.npiv * 4);
   zd *inv = c->buf<zd>("inv", (size_t)nb * dm.isz);
-  lu_factor(dm, nb, m1, pool, piv, o.pflag, sig, inv, s);
+  lu_factor(dm, nb, m1, pool, piv, o.pflag, sig, gx, inv, s);
   tm.mark("lu_factor");
-  o.er = krylov_schur(c, dm, nb, nval, m, bq, (int)a.itermax, a.tol, inv, sig, m2p, Xn, tm);
+  o.er = krylov_schur(c, dm, nb, nval, m, bq, (int)a.itermax, a.tol, inv, gx, m2p, Xn, tm);
   launch_permute_vec(dm, nb, nval, Xn, Xo, false, s);
   o.beta.assign((size_t)nb * nval, 0);
   std::vector<int32_t> order((size_t)nb * nval);

@@ -276,8 +276,25 @@ __global__ __launch_bounds__(256) void k_extend_add(const int32_t *__restrict__ 
   }
 }
 
+// Forward gather records, one per pivot: the right-hand-side index of the
+// pivot's row (fb + j, j = sig[k]) and the two child update entries that add
+// into that row (-1: none).  The solves of one factorisation then fetch a
+// front's inputs with one dependent load instead of three.
+__global__ __launch_bounds__(64) void k_gather_index(FrontArrays A, const int32_t *__restrict__ sig, int64_t npiv,
+                                                     int4 *__restrict__ gx) {
+  const int t = blockIdx.x, b = blockIdx.y;
+  const int s = A.s[t], fb = A.b[t];
+  const int64_t po = A.pivoff[t], fvo = A.fvoff[t];
+  const int32_t *sg = sig + b * npiv + po;
+  int4 *g = gx + b * npiv + po;
+  for (int k = threadIdx.x; k < s; k += 64) {
+    const int j = sg[k];
+    g[k] = make_int4(fb + j, A.cmap[2 * (fvo + j)], A.cmap[2 * (fvo + j) + 1], 0);
+  }
+}
+
 void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int32_t *pflag, int32_t *sig,
-               zd *inv, hipStream_t s) {
+               int32_t *gx, zd *inv, hipStream_t s) {
   const Symbolic &sy = dm.host.sym;
   const int64_t psz = sy.pool;
   FrontArrays A = fa(dm);
@@ -308,6 +325,11 @@ void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int3
     EMU_LAUNCH_CHECK();
   }
   lu_invert(dm, nb, pool, inv, s);
+  const int nf = (int)sy.fronts.size();
+  if (nf)
+    hipLaunchKernelGGL(k_gather_index, dim3((unsigned)nf, nb), dim3(64), 0, s, A, sig, sy.npiv,
+                       reinterpret_cast<int4 *>(gx));
+  EMU_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------- inverted-factor form
@@ -320,7 +342,7 @@ void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int3
 // rows) and its backward step is x_S = Bwd * [y_S ; x_U]: dense matrix-vector
 // products that split over row tiles across the whole GPU instead of a
 // column-sequential substitution inside one workgroup.  P is the front's
-// row-pivot permutation, applied as a gather (sig).
+// row-pivot permutation, applied as a gather (the gx records).
 
 constexpr int TB = 32;  // triangular block of the inversion
 
@@ -518,78 +540,90 @@ void lu_invert(DevMesh &dm, int nb, const zd *pool, zd *inv, hipStream_t s) {
 constexpr int KC = 64;  // staged input chunk
 
 template <int RC>
-__global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, FrontArrays A,
-                                             const zd *__restrict__ inv, int64_t isz, const int32_t *__restrict__ sig,
+__global__ __launch_bounds__(256) void k_fwd(const SolveTask *__restrict__ tasks, const int32_t *__restrict__ cmap,
+                                             const zd *__restrict__ inv, int64_t isz, const int4 *__restrict__ gx,
                                              int64_t npiv, int nrhs, int64_t n, int64_t nw,
                                              const zd *__restrict__ bv, zd *__restrict__ y, zd *__restrict__ wbuf,
                                              zd *__restrict__ part, int nslot, int span) {
-  const int32_t *tk = tasks + 5 * blockIdx.x;
-  const int t = tk[0], r0 = tk[1], tpw = tk[2], kb = tk[3], slot = tk[4];
+  const SolveTask T = tasks[blockIdx.x];
   const int b = blockIdx.y;
   const int q0 = blockIdx.z * RC;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int ks = 4 / tpw, tile = wave / ks, kp = wave - tile * ks;
-  const int s = A.s[t], f = A.f[t], fb = A.b[t];
-  const zd *Fw = inv + b * isz + A.ioff[t];
-  const int32_t *sg = sig + b * npiv + A.pivoff[t];
-  const int32_t *cm = A.cmap + 2 * A.fvoff[t];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: column offsets in SGPRs
+  const int ks = 4 / T.tpw, tile = wave / ks, kp = wave - tile * ks;
+  const int s = T.s, f = T.f;
+  const zd *Fw = inv + b * isz + T.ioff;
+  const int4 *gi = gx + b * npiv + T.pivoff;
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
-  const int row = r0 + 64 * tile + lane;
+  const int row0 = T.r0 + 64 * tile;  // first row of this wave's tile
+  const int row = row0 + lane;
+  const int rowc = min(row, f - 1);   // clamped: addresses stay inside the front
   // columns of this task: the workgroup's range, cut to [kb, kb + span)
   // for a split task; lower-triangular rows need k <= row
-  int kbeg = 0, kend = min(s, r0 + 64 * tpw);
-  if (kb >= 0) {
-    kbeg = kb;
-    kend = min(kend, kb + span);
+  int kbeg = 0, kend = min(s, T.r0 + 64 * T.tpw);
+  if (T.kb >= 0) {
+    kbeg = T.kb;
+    kend = min(kend, T.kb + span);
   }
-  const int kend_w = min(kend, min(s, r0 + 64 * (tile + 1)));
+  const int kend_w = row0 < f ? min(kend, min(s, row0 + 64)) : 0;
+  // child update entries of this row (static), added by the epilogue on the
+  // update rows; requested now so that the epilogue waits for one load only
+  const int2 cr = reinterpret_cast<const int2 *>(cmap)[T.fvoff + rowc];
   __shared__ zd xin[KC][RC];
   __shared__ zd red[3][64][RC];
   zd acc[RC];
 #pragma unroll
   for (int q = 0; q < RC; ++q) acc[q] = zmk(0, 0);
-  // All loads below are unconditional from clamped (in-range) addresses with
-  // the unused values zeroed afterwards, so that each thread keeps its loads
-  // in flight together instead of waiting on each one inside a branch.
+  // Load order per chunk: gather records, then this wave's first 8 columns,
+  // then the input values (which depend on the records); the column loads are
+  // thus in flight during the gather, and each later batch of 8 is requested
+  // before the previous one is used.  All loads are unconditional from
+  // clamped addresses, unused values zeroed or discarded afterwards.
   for (int kc = kbeg; kc < kend; kc += KC) {
-    {
-      // input row k of the chunk (pivot gather + child update entries), four
-      // threads per row sharing its RC columns
-      const int k = tid & (KC - 1), qg = tid / KC;
-      const int kk = kc + k;
-      const bool okk = kk < kend;
-      const int j = sg[okk ? kk : kbeg];
-      const int c0i = cm[2 * j], c1i = cm[2 * j + 1];
+    const int k = tid & (KC - 1), qg = tid / KC;
+    const int kk = kc + k;
+    const bool okk = kk < kend;
+    const int4 g = gi[okk ? kk : kbeg];
+    const int kl = min(KC, kend_w - kc);
+    const int nk = kl > kp ? (kl - kp + ks - 1) / ks : 0;  // this wave's columns kp, kp + ks, ...
+    const zd *col = Fw + rowc;
+    const int cbase = kc + kp, clast = kc + kp + max(nk - 1, 0) * ks;
+    zd a[8];
+    if (nk > 0) {
 #pragma unroll
-      for (int u = 0; u < (RC + 3) / 4; ++u) {
-        const int qq = qg + 4 * u;
-        if (qq < RC) {
-          const int64_t cq = col0 + min(qq, nq - 1);
-          zd v = bv[cq * n + fb + j];
-          const zd w0 = wbuf[cq * nw + max(c0i, 0)], w1 = wbuf[cq * nw + max(c1i, 0)];
-          if (c0i >= 0) v = zadd(v, w0);
-          if (c1i >= 0) v = zadd(v, w1);
-          xin[k][qq] = (okk && qq < nq) ? v : zmk(0, 0);
-        }
+      for (int u = 0; u < 8; ++u) a[u] = col[(int64_t)min(cbase + u * ks, clast) * f];
+    }
+#pragma unroll
+    for (int u = 0; u < (RC + 3) / 4; ++u) {
+      const int qq = qg + 4 * u;
+      if (qq < RC) {
+        const int64_t cq = col0 + min(qq, nq - 1);
+        zd v = bv[cq * n + g.x];
+        const zd w0 = wbuf[cq * nw + max(g.y, 0)], w1 = wbuf[cq * nw + max(g.z, 0)];
+        if (g.y >= 0) v = zadd(v, w0);
+        if (g.z >= 0) v = zadd(v, w1);
+        xin[k][qq] = (okk && qq < nq) ? v : zmk(0, 0);
       }
     }
     __syncthreads();
-    const int kl = min(KC, kend_w - kc);
-    if (row < f && kl > 0) {
-      const zd *col = Fw + row + (int64_t)kc * f;
-      const int nk = (kl - kp + ks - 1) / ks;  // this wave's columns: kp, kp + ks, ...
-      for (int t0 = 0; t0 < nk; t0 += 8) {
-        zd a[8];
+    for (int t0 = 0; t0 < nk; t0 += 8) {
+      const bool more = t0 + 8 < nk;
+      zd nx[8];
+      if (more) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] = col[(int64_t)(kp + min(t0 + u, nk - 1) * ks) * f];
+        for (int u = 0; u < 8; ++u) nx[u] = col[(int64_t)min(cbase + (t0 + 8 + u) * ks, clast) * f];
+      }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-          if (t0 + u < nk) {
-            const int k = kp + (t0 + u) * ks;
+      for (int u = 0; u < 8; ++u)
+        if (t0 + u < nk) {
+          const int kx = kp + (t0 + u) * ks;
 #pragma unroll
-            for (int q = 0; q < RC; ++q) acc[q] = zfma(a[u], xin[k][q], acc[q]);
-          }
+          for (int q = 0; q < RC; ++q) acc[q] = zfma(a[u], xin[kx][q], acc[q]);
+        }
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = nx[u];
       }
     }
     __syncthreads();
@@ -605,24 +639,31 @@ __global__ __launch_bounds__(256) void k_fwd(const int32_t *__restrict__ tasks, 
         for (int q = 0; q < RC; ++q) acc[q] = zadd(acc[q], red[wave + j - 1][lane][q]);
   }
   if (kp != 0) return;
-  if (slot >= 0) {  // split task: partial sums of the tile, added by k_fwd_red
+  if (T.slot >= 0) {  // split task: partial sums of the tile, added by k_fwd_red
 #pragma unroll
     for (int q = 0; q < RC; ++q)
-      if (q < nq) part[((col0 + q) * nslot + slot) * 64 + lane] = acc[q];
+      if (q < nq) part[((col0 + q) * nslot + T.slot) * 64 + lane] = acc[q];
     return;
   }
   if (row < f) {
+    if (row < s) {
 #pragma unroll
-    for (int q = 0; q < RC; ++q) {
-      if (q >= nq) break;
-      zd v = acc[q];
-      if (row < s) {
-        y[(col0 + q) * n + fb + row] = v;
-      } else {
-        const int c0i = cm[2 * row], c1i = cm[2 * row + 1];
-        if (c0i >= 0) v = zadd(v, wbuf[(col0 + q) * nw + c0i]);
-        if (c1i >= 0) v = zadd(v, wbuf[(col0 + q) * nw + c1i]);
-        wbuf[(col0 + q) * nw + A.uoff[t] + (row - s)] = v;
+      for (int q = 0; q < RC; ++q)
+        if (q < nq) y[(col0 + q) * n + T.fb + row] = acc[q];
+    } else {
+      zd w0[RC], w1[RC];
+#pragma unroll
+      for (int q = 0; q < RC; ++q) {
+        const int64_t cq = col0 + min(q, nq - 1);
+        w0[q] = wbuf[cq * nw + max(cr.x, 0)];
+        w1[q] = wbuf[cq * nw + max(cr.y, 0)];
+      }
+#pragma unroll
+      for (int q = 0; q < RC; ++q) {
+        zd v = acc[q];
+        if (cr.x >= 0) v = zadd(v, w0[q]);
+        if (cr.y >= 0) v = zadd(v, w1[q]);
+        if (q < nq) wbuf[(col0 + q) * nw + T.uoff + (row - s)] = v;
       }
     }
   }
@@ -659,70 +700,81 @@ __global__ __launch_bounds__(64) void k_fwd_red(const int32_t *__restrict__ task
 }
 
 template <int RC>
-__global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, FrontArrays A,
+__global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks, const int32_t *__restrict__ uidx,
                                              const zd *__restrict__ inv, int64_t isz, int nrhs, int64_t n,
                                              const zd *__restrict__ y, zd *__restrict__ x, zd *__restrict__ part,
                                              int nslot, int span) {
-  const int32_t *tk = tasks + 5 * blockIdx.x;
-  const int t = tk[0], r0 = tk[1], tpw = tk[2], kb = tk[3], slot = tk[4];
+  const SolveTask T = tasks[blockIdx.x];
   const int b = blockIdx.y;
   const int q0 = blockIdx.z * RC;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int ks = 4 / tpw, tile = wave / ks, kp = wave - tile * ks;
-  const int s = A.s[t], f = A.f[t], fb = A.b[t];
-  const zd *Bw = inv + b * isz + A.ioff[t] + (int64_t)f * s;
-  const int32_t *ui = A.uidx + A.uoff[t];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: column offsets in SGPRs
+  const int ks = 4 / T.tpw, tile = wave / ks, kp = wave - tile * ks;
+  const int s = T.s, f = T.f;
+  const zd *Bw = inv + b * isz + T.ioff + (int64_t)f * s;
+  const int32_t *ui = uidx + T.uoff;
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
-  const int row = r0 + 64 * tile;
+  const int row = T.r0 + 64 * tile;  // first row of this wave's tile
   const int rowl = row + lane;
+  const int rowc = min(rowl, s - 1);
   // columns: [r0, f) for the workgroup (upper-triangular rows need k >= row),
   // cut to [kb, kb + span) for a split task
-  int kbeg = r0, kend = f;
-  if (kb >= 0) {
-    kbeg = kb;
-    kend = min(f, kb + span);
+  int kbeg = T.r0, kend = f;
+  if (T.kb >= 0) {
+    kbeg = T.kb;
+    kend = min(f, T.kb + span);
   }
   __shared__ zd xin[KC][RC];
   __shared__ zd red[3][64][RC];
   zd acc[RC];
 #pragma unroll
   for (int q = 0; q < RC; ++q) acc[q] = zmk(0, 0);
+  // load order as in k_fwd: input index, first 8 columns, input values
   for (int kc = kbeg; kc < kend; kc += KC) {
-    {
-      // input row k of the chunk: y of this front or x of an ancestor
-      const int k = tid & (KC - 1), qg = tid / KC;
-      const int kk = kc + k;
-      const bool okk = kk < kend;
-      const int kkc = okk ? kk : kbeg;
-      const zd *src = kkc < s ? y : x;
-      const int64_t off = kkc < s ? (int64_t)fb + kkc : (int64_t)ui[kkc - s];
+    const int k = tid & (KC - 1), qg = tid / KC;
+    const int kk = kc + k;
+    const bool okk = kk < kend;
+    const int kkc = okk ? kk : kbeg;
+    // input row kkc: y of this front or x of an ancestor
+    const int64_t off = kkc < s ? (int64_t)T.fb + kkc : (int64_t)ui[kkc - s];
+    const int klo = max(0, row - kc), khi = min(KC, kend - kc);
+    const int kst = klo + kp;
+    const int nk = (row < s && khi > kst) ? (khi - kst + ks - 1) / ks : 0;
+    const zd *col = Bw + rowc;
+    const int cbase = kc + kst, clast = kc + kst + max(nk - 1, 0) * ks;
+    zd a[8];
+    if (nk > 0) {
 #pragma unroll
-      for (int u = 0; u < (RC + 3) / 4; ++u) {
-        const int qq = qg + 4 * u;
-        if (qq < RC) {
-          const zd v = src[(col0 + min(qq, nq - 1)) * n + off];
-          xin[k][qq] = (okk && qq < nq) ? v : zmk(0, 0);
-        }
+      for (int u = 0; u < 8; ++u) a[u] = col[(int64_t)min(cbase + u * ks, clast) * s];
+    }
+    const zd *src = kkc < s ? y : x;
+#pragma unroll
+    for (int u = 0; u < (RC + 3) / 4; ++u) {
+      const int qq = qg + 4 * u;
+      if (qq < RC) {
+        const zd v = src[(col0 + min(qq, nq - 1)) * n + off];
+        xin[k][qq] = (okk && qq < nq) ? v : zmk(0, 0);
       }
     }
     __syncthreads();
-    const int klo = max(0, row - kc), khi = min(KC, kend - kc);
-    if (rowl < s && klo < khi) {
-      const zd *col = Bw + rowl + (int64_t)kc * s;
-      const int kst = klo + kp;
-      const int nk = (khi - kst + ks - 1) / ks;
-      for (int t0 = 0; t0 < nk; t0 += 8) {
-        zd a[8];
+    for (int t0 = 0; t0 < nk; t0 += 8) {
+      const bool more = t0 + 8 < nk;
+      zd nx[8];
+      if (more) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] = col[(int64_t)(kst + min(t0 + u, nk - 1) * ks) * s];
+        for (int u = 0; u < 8; ++u) nx[u] = col[(int64_t)min(cbase + (t0 + 8 + u) * ks, clast) * s];
+      }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-          if (t0 + u < nk) {
-            const int k = kst + (t0 + u) * ks;
+      for (int u = 0; u < 8; ++u)
+        if (t0 + u < nk) {
+          const int kx = kst + (t0 + u) * ks;
 #pragma unroll
-            for (int q = 0; q < RC; ++q) acc[q] = zfma(a[u], xin[k][q], acc[q]);
-          }
+          for (int q = 0; q < RC; ++q) acc[q] = zfma(a[u], xin[kx][q], acc[q]);
+        }
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = nx[u];
       }
     }
     __syncthreads();
@@ -738,16 +790,16 @@ __global__ __launch_bounds__(256) void k_bwd(const int32_t *__restrict__ tasks, 
         for (int q = 0; q < RC; ++q) acc[q] = zadd(acc[q], red[wave + j - 1][lane][q]);
   }
   if (kp != 0) return;
-  if (slot >= 0) {  // split task: partial sums, added by k_bwd_red
+  if (T.slot >= 0) {  // split task: partial sums, added by k_bwd_red
 #pragma unroll
     for (int q = 0; q < RC; ++q)
-      if (q < nq) part[((col0 + q) * nslot + slot) * 64 + lane] = acc[q];
+      if (q < nq) part[((col0 + q) * nslot + T.slot) * 64 + lane] = acc[q];
     return;
   }
   if (rowl < s)
 #pragma unroll
     for (int q = 0; q < RC; ++q)
-      if (q < nq) x[(col0 + q) * n + fb + rowl] = acc[q];
+      if (q < nq) x[(col0 + q) * n + T.fb + rowl] = acc[q];
 }
 
 template <int RC>
@@ -782,12 +834,13 @@ static int pick_rc(int nrhs, int64_t tasks, int nb) {
 
 template <int RC>
 static void launch_fwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *inv,
-                       const int32_t *sig, const zd *bnew, zd *y, zd *w, zd *part, hipStream_t s, KTime kt) {
+                       const int32_t *gx, const zd *bnew, zd *y, zd *w, zd *part, hipStream_t s, KTime kt) {
   const Symbolic &sy = dm.host.sym;
   const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
   const int64_t nw = std::max<int64_t>(sy.nw, 1);
   hipExtLaunchKernelGGL(k_fwd<RC>, dim3((unsigned)T.cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
-                        dm.tasks.p + T.off, A, inv, dm.isz, sig, sy.npiv, nrhs, sy.n, nw, bnew, y, w, part,
+                        reinterpret_cast<const SolveTask *>(dm.tasks.p + T.off), dm.cmap.p, inv, dm.isz,
+                        reinterpret_cast<const int4 *>(gx), sy.npiv, nrhs, sy.n, nw, bnew, y, w, part,
                         (int)T.nslot, (int)std::min<int64_t>(T.span, INT32_MAX));
   if (T.rcnt > 0)
     hipLaunchKernelGGL(k_fwd_red<RC>, dim3((unsigned)T.rcnt, nb, nz), dim3(64), 0, s, dm.tasks.p + T.roff, A, nrhs,
@@ -800,8 +853,8 @@ static void launch_bwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, 
   const Symbolic &sy = dm.host.sym;
   const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
   hipExtLaunchKernelGGL(k_bwd<RC>, dim3((unsigned)T.cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
-                        dm.tasks.p + T.off, A, inv, dm.isz, nrhs, sy.n, y, x, part, (int)T.nslot,
-                        (int)std::min<int64_t>(T.span, INT32_MAX));
+                        reinterpret_cast<const SolveTask *>(dm.tasks.p + T.off), dm.uidx.p, inv, dm.isz, nrhs,
+                        sy.n, y, x, part, (int)T.nslot, (int)std::min<int64_t>(T.span, INT32_MAX));
   if (T.rcnt > 0)
     hipLaunchKernelGGL(k_bwd_red<RC>, dim3((unsigned)T.rcnt, nb, nz), dim3(64), 0, s, dm.tasks.p + T.roff, A, nrhs,
                        sy.n, part, (int)T.nslot, x);
@@ -816,7 +869,7 @@ int64_t solve_ws(const DevMesh &dm) { return std::max<int64_t>(dm.host.sym.nw, 1
 // traffic.
 static const SolveTasks &pick_tasks(const SolveTasks *v, int nrhs) { return nrhs <= 8 ? v[1] : v[0]; }
 
-void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *sig, const zd *bnew, zd *y, zd *x,
+void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *gx, const zd *bnew, zd *y, zd *x,
               zd *w, hipStream_t s, Profiler *prof) {
   FrontArrays A = fa(dm);
   const double vecs = (double)nb * nrhs;
@@ -827,10 +880,10 @@ void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *sig, 
     const int rc = pick_rc(nrhs, T.cnt, nb);
     KTime kt = prof ? prof->time(0, L.fwd_mat * nb + L.fwd_vec * vecs) : KTime{};
     switch (rc) {
-      case 8: launch_fwd<8>(T, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
-      case 4: launch_fwd<4>(T, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
-      case 2: launch_fwd<2>(T, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
-      default: launch_fwd<1>(T, dm, A, nb, nrhs, inv, sig, bnew, y, w, part, s, kt); break;
+      case 8: launch_fwd<8>(T, dm, A, nb, nrhs, inv, gx, bnew, y, w, part, s, kt); break;
+      case 4: launch_fwd<4>(T, dm, A, nb, nrhs, inv, gx, bnew, y, w, part, s, kt); break;
+      case 2: launch_fwd<2>(T, dm, A, nb, nrhs, inv, gx, bnew, y, w, part, s, kt); break;
+      default: launch_fwd<1>(T, dm, A, nb, nrhs, inv, gx, bnew, y, w, part, s, kt); break;
     }
   }
   EMU_LAUNCH_CHECK();
