@@ -835,7 +835,14 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
     }
     if (nact == 0) break;
     if (last) break;
-    // restart: V2[:, :keep] = V[:, :m] Q[:, :keep]; V2[:, keep:keep+bq] = V[:, m:m+bq]
+    // restart: V2[:, :keep] = V[:, :m] Q[:, :keep]; V2[:, keep:keep+bq] = V[:, m:m+bq].
+    // Every instance's Q is the identity on its first min(nlock, nev)
+    // columns (the Schur step and its reordering act on the rest only), so
+    // with a0 the smallest such count over the batch, V2[:, :a0] = V[:, :a0]
+    // is a copy and the product runs over the trailing (m - a0) x (keep - a0)
+    // block: the same values with fewer flops.
+    int a0 = keep;
+    for (int b = 0; b < nact; ++b) a0 = std::min(a0, std::min(st[b].nlock, nev));
     std::vector<zd> Qh((size_t)nact * m * keep);
     std::vector<zd> Hn((size_t)nact * sH, zmk(0, 0));
     for (int b = 0; b < nact; ++b) {
@@ -862,7 +869,9 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
     EMU_HIP(hipMemcpyAsync(Qd, Qh.data(), sizeof(zd) * Qh.size(), hipMemcpyHostToDevice, s));
     EMU_HIP(hipMemcpyAsync(Hd, Hn.data(), sizeof(zd) * Hn.size(), hipMemcpyHostToDevice, s));
     EMU_HIP(hipStreamSynchronize(s));
-    launch_gemm_nn(nact, n, keep, m, zmk(1, 0), V, n, sV, Qd, m, (int64_t)m * keep, zmk(0, 0), V2, n, sV, s);
+    if (a0 > 0) launch_copy(nact, (int64_t)a0 * n, V, sV, V2, sV, s);
+    launch_gemm_nn(nact, n, keep - a0, m - a0, zmk(1, 0), V + (int64_t)a0 * n, n, sV, Qd + a0 + (int64_t)a0 * m, m,
+                   (int64_t)m * keep, zmk(0, 0), V2 + (int64_t)a0 * n, n, sV, s);
     launch_copy(nact, (int64_t)bq * n, V + (int64_t)m * n, sV, V2 + (int64_t)keep * n, sV, s);
     std::swap(V, V2);
     k = keep;

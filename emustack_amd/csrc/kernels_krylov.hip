@@ -16,7 +16,12 @@ namespace emu {
 // value / index reads, one read of the row per column block), accumulates
 // the SM_C column sums, and the eight partial sums are combined with
 // cross-lane shuffles; lane j then writes column j.  A wavefront covers 8
-// rows, a workgroup 32.
+// rows, a workgroup 32.  Workgroups are dealt to the 8 XCDs round-robin by
+// launch index, so the kernel maps launch index p to row block
+// (p % 8) * per + p / 8: each XCD works through one contiguous eighth of the
+// rows and, the rows being in nested-dissection order, gathers x from a
+// compact range that stays in its own L2 instead of every L2 fetching all of
+// x.
 constexpr int SM_C = 8;
 
 __device__ __forceinline__ zd shfl_xor_z(zd v, int m) {
@@ -26,9 +31,10 @@ __device__ __forceinline__ zd shfl_xor_z(zd v, int m) {
 __global__ __launch_bounds__(256) void k_spmm(int64_t n, int ncol, const int64_t *__restrict__ rp,
                                               const int32_t *__restrict__ ci, const zd *__restrict__ vals,
                                               int64_t vstride, const zd *__restrict__ x, int64_t xstride,
-                                              zd *__restrict__ y, int64_t ystride) {
+                                              zd *__restrict__ y, int64_t ystride, int per) {
   const int lane8 = threadIdx.x & 7;
-  const int64_t r = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 3);
+  const int64_t blk = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int64_t r = blk * 32 + (threadIdx.x >> 3);
   const int b = blockIdx.y;
   const int c0 = blockIdx.z * SM_C;
   const int nc = min(SM_C, ncol - c0);
@@ -66,9 +72,11 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, int ncol, const int64_t
 
 void launch_spmm(int nb, int64_t n, int ncol, const int64_t *rp, const int32_t *ci, const zd *vals,
                  int64_t vstride, const zd *x, int64_t xstride, zd *y, int64_t ystride, hipStream_t s, KTime kt) {
-  dim3 g((unsigned)((n + 31) / 32), nb, (unsigned)((ncol + SM_C - 1) / SM_C));
+  // 8 * per launch indices cover the (n + 31) / 32 row blocks; rows >= n idle
+  const int per = (int)(((n + 31) / 32 + 7) / 8);
+  dim3 g((unsigned)(8 * per), nb, (unsigned)((ncol + SM_C - 1) / SM_C));
   hipExtLaunchKernelGGL(k_spmm, g, dim3(256), 0, s, kt.a, kt.b, 0, n, ncol, rp, ci, vals, vstride, x, xstride, y,
-                        ystride);
+                        ystride, per);
   EMU_LAUNCH_CHECK();
 }
 
