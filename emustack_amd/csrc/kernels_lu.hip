@@ -670,31 +670,47 @@ __global__ __launch_bounds__(256) void k_fwd(const SolveTask *__restrict__ tasks
 }
 
 // Sum of the partial sums of a split forward tile (task: front, r0, first
-// slot, count; one lane per row), then the forward epilogue of k_fwd.
+// slot, count; one lane per row, the four waves take every fourth
+// right-hand side), then the forward epilogue of k_fwd.  The slot loads of a
+// right-hand side are issued eight at a time from clamped slots and added in
+// slot order.
 template <int RC>
-__global__ __launch_bounds__(64) void k_fwd_red(const int32_t *__restrict__ tasks, FrontArrays A, int nrhs,
-                                                int64_t n, int64_t nw, const zd *__restrict__ part, int nslot,
-                                                zd *__restrict__ y, zd *__restrict__ wbuf) {
+__global__ __launch_bounds__(256) void k_fwd_red(const int32_t *__restrict__ tasks, FrontArrays A, int nrhs,
+                                                 int64_t n, int64_t nw, const zd *__restrict__ part, int nslot,
+                                                 zd *__restrict__ y, zd *__restrict__ wbuf) {
   const int32_t *tk = tasks + 4 * blockIdx.x;
   const int t = tk[0], r0 = tk[1], p0 = tk[2], np = tk[3];
-  const int b = blockIdx.y, q0 = blockIdx.z * RC, lane = threadIdx.x;
+  const int b = blockIdx.y, q0 = blockIdx.z * RC;
+  const int lane = threadIdx.x & 63, qg = threadIdx.x >> 6;
   const int s = A.s[t], f = A.f[t], fb = A.b[t];
   const int32_t *cm = A.cmap + 2 * A.fvoff[t];
+  const int64_t uo = A.uoff[t];
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
   const int row = r0 + lane;
   if (row >= f) return;
-  for (int q = 0; q < nq; ++q) {
+  const int c0i = row >= s ? cm[2 * row] : -1, c1i = row >= s ? cm[2 * row + 1] : -1;
+#pragma unroll
+  for (int u = 0; u < (RC + 3) / 4; ++u) {
+    const int q = qg + 4 * u;
+    if (q >= nq) break;
     const zd *pp = part + ((col0 + q) * nslot + p0) * 64 + lane;
-    zd v = pp[0];
-    for (int c = 1; c < np; ++c) v = zadd(v, pp[c * 64]);
+    zd v = zmk(0, 0);
+    for (int c = 0; c < np; c += 8) {
+      zd h[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) h[i] = pp[(int64_t)min(c + i, np - 1) * 64];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (c + i < np) v = zadd(v, h[i]);
+    }
     if (row < s) {
       y[(col0 + q) * n + fb + row] = v;
     } else {
-      const int c0i = cm[2 * row], c1i = cm[2 * row + 1];
-      if (c0i >= 0) v = zadd(v, wbuf[(col0 + q) * nw + c0i]);
-      if (c1i >= 0) v = zadd(v, wbuf[(col0 + q) * nw + c1i]);
-      wbuf[(col0 + q) * nw + A.uoff[t] + (row - s)] = v;
+      const zd w0 = wbuf[(col0 + q) * nw + max(c0i, 0)], w1 = wbuf[(col0 + q) * nw + max(c1i, 0)];
+      if (c0i >= 0) v = zadd(v, w0);
+      if (c1i >= 0) v = zadd(v, w1);
+      wbuf[(col0 + q) * nw + uo + (row - s)] = v;
     }
   }
 }
@@ -803,21 +819,32 @@ __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks
 }
 
 template <int RC>
-__global__ __launch_bounds__(64) void k_bwd_red(const int32_t *__restrict__ tasks, FrontArrays A, int nrhs,
-                                                int64_t n, const zd *__restrict__ part, int nslot,
-                                                zd *__restrict__ x) {
+__global__ __launch_bounds__(256) void k_bwd_red(const int32_t *__restrict__ tasks, FrontArrays A, int nrhs,
+                                                 int64_t n, const zd *__restrict__ part, int nslot,
+                                                 zd *__restrict__ x) {
   const int32_t *tk = tasks + 4 * blockIdx.x;
   const int t = tk[0], r0 = tk[1], p0 = tk[2], np = tk[3];
-  const int b = blockIdx.y, q0 = blockIdx.z * RC, lane = threadIdx.x;
+  const int b = blockIdx.y, q0 = blockIdx.z * RC;
+  const int lane = threadIdx.x & 63, qg = threadIdx.x >> 6;
   const int s = A.s[t], fb = A.b[t];
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
   const int row = r0 + lane;
   if (row >= s) return;
-  for (int q = 0; q < nq; ++q) {
+#pragma unroll
+  for (int u = 0; u < (RC + 3) / 4; ++u) {
+    const int q = qg + 4 * u;
+    if (q >= nq) break;
     const zd *pp = part + ((col0 + q) * nslot + p0) * 64 + lane;
-    zd v = pp[0];
-    for (int c = 1; c < np; ++c) v = zadd(v, pp[c * 64]);
+    zd v = zmk(0, 0);
+    for (int c = 0; c < np; c += 8) {
+      zd h[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) h[i] = pp[(int64_t)min(c + i, np - 1) * 64];
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (c + i < np) v = zadd(v, h[i]);
+    }
     x[(col0 + q) * n + fb + row] = v;
   }
 }
@@ -843,7 +870,7 @@ static void launch_fwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, 
                         reinterpret_cast<const int4 *>(gx), sy.npiv, nrhs, sy.n, nw, bnew, y, w, part,
                         (int)T.nslot, (int)std::min<int64_t>(T.span, INT32_MAX));
   if (T.rcnt > 0)
-    hipLaunchKernelGGL(k_fwd_red<RC>, dim3((unsigned)T.rcnt, nb, nz), dim3(64), 0, s, dm.tasks.p + T.roff, A, nrhs,
+    hipLaunchKernelGGL(k_fwd_red<RC>, dim3((unsigned)T.rcnt, nb, nz), dim3(256), 0, s, dm.tasks.p + T.roff, A, nrhs,
                        sy.n, nw, part, (int)T.nslot, y, w);
 }
 
@@ -856,7 +883,7 @@ static void launch_bwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, 
                         reinterpret_cast<const SolveTask *>(dm.tasks.p + T.off), dm.uidx.p, inv, dm.isz, nrhs,
                         sy.n, y, x, part, (int)T.nslot, (int)std::min<int64_t>(T.span, INT32_MAX));
   if (T.rcnt > 0)
-    hipLaunchKernelGGL(k_bwd_red<RC>, dim3((unsigned)T.rcnt, nb, nz), dim3(64), 0, s, dm.tasks.p + T.roff, A, nrhs,
+    hipLaunchKernelGGL(k_bwd_red<RC>, dim3((unsigned)T.rcnt, nb, nz), dim3(256), 0, s, dm.tasks.p + T.roff, A, nrhs,
                        sy.n, part, (int)T.nslot, x);
 }
 
