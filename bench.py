@@ -159,7 +159,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=2)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--batch', type=int, default=8, help='wavelength points per step')
+    ap.add_argument('--batch', type=int, default=16, help='wavelength points per step')
     ap.add_argument('--groups', type=int, default=2,
                     help='concurrent sub-batches (host threads, each with its own context and stream)')
     ap.add_argument('--ncv', type=int, default=384,

@@ -48,11 +48,14 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, int ncol, const int64_t
     // the column loads are unconditional (clamped column index, unused
     // columns multiply zero) so that all of them are in flight together
     for (int64_t k = k0 + lane8; k < k1; k += 8) {
-      const zd a = v[k];
-      const int64_t col = ci[k];
+      const zd a = ld_stream(v + k);
+      const int64_t col = ld_stream(ci + k);
       zd xv[SM_C];
 #pragma unroll
       for (int c = 0; c < SM_C; ++c) xv[c] = xb[(int64_t)min(c, nc - 1) * n + col];
+      // keep the scheduler from interleaving each gather with its use (it
+      // otherwise reuses one register quad and waits for every load in turn)
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int c = 0; c < SM_C; ++c) acc[c] = zfma(c < nc ? a : zmk(0, 0), xv[c], acc[c]);
     }

@@ -559,6 +559,7 @@ __global__ __launch_bounds__(256) void k_fwd(const SolveTask *__restrict__ tasks
   const int row0 = T.r0 + 64 * tile;  // first row of this wave's tile
   const int row = row0 + lane;
   const int rowc = min(row, f - 1);   // clamped: addresses stay inside the front
+  const uint32_t rowb = (uint32_t)rowc * 16u;  // byte offset of the row in a column
   // columns of this task: the workgroup's range, cut to [kb, kb + span)
   // for a split task; lower-triangular rows need k <= row
   int kbeg = 0, kend = min(s, T.r0 + 64 * T.tpw);
@@ -587,45 +588,57 @@ __global__ __launch_bounds__(256) void k_fwd(const SolveTask *__restrict__ tasks
     const int4 g = gi[okk ? kk : kbeg];
     const int kl = min(KC, kend_w - kc);
     const int nk = kl > kp ? (kl - kp + ks - 1) / ks : 0;  // this wave's columns kp, kp + ks, ...
-    const zd *col = Fw + rowc;
-    const int cbase = kc + kp, clast = kc + kp + max(nk - 1, 0) * ks;
+    // column c of the wave's rows: byte c f 16 + rowb of the front's block
+    const __amdgpu_buffer_rsrc_t Fcol = buf_view(Fw, (uint32_t)f * (uint32_t)s * 16u);
+    // column indices clamped into [0, s): an idle wave (nk = 0) reads a
+    // valid column and discards it, so the loads need no branch (a branch
+    // would make the gather below wait for them)
+    const int cbase = kc + kp, clast = min(kc + kp + max(nk - 1, 0) * ks, s - 1);
     zd a[8];
-    if (nk > 0) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] = col[(int64_t)min(cbase + u * ks, clast) * f];
+    for (int u = 0; u < 8; ++u) a[u] = buf_ld(Fcol, rowb, (uint32_t)(min(cbase + u * ks, clast) * f) * 16u);
+    constexpr int NG = (RC + 3) / 4;  // right-hand sides gathered per thread
+    zd gv[NG], g0[NG], g1[NG];
+#pragma unroll
+    for (int u = 0; u < NG; ++u) {
+      const int64_t cq = col0 + min(min(qg + 4 * u, RC - 1), nq - 1);
+      gv[u] = bv[cq * n + g.x];
+      g0[u] = wbuf[cq * nw + max(g.y, 0)];
+      g1[u] = wbuf[cq * nw + max(g.z, 0)];
     }
 #pragma unroll
-    for (int u = 0; u < (RC + 3) / 4; ++u) {
+    for (int u = 0; u < NG; ++u) {
       const int qq = qg + 4 * u;
-      if (qq < RC) {
-        const int64_t cq = col0 + min(qq, nq - 1);
-        zd v = bv[cq * n + g.x];
-        const zd w0 = wbuf[cq * nw + max(g.y, 0)], w1 = wbuf[cq * nw + max(g.z, 0)];
-        if (g.y >= 0) v = zadd(v, w0);
-        if (g.z >= 0) v = zadd(v, w1);
-        xin[k][qq] = (okk && qq < nq) ? v : zmk(0, 0);
-      }
+      // selects, not branches: a branch would let the compiler sink the
+      // load into it and wait there for every load in flight
+      const zd v = zadd(zadd(gv[u], g.y >= 0 ? g0[u] : zmk(0, 0)), g.z >= 0 ? g1[u] : zmk(0, 0));
+      if (qq < RC) xin[k][qq] = (okk && qq < nq) ? v : zmk(0, 0);
     }
     __syncthreads();
-    for (int t0 = 0; t0 < nk; t0 += 8) {
-      const bool more = t0 + 8 < nk;
+    // full batches: the next batch is requested before this one is used (no
+    // branch around the loads: a branch join would wait for them at once)
+    int t0 = 0;
+    for (; t0 + 8 < nk; t0 += 8) {
       zd nx[8];
-      if (more) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) nx[u] = col[(int64_t)min(cbase + (t0 + 8 + u) * ks, clast) * f];
+      for (int u = 0; u < 8; ++u) nx[u] = buf_ld(Fcol, rowb, (uint32_t)(min(cbase + (t0 + 8 + u) * ks, clast) * f) * 16u);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int kx = kp + (t0 + u) * ks;
+#pragma unroll
+        for (int q = 0; q < RC; ++q) acc[q] = zfma(a[u], xin[kx][q], acc[q]);
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (t0 + u < nk) {
-          const int kx = kp + (t0 + u) * ks;
-#pragma unroll
-          for (int q = 0; q < RC; ++q) acc[q] = zfma(a[u], xin[kx][q], acc[q]);
-        }
-      if (more) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] = nx[u];
-      }
+      for (int u = 0; u < 8; ++u) a[u] = nx[u];
     }
+    // the last, possibly partial, batch
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (t0 + u < nk) {
+        const int kx = kp + (t0 + u) * ks;
+#pragma unroll
+        for (int q = 0; q < RC; ++q) acc[q] = zfma(a[u], xin[kx][q], acc[q]);
+      }
     __syncthreads();
   }
   if (ks > 1) {
@@ -660,9 +673,7 @@ __global__ __launch_bounds__(256) void k_fwd(const SolveTask *__restrict__ tasks
       }
 #pragma unroll
       for (int q = 0; q < RC; ++q) {
-        zd v = acc[q];
-        if (cr.x >= 0) v = zadd(v, w0[q]);
-        if (cr.y >= 0) v = zadd(v, w1[q]);
+        const zd v = zadd(zadd(acc[q], cr.x >= 0 ? w0[q] : zmk(0, 0)), cr.y >= 0 ? w1[q] : zmk(0, 0));
         if (q < nq) wbuf[(col0 + q) * nw + T.uoff + (row - s)] = v;
       }
     }
@@ -734,6 +745,7 @@ __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks
   const int row = T.r0 + 64 * tile;  // first row of this wave's tile
   const int rowl = row + lane;
   const int rowc = min(rowl, s - 1);
+  const uint32_t rowb = (uint32_t)max(rowc, 0) * 16u;  // byte offset of the row in a column
   // columns: [r0, f) for the workgroup (upper-triangular rows need k >= row),
   // cut to [kb, kb + span) for a split task
   int kbeg = T.r0, kend = f;
@@ -757,42 +769,48 @@ __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks
     const int klo = max(0, row - kc), khi = min(KC, kend - kc);
     const int kst = klo + kp;
     const int nk = (row < s && khi > kst) ? (khi - kst + ks - 1) / ks : 0;
-    const zd *col = Bw + rowc;
-    const int cbase = kc + kst, clast = kc + kst + max(nk - 1, 0) * ks;
+    // column c of the wave's rows: byte c s 16 + rowb of the front's block
+    const __amdgpu_buffer_rsrc_t Fcol = buf_view(Bw, (uint32_t)f * (uint32_t)s * 16u);
+    // column indices clamped into [0, f) (see k_fwd)
+    const int cbase = kc + kst, clast = min(kc + kst + max(nk - 1, 0) * ks, f - 1);
     zd a[8];
-    if (nk > 0) {
 #pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] = col[(int64_t)min(cbase + u * ks, clast) * s];
-    }
+    for (int u = 0; u < 8; ++u) a[u] = buf_ld(Fcol, rowb, (uint32_t)(min(cbase + u * ks, clast) * s) * 16u);
     const zd *src = kkc < s ? y : x;
+    constexpr int NG = (RC + 3) / 4;
+    zd gv[NG];
 #pragma unroll
-    for (int u = 0; u < (RC + 3) / 4; ++u) {
+    for (int u = 0; u < NG; ++u) gv[u] = src[(col0 + min(min(qg + 4 * u, RC - 1), nq - 1)) * n + off];
+#pragma unroll
+    for (int u = 0; u < NG; ++u) {
       const int qq = qg + 4 * u;
-      if (qq < RC) {
-        const zd v = src[(col0 + min(qq, nq - 1)) * n + off];
-        xin[k][qq] = (okk && qq < nq) ? v : zmk(0, 0);
-      }
+      if (qq < RC) xin[k][qq] = (okk && qq < nq) ? gv[u] : zmk(0, 0);
     }
     __syncthreads();
-    for (int t0 = 0; t0 < nk; t0 += 8) {
-      const bool more = t0 + 8 < nk;
+    // full batches: the next batch is requested before this one is used (no
+    // branch around the loads: a branch join would wait for them at once)
+    int t0 = 0;
+    for (; t0 + 8 < nk; t0 += 8) {
       zd nx[8];
-      if (more) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) nx[u] = col[(int64_t)min(cbase + (t0 + 8 + u) * ks, clast) * s];
+      for (int u = 0; u < 8; ++u) nx[u] = buf_ld(Fcol, rowb, (uint32_t)(min(cbase + (t0 + 8 + u) * ks, clast) * s) * 16u);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int kx = kst + (t0 + u) * ks;
+#pragma unroll
+        for (int q = 0; q < RC; ++q) acc[q] = zfma(a[u], xin[kx][q], acc[q]);
       }
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-        if (t0 + u < nk) {
-          const int kx = kst + (t0 + u) * ks;
-#pragma unroll
-          for (int q = 0; q < RC; ++q) acc[q] = zfma(a[u], xin[kx][q], acc[q]);
-        }
-      if (more) {
-#pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] = nx[u];
-      }
+      for (int u = 0; u < 8; ++u) a[u] = nx[u];
     }
+    // the last, possibly partial, batch
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (t0 + u < nk) {
+        const int kx = kst + (t0 + u) * ks;
+#pragma unroll
+        for (int q = 0; q < RC; ++q) acc[q] = zfma(a[u], xin[kx][q], acc[q]);
+      }
     __syncthreads();
   }
   if (ks > 1) {
