@@ -78,14 +78,14 @@ def layers_for(mesh_paths):
     return ns, film, sub, sup
 
 
-def run_step(ns, film, sub, sup, wls, groups=1, ncv=0):
+def run_step(ns, film, sub, sup, wls, groups=1, ncv=0, workers=None):
     """All three layers for the wavelength batch `wls` (`groups` concurrent
     sub-batches per layer, Krylov dimension ncv), then the cascade."""
     from emustack_amd import objects, Stack
     from emustack_amd.batch import calc_modes_stack, calc_scat_many
     lights = [objects.Light(wl, max_order_PWs=ORDER, theta=0.0, phi=0.0) for wl in wls]
     # the three layers' sub-batches share one pool of host threads
-    sims = calc_modes_stack(ns, lights, num_BMs=NUM_BMS, groups=groups, ncv=ncv)
+    sims = calc_modes_stack(ns, lights, num_BMs=NUM_BMS, groups=groups, ncv=ncv, workers=workers)
     ops = int(calc_modes_stack.last_stats[0])
     stacks = [Stack((sub.calc_modes(light), sims[0][i], film.calc_modes(light), sims[1][i],
                      film.calc_modes(light), sims[2][i], sup.calc_modes(light))) for i, light in enumerate(lights)]
@@ -164,6 +164,8 @@ def main():
                     help='concurrent sub-batches (host threads, each with its own context and stream)')
     ap.add_argument('--ncv', type=int, default=384,
                     help='Krylov dimension (0: the reference rule 2 num_BMs + num_BMs/2 + 3 = 503)')
+    ap.add_argument('--workers', type=int, default=0,
+                    help='host threads (contexts) for the layer sub-batches (0: groups + 1)')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline sample')
     ap.add_argument('--workdir', default=os.path.join(ROOT, 'gpurun_out', 'bench_meshes'))
     args = ap.parse_args()
@@ -190,7 +192,7 @@ def main():
     for _ in range(args.warmup):
         wl = next_batch()
         print('warmup batch %s' % np.round(wl, 1), file=sys.stderr, flush=True)
-        run_step(ns, film, sub, sup, wl, args.groups, args.ncv)
+        run_step(ns, film, sub, sup, wl, args.groups, args.ncv, args.workers)
     ctxs = EMUstack.all_contexts()
     for c in ctxs:
         c.profile(True, every=1)
@@ -199,7 +201,7 @@ def main():
     t0 = time.perf_counter()
     for k in range(args.steps):
         wl = next_batch()
-        res, o = run_step(ns, film, sub, sup, wl, args.groups, args.ncv)
+        res, o = run_step(ns, film, sub, sup, wl, args.groups, args.ncv, args.workers)
         ops += o
         results += res
         print('step %d done (%.1f s)' % (k, time.perf_counter() - t0), file=sys.stderr, flush=True)
@@ -223,7 +225,7 @@ def main():
         'config': {'workload': 'stacked gratings: 3 structured layers (r=100/150/80 nm, ~%d-triangle P2 meshes) '
                                '+ 2 films, period 500 nm, num_BMs=%d, max_order_PWs=%d (98-dim S-matrix), '
                                '%d wavelengths 400-1000 nm, normal incidence' % (NTRI, NUM_BMS, ORDER, NWL),
-                   'batch_wavelengths': args.batch, 'groups': args.groups, 'krylov_dim': args.ncv or None, 'precision': 'fp64 (complex128) everywhere',
+                   'batch_wavelengths': args.batch, 'groups': args.groups, 'workers': args.workers or args.groups + 1, 'krylov_dim': args.ncv or None, 'precision': 'fp64 (complex128) everywhere',
                    'op_applications_per_instance': ops / (npts * 6.0)},
         'roofline': {'bound': 'hbm', 'kernel': kernel_name, 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': TRAFFIC_PER_DISPATCH.get(kernel_name),

@@ -48,6 +48,12 @@ __device__ __forceinline__ zd buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, ui
   const d2 v = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
   return zmk(v.x, v.y);
 }
+// the same for data read once (cache policy nt, as ld_stream)
+__device__ __forceinline__ zd buf_ld_stream(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2 v = __builtin_bit_cast(d2, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 2));
+  return zmk(v.x, v.y);
+}
 __host__ __device__ __forceinline__ zd zadd(zd a, zd b) { return zmk(a.x + b.x, a.y + b.y); }
 __host__ __device__ __forceinline__ zd zsub(zd a, zd b) { return zmk(a.x - b.x, a.y - b.y); }
 __host__ __device__ __forceinline__ zd zmul(zd a, zd b) {

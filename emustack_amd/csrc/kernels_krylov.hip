@@ -45,11 +45,17 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, int ncol, const int64_t
   for (int c = 0; c < SM_C; ++c) acc[c] = zmk(0, 0);
   if (r < n) {
     const int64_t k0 = rp[r], k1 = rp[r + 1];
-    // the column loads are unconditional (clamped column index, unused
-    // columns multiply zero) so that all of them are in flight together
-    for (int64_t k = k0 + lane8; k < k1; k += 8) {
-      const zd a = ld_stream(v + k);
-      const int64_t col = ld_stream(ci + k);
+    // The value and column index of the lane's next nonzero are requested
+    // before this one's gathers (indices clamped to the row), so each step
+    // waits for one round of loads; the gathers are unconditional (clamped
+    // column, unused columns multiply zero) so that all are in flight together.
+    int64_t k = k0 + lane8;
+    const int64_t klast = max(k1 - 1, k0);
+    zd a = ld_stream(v + min(k, klast));
+    int64_t col = ld_stream(ci + min(k, klast));
+    for (; k < k1; k += 8) {
+      const zd an = ld_stream(v + min(k + 8, klast));
+      const int64_t coln = ld_stream(ci + min(k + 8, klast));
       zd xv[SM_C];
 #pragma unroll
       for (int c = 0; c < SM_C; ++c) xv[c] = xb[(int64_t)min(c, nc - 1) * n + col];
@@ -58,6 +64,8 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, int ncol, const int64_t
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int c = 0; c < SM_C; ++c) acc[c] = zfma(c < nc ? a : zmk(0, 0), xv[c], acc[c]);
+      a = an;
+      col = coln;
     }
   }
 #pragma unroll

@@ -596,7 +596,7 @@ __global__ __launch_bounds__(256) void k_fwd(const SolveTask *__restrict__ tasks
     const int cbase = kc + kp, clast = min(kc + kp + max(nk - 1, 0) * ks, s - 1);
     zd a[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] = buf_ld(Fcol, rowb, (uint32_t)(min(cbase + u * ks, clast) * f) * 16u);
+    for (int u = 0; u < 8; ++u) a[u] = buf_ld_stream(Fcol, rowb, (uint32_t)(min(cbase + u * ks, clast) * f) * 16u);
     constexpr int NG = (RC + 3) / 4;  // right-hand sides gathered per thread
     zd gv[NG], g0[NG], g1[NG];
 #pragma unroll
@@ -621,7 +621,7 @@ __global__ __launch_bounds__(256) void k_fwd(const SolveTask *__restrict__ tasks
     for (; t0 + 8 < nk; t0 += 8) {
       zd nx[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) nx[u] = buf_ld(Fcol, rowb, (uint32_t)(min(cbase + (t0 + 8 + u) * ks, clast) * f) * 16u);
+      for (int u = 0; u < 8; ++u) nx[u] = buf_ld_stream(Fcol, rowb, (uint32_t)(min(cbase + (t0 + 8 + u) * ks, clast) * f) * 16u);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int kx = kp + (t0 + u) * ks;
@@ -775,7 +775,7 @@ __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks
     const int cbase = kc + kst, clast = min(kc + kst + max(nk - 1, 0) * ks, f - 1);
     zd a[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] = buf_ld(Fcol, rowb, (uint32_t)(min(cbase + u * ks, clast) * s) * 16u);
+    for (int u = 0; u < 8; ++u) a[u] = buf_ld_stream(Fcol, rowb, (uint32_t)(min(cbase + u * ks, clast) * s) * 16u);
     const zd *src = kkc < s ? y : x;
     constexpr int NG = (RC + 3) / 4;
     zd gv[NG];
@@ -793,7 +793,7 @@ __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks
     for (; t0 + 8 < nk; t0 += 8) {
       zd nx[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) nx[u] = buf_ld(Fcol, rowb, (uint32_t)(min(cbase + (t0 + 8 + u) * ks, clast) * s) * 16u);
+      for (int u = 0; u < 8; ++u) nx[u] = buf_ld_stream(Fcol, rowb, (uint32_t)(min(cbase + (t0 + 8 + u) * ks, clast) * s) * 16u);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int kx = kst + (t0 + u) * ks;
