@@ -12,6 +12,10 @@
 #include <random>
 #include <thread>
 
+#include <sys/resource.h>
+#include <sys/syscall.h>
+#include <unistd.h>
+
 #include "emu_host.h"
 #include "engine.h"
 #include "fem_basis.h"
@@ -792,6 +796,11 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
     std::vector<char> convflag(nact, 0);
     for (int b = 0; b < nact; ++b) {
       th.emplace_back([&, b]() {
+        // The host Schur steps of concurrent sub-batches can occupy every
+        // core; at lower priority they leave the threads that launch GPU
+        // work runnable at once (a launch thread waiting for a core leaves
+        // the GPU idle).  Raising one's own nice value needs no privilege.
+        (void)setpriority(PRIO_PROCESS, (id_t)syscall(SYS_gettid), 10);
         try {
           KSState &S = st[b];
           for (int64_t i = 0; i < sH; ++i) S.H[i] = cplx(Hh[(size_t)b * sH + i].x, Hh[(size_t)b * sH + i].y);
