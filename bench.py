@@ -179,9 +179,13 @@ def main():
     mesh_paths = build_meshes(args.workdir)
     ns, film, sub, sup = layers_for(mesh_paths)
     wavelengths = np.linspace(400.0, 1000.0, NWL)
+    from emustack_amd.batch import preload_meshes
     ctx = EMUstack.context()
     for p in mesh_paths:
         EMUstack.mesh(p)
+    # one-time setup outside the timed steps: every worker context loads the
+    # three meshes (a first load inside a step would stall the other workers)
+    preload_meshes(ns, groups=args.groups, workers=args.workers or None)
     step_i = 0
 
     def next_batch():

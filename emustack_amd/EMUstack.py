@@ -39,15 +39,19 @@ def all_contexts():
 
 
 def mesh(path, leaf_size=0):
-    """Loaded mesh handle for `path` (cached per context and path)."""
+    """Loaded mesh handle for `path` (cached per context and path).  The load
+    (file, symbolic analysis, upload) runs outside the cache lock, so that
+    other threads' contexts and cached meshes stay available meanwhile; a
+    handle is only ever used by its own context's thread."""
     ctx = context()
     key = (id(ctx), os.path.abspath(path), leaf_size)
     with _cache_lock:
         h = _mesh_cache.get(key)
-        if h is None:
-            h = _lib.MeshHandle(os.path.abspath(path), ctx, leaf_size)
-            _mesh_cache[key] = h
-        return h
+    if h is None:
+        h = _lib.MeshHandle(os.path.abspath(path), ctx, leaf_size)
+        with _cache_lock:
+            h = _mesh_cache.setdefault(key, h)
+    return h
 
 
 def calc_modes_2d(lambda_, nval, ordre_ls, debug, msh_path, mesh_file, npt, nel, nb_typ_el, n_eff, bloch_vec,

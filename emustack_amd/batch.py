@@ -127,6 +127,26 @@ def calc_modes_stack(structures, lights, num_BMs=None, tol=0.0, itermax=30, grou
     return out
 
 
+def preload_meshes(structures, groups=2, workers=None):
+    """Load the meshes of the 2D layers in `structures` into the context of
+    every thread of the pool that calc_modes_stack(..., groups, workers=)
+    uses, so that later batches find them loaded (a load is a one-time
+    setup: mesh file, symbolic analysis, upload).  One task per pool thread
+    (the tasks wait for each other at a barrier)."""
+    n = int(workers) if workers else groups + 1
+    files = sorted({paths.msh_path + st.mesh_file for st in structures if st.periodicity == '2D_array'})
+    pool = _pool(n)
+    bar = threading.Barrier(n)
+
+    def load():
+        bar.wait(timeout=600)
+        for f in files:
+            EMUstack.mesh(f)
+
+    for f in [pool.submit(load) for _ in range(n)]:
+        f.result()
+
+
 calc_modes_stack.last_stats = None
 calc_modes_many.last_stats = None
 

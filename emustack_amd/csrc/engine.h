@@ -198,6 +198,10 @@ struct DevMesh {
 };
 
 // A device context: one HIP stream and named, grow-only work buffers.
+// Buffers are allocated with 1/16 headroom and grow by at least a quarter:
+// batches on meshes of nearly equal size then reuse them, and hipFree (which
+// waits for the whole device, stalling every other context's stream) stays
+// out of the steady state.
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -212,15 +216,17 @@ struct Ctx {
         if (sizes[i] >= bytes) return bufs[i];
         (void)hipFree(bufs[i]);
         bufs[i] = nullptr;
-        if (hipMalloc(&bufs[i], bytes) != hipSuccess) throw HipError("hipMalloc failed for " + name);
-        sizes[i] = bytes;
+        const size_t grow = std::max(bytes + bytes / 16, sizes[i] + sizes[i] / 4);
+        if (hipMalloc(&bufs[i], grow) != hipSuccess) throw HipError("hipMalloc failed for " + name);
+        sizes[i] = grow;
         return bufs[i];
       }
     void *p = nullptr;
-    if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) throw HipError("hipMalloc failed for " + name);
+    const size_t first = std::max<size_t>(bytes + bytes / 16, 256);
+    if (hipMalloc(&p, first) != hipSuccess) throw HipError("hipMalloc failed for " + name);
     names.push_back(name);
     bufs.push_back(p);
-    sizes.push_back(bytes);
+    sizes.push_back(first);
     return p;
   }
   template <typename T>

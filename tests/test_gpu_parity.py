@@ -187,6 +187,8 @@ def test_stack_batch_equals_per_layer(gpu_ctx):
                                   materials.Material(3.0 + 0.01j), make_mesh_now=False, mesh_file=MESH_600)
                for h in (200.0, 400.0)]
     many = [calc_modes_many(s, lights, num_BMs=20, groups=2) for s in structs]
+    from emustack_amd.batch import preload_meshes
+    preload_meshes(structs, groups=2)  # every pool thread's context holds the mesh beforehand
     stack = calc_modes_stack(structs, lights, num_BMs=20, groups=2)
     for a, b in zip(many, stack):
         for sa, sb in zip(a, b):
