@@ -159,13 +159,14 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=2)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--batch', type=int, default=16, help='wavelength points per step')
-    ap.add_argument('--groups', type=int, default=2,
+    ap.add_argument('--batch', type=int, default=48, help='wavelength points per step')
+    ap.add_argument('--groups', type=int, default=6,
                     help='concurrent sub-batches (host threads, each with its own context and stream)')
     ap.add_argument('--ncv', type=int, default=384,
                     help='Krylov dimension (0: the reference rule 2 num_BMs + num_BMs/2 + 3 = 503)')
-    ap.add_argument('--workers', type=int, default=0,
-                    help='host threads (contexts) for the layer sub-batches (0: groups + 1)')
+    ap.add_argument('--workers', type=int, default=3,
+                    help='host threads (contexts) for the layer sub-batches (0: groups + 1); each context '
+                         'holds ~40 GB at 8 points per sub-batch, and 16 host cores serve the Schur steps')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline sample')
     ap.add_argument('--workdir', default=os.path.join(ROOT, 'gpurun_out', 'bench_meshes'))
     args = ap.parse_args()

@@ -139,6 +139,7 @@ struct SolveTasks {
 struct StepPlan {
   int32_t k0;
   int64_t panel_off, panel_cnt;  // front ids
+  int64_t rows_off, rows_cnt;    // (front, row0): update rows s..f-1 of the panel, 64 per task
   int64_t trsm_off, trsm_cnt;    // (front, col0)
   int64_t gemm_off, gemm_cnt;    // (front, row0, col0)
 };

@@ -195,6 +195,13 @@ void devmesh_build(DevMesh &dm, hipStream_t s) {
       for (int32_t t : LP.fronts)
         if (sy.fronts[t].s > k0) tasks.push_back(t);
       st.panel_cnt = (int64_t)tasks.size() - st.panel_off;
+      st.rows_off = (int64_t)tasks.size();
+      for (int32_t t : LP.fronts) {
+        const FrontInfo &F = sy.fronts[t];
+        if (F.s <= k0) continue;
+        for (int32_t r = F.s; r < F.f; r += 64) { tasks.push_back(t); tasks.push_back(r); }
+      }
+      st.rows_cnt = ((int64_t)tasks.size() - st.rows_off) / 2;
       st.trsm_off = (int64_t)tasks.size();
       for (int32_t t : LP.fronts) {
         const FrontInfo &F = sy.fronts[t];

@@ -304,22 +304,17 @@ __global__ __launch_bounds__(256) void k_atb_mfma16(int64_t K, int M, int64_t kc
   const int64_t kb = chunk * kchunk, ke = min(K, kb + kchunk);
   const int lr = tid & 31, lc = tid >> 5;  // staging: row lr, columns lc + 8 i
   zd ra[8], rb[2];
-  // unconditional loads from clamped addresses (see k_atb_tall)
+  // unconditional loads from clamped addresses (see k_atb_tall); the
+  // out-of-range values are zeroed when they are stored to LDS, not here: a
+  // select right after the load would make the wave wait for the prefetch
+  // before the matrix-core phase instead of during it
   auto fetch = [&](int64_t k0) {
     const int64_t k = k0 + lr;
-    const bool okk = k < ke;
-    const int64_t kc = okk ? k : ke - 1;
+    const int64_t kc = k < ke ? k : ke - 1;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int m = m0 + lc + 8 * i;
-      const zd v = Ab[kc + (int64_t)min(m, M - 1) * lda];
-      ra[i] = (okk && m < M) ? v : zmk(0, 0);
-    }
+    for (int i = 0; i < 8; ++i) ra[i] = Ab[kc + (int64_t)min(m0 + lc + 8 * i, M - 1) * lda];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const zd v = Bb[kc + (int64_t)(lc + 8 * i) * ldb];
-      rb[i] = okk ? v : zmk(0, 0);
-    }
+    for (int i = 0; i < 2; ++i) rb[i] = Bb[kc + (int64_t)(lc + 8 * i) * ldb];
   };
   v4d pr[4], pi[4];
 #pragma unroll
@@ -331,10 +326,11 @@ __global__ __launch_bounds__(256) void k_atb_mfma16(int64_t K, int M, int64_t kc
   const int fi = lane & 15, fk = lane >> 4;
   for (int64_t k0 = kb; k0 < ke; k0 += TK) {
     __syncthreads();
+    const bool okk = k0 + lr < ke;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) As[lr][lc + 8 * i] = ra[i];
+    for (int i = 0; i < 8; ++i) As[lr][lc + 8 * i] = (okk && m0 + lc + 8 * i < M) ? ra[i] : zmk(0, 0);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) Bs[lr][lc + 8 * i] = rb[i];
+    for (int i = 0; i < 2; ++i) Bs[lr][lc + 8 * i] = okk ? rb[i] : zmk(0, 0);
     __syncthreads();
     if (k0 + TK < ke) fetch(k0 + TK);
 #pragma unroll
@@ -737,17 +733,21 @@ __global__ __launch_bounds__(256) void k_combine(int64_t n, int k, const zd *__r
   for (int idx = threadIdx.x; idx < NQ * NQ; idx += blockDim.x) ls[idx % NQ][idx / NQ] = L[(int64_t)b * NQ * NQ + idx];
   __syncthreads();
   zd acc[NQ];
-  {
-    zd x[NQ];
 #pragma unroll
-    for (int p = 0; p < NQ; ++p) x[p] = W[b * sW + p * ldw + rr];
+  for (int q = 0; q < NQ; ++q) acc[q] = zmk(0, 0);
+  // acc = W L, W read in halves of NQ / 2 columns, each half's loads all in
+  // flight before the first use (the scheduler would otherwise wait for each
+  // in turn); halves keep the live registers within four waves per SIMD
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      zd a = zmk(0, 0);
+  for (int h = 0; h < NQ; h += NQ / 2) {
+    zd x[NQ / 2];
 #pragma unroll
-      for (int p = 0; p < NQ; ++p) a = zfma(x[p], ls[p][q], a);
-      acc[q] = a;
-    }
+    for (int p = 0; p < NQ / 2; ++p) x[p] = W[b * sW + (h + p) * ldw + rr];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+      for (int p = 0; p < NQ / 2; ++p) acc[q] = zfma(x[p], ls[h + p][q], acc[q]);
   }
 #pragma unroll 2
   for (int c = 0; c < k; ++c) {

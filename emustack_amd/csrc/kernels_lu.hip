@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ tasks
     const int nrest = w - jj - 1;
     for (int c = tid; c < nrest; c += 256) urow[c] = F[k + (int64_t)(k + 1 + c) * f];
     __syncthreads();
-    for (int r = k + 1 + tid; r < f; r += 256) {
+    for (int r = k + 1 + tid; r < s; r += 256) {
       zd l = zmul(F[r + (int64_t)k * f], dinv);
       F[r + (int64_t)k * f] = l;
       // rank-1 update of the row's remaining panel columns, eight at a time
@@ -156,6 +156,51 @@ __global__ __launch_bounds__(256) void k_panel(const int32_t *__restrict__ tasks
     }
     __syncthreads();
   }
+}
+
+// The panel's update rows (s <= r < f: not pivot candidates) by a row-wise
+// triangular solve against the factored panel block U (w x w, unit-free
+// upper triangle incl. the possibly perturbed diagonal): x U = a_r.  The
+// operations are those k_panel's column steps would apply to the row, in the
+// same order (l = x_j / U_jj via the reciprocal, then x_c -= l U_jc), so the
+// result is the same; the rows are independent, so they run on all CUs
+// instead of inside the panel's single workgroup.  One thread per row.
+__global__ __launch_bounds__(64) void k_panel_rows(const int32_t *__restrict__ tasks, int k0, FrontArrays A,
+                                                   zd *__restrict__ pool, int64_t psz) {
+  const int t = tasks[2 * blockIdx.x], r0 = tasks[2 * blockIdx.x + 1];
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  const int s = A.s[t], f = A.f[t];
+  zd *F = pool + b * psz + A.off[t];
+  const int w = min(NB, s - k0);
+  __shared__ zd U[NB][NB + 1];
+  __shared__ zd dinv[NB];
+  for (int idx = tid; idx < w * w; idx += 64) {
+    const int i = idx % w, j = idx / w;
+    U[i][j] = F[(k0 + i) + (int64_t)(k0 + j) * f];
+  }
+  __syncthreads();
+  if (tid < w) dinv[tid] = zdiv(zmk(1.0, 0.0), U[tid][tid]);
+  __syncthreads();
+  const int r = r0 + tid;
+  if (r >= f) return;
+  zd *row = F + r + (int64_t)k0 * f;
+  zd x[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) x[j] = row[(int64_t)min(j, w - 1) * f];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    if (j < w) {
+      const zd l = zmul(x[j], dinv[j]);
+      x[j] = l;
+#pragma unroll
+      for (int c = j + 1; c < NB; ++c)
+        if (c < w) x[c] = zfms(l, U[j][c], x[c]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+    if (j < w) row[(int64_t)j * f] = x[j];
 }
 
 __global__ __launch_bounds__(64) void k_trsm(const int32_t *__restrict__ tasks, int k0, FrontArrays A,
@@ -311,6 +356,9 @@ void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int3
       if (st.panel_cnt)
         hipLaunchKernelGGL(k_panel, dim3((unsigned)st.panel_cnt, nb), dim3(256), 0, s,
                            dm.tasks.p + st.panel_off, st.k0, A, pool, psz, piv, sy.npiv, pflag, sig, pert);
+      if (st.rows_cnt)
+        hipLaunchKernelGGL(k_panel_rows, dim3((unsigned)st.rows_cnt, nb), dim3(64), 0, s,
+                           dm.tasks.p + st.rows_off, st.k0, A, pool, psz);
       if (st.trsm_cnt)
         hipLaunchKernelGGL(k_trsm, dim3((unsigned)st.trsm_cnt, nb), dim3(64), 0, s,
                            dm.tasks.p + st.trsm_off, st.k0, A, pool, psz);
