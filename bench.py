@@ -39,14 +39,14 @@ ORDER = 4
 NWL = 512
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per dispatch of the kernels that can be the roofline kernel, from
-# two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this workload at these
-# defaults, calibrated against a plain 2 GiB copy; the runs and their
-# commands are summarised in profiles/kernels_stacked_gratings_8k.md.
+# two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this workload at 16
+# points in 2 sub-batches (8 points = 16 instances per dispatch, as in the
+# default step), each scaled by what the counter reads for a plain 2 GiB copy.
 TRAFFIC_PER_DISPATCH = {
-    'k_atb_mfma16<true>': 3802547283,
-    'k_combine<16>': 3656499385,
-    'k_fwd': 367811805,
-    'k_bwd': 368452440,
+    'k_atb_mfma16<true>': 8473211998,
+    'k_combine<16>': 7324606084,
+    'k_fwd': 433422318,
+    'k_bwd': 412855311,
 }
 
 
