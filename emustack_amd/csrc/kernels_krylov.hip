@@ -43,8 +43,8 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, int ncol, const int64_t
   zd acc[SM_C];
 #pragma unroll
   for (int c = 0; c < SM_C; ++c) acc[c] = zmk(0, 0);
-  if (r < n) {
-    const int64_t k0 = rp[r], k1 = rp[r + 1];
+  const int64_t k0 = r < n ? rp[r] : 0, k1 = r < n ? rp[r + 1] : 0;
+  if (k1 > k0) {  // rows >= n and empty rows read nothing
     // The value and column index of the lane's next nonzero are requested
     // before this one's gathers (indices clamped to the row), so each step
     // waits for one round of loads; the gathers are unconditional (clamped

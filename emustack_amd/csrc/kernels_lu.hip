@@ -240,6 +240,13 @@ __global__ __launch_bounds__(64) void k_trsm(const int32_t *__restrict__ tasks, 
     if (i < w) col[i] = x[i];
 }
 
+// C -= L21 U12 on one 64 x 64 tile of the trailing block.  The panel's K = w
+// <= 32 columns are staged through LDS in two halves of 16 (32 KB of LDS, so
+// that four workgroups share a CU); thread (tx, ty) owns rows tx + 16 i and
+// columns ty + 16 j, so a wave's C accesses cover 16 consecutive rows of 4
+// columns; the 16 reads of C are issued together from clamped addresses
+// before any is used, the stores are masked.
+constexpr int GK = 16;  // K staged per round
 __global__ __launch_bounds__(256) void k_gemm_update(const int32_t *__restrict__ tasks, int k0,
                                                      FrontArrays A, zd *__restrict__ pool,
                                                      int64_t psz) {
@@ -249,51 +256,62 @@ __global__ __launch_bounds__(256) void k_gemm_update(const int32_t *__restrict__
   const int s = A.s[t], f = A.f[t];
   zd *F = pool + b * psz + A.off[t];
   const int w = min(NB, s - k0);
-  __shared__ zd As[NB][TILE];
-  __shared__ zd Bs[NB][TILE];
-  // staging: unconditional loads from clamped addresses (a load inside a
-  // branch is waited for before the branch merges, which would serialise
-  // the tile's loads), out-of-range entries zeroed afterwards
-#pragma unroll
-  for (int u = 0; u < NB * TILE / 256; ++u) {
-    const int idx = tid + 256 * u, i = idx % TILE, kk = idx / TILE, r = r0 + i;
-    const zd v = F[min(r, f - 1) + (int64_t)(k0 + min(kk, w - 1)) * f];
-    As[kk][i] = (kk < w && r < f) ? v : zmk(0, 0);
-  }
-#pragma unroll
-  for (int u = 0; u < NB * TILE / 256; ++u) {
-    const int idx = tid + 256 * u, kk = idx % NB, j = idx / NB, c = c0 + j;
-    const zd v = F[(k0 + min(kk, w - 1)) + (int64_t)min(c, f - 1) * f];
-    Bs[kk][j] = (kk < w && c < f) ? v : zmk(0, 0);
-  }
-  __syncthreads();
+  __shared__ zd As[GK][TILE];
+  __shared__ zd Bs[GK][TILE];
   const int tx = tid % 16, ty = tid / 16;
   zd acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = zmk(0, 0);
-  for (int kk = 0; kk < w; ++kk) {
-    zd a[4], bb[4];
+  for (int kb = 0; kb < w; kb += GK) {
+    if (kb > 0) __syncthreads();
+    // staging: unconditional loads from clamped addresses (a load inside a
+    // branch is waited for before the branch merges, which would serialise
+    // the tile's loads), out-of-range entries zeroed afterwards
 #pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+    for (int u = 0; u < GK * TILE / 256; ++u) {
+      const int idx = tid + 256 * u, i = idx % TILE, kk = idx / TILE, r = r0 + i;
+      const zd v = F[min(r, f - 1) + (int64_t)(k0 + min(kb + kk, w - 1)) * f];
+      As[kk][i] = (kb + kk < w && r < f) ? v : zmk(0, 0);
+    }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) bb[j] = Bs[kk][tx + 16 * j];
+    for (int u = 0; u < GK * TILE / 256; ++u) {
+      const int idx = tid + 256 * u, kk = idx % GK, jj = idx / GK, c = c0 + jj;
+      const zd v = F[(k0 + min(kb + kk, w - 1)) + (int64_t)min(c, f - 1) * f];
+      Bs[kk][jj] = (kb + kk < w && c < f) ? v : zmk(0, 0);
+    }
+    __syncthreads();
+    const int kn = min(GK, w - kb);
+    for (int kk = 0; kk < kn; ++kk) {
+      zd a[4], bb[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = zfma(a[i], bb[j], acc[i][j]);
+      for (int j = 0; j < 4; ++j) bb[j] = Bs[kk][ty + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = zfma(a[i], bb[j], acc[i][j]);
+    }
   }
+  // in two halves of 8 entries (fewer live registers)
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    int c = c0 + tx + 16 * j;
-    if (c >= f) continue;
+  for (int jh = 0; jh < 4; jh += 2) {
+    zd cv[4][2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      int r = r0 + ty + 16 * i;
-      if (r < f) {
-        zd *p = &F[r + (int64_t)c * f];
-        *p = zsub(*p, acc[i][j]);
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        cv[i][j] = F[min(r0 + tx + 16 * i, f - 1) + (int64_t)min(c0 + ty + 16 * (jh + j), f - 1) * f];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = c0 + ty + 16 * (jh + j);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = r0 + tx + 16 * i;
+        if (r < f && c < f) F[r + (int64_t)c * f] = zsub(cv[i][j], acc[i][jh + j]);
       }
     }
   }
