@@ -7,11 +7,13 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <condition_variable>
 #include <mutex>
 #include <numeric>
 #include <random>
 #include <thread>
 
+#include <sched.h>
 #include <sys/resource.h>
 #include <sys/syscall.h>
 #include <unistd.h>
@@ -657,6 +659,43 @@ static void block_cholqr2(Ctx *c, int nb, int64_t n, int bq, zd *W, int64_t sV, 
   launch_trmm_right(nb, n, bq, W, n, sV, Ri, W, n, sV, s);
 }
 
+// First-come admission of host Schur steps to the cores: at most one running
+// step per available CPU, admitted in ticket order.  Sub-batches that reach
+// their Schur phase together would otherwise share the cores and all finish
+// late; in arrival order the first sub-batch's steps finish first and its
+// next GPU phase starts while the others compute (the total CPU work is the
+// same).
+class CoreGate {
+ public:
+  CoreGate() {
+    cpu_set_t cs;
+    cap_ = sched_getaffinity(0, sizeof(cs), &cs) == 0 ? std::max(1, CPU_COUNT(&cs)) : 8;
+  }
+  uint64_t enter() {
+    std::unique_lock<std::mutex> lk(mu_);
+    const uint64_t t = next_++;
+    cv_.wait(lk, [&] { return t < done_ + cap_; });
+    return t;
+  }
+  void leave() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      ++done_;
+    }
+    cv_.notify_all();
+  }
+
+ private:
+  std::mutex mu_;
+  std::condition_variable cv_;
+  uint64_t next_ = 0, done_ = 0;
+  uint64_t cap_ = 8;
+};
+static CoreGate &core_gate() {
+  static CoreGate g;
+  return g;
+}
+
 static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int bq, int itermax, double tol,
                               const zd *inv, const int32_t *gx, const zd *mvals, zd *Xout, StageTimer &tm) {
   hipStream_t s = c->stream;
@@ -808,6 +847,10 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
         // work runnable at once (a launch thread waiting for a core leaves
         // the GPU idle).  Raising one's own nice value needs no privilege.
         (void)setpriority(PRIO_PROCESS, (id_t)syscall(SYS_gettid), 10);
+        core_gate().enter();
+        struct Leave {
+          ~Leave() { core_gate().leave(); }
+        } leave_gate;
         try {
           KSState &S = st[b];
           for (int64_t i = 0; i < sH; ++i) S.H[i] = cplx(Hh[(size_t)b * sH + i].x, Hh[(size_t)b * sH + i].y);
