@@ -479,7 +479,34 @@ struct KSState {
   std::vector<cplx> theta;
   int nlock = 0;           // leading Schur vectors locked (their residual coupling is zero in H)
   int nlock_next = 0;      // locked after this projection (applied at the restart)
+  std::vector<zd> Qk, Hk;  // restart blocks: Q[:, :keep] and the restarted H (filled by ks_restart_blocks)
 };
+
+// The restart blocks of one instance, computed on its host thread right after
+// its Schur step: Q[:, :keep] (m x keep) and the restarted projected matrix
+// (ld x m: T[:keep, :keep] on top, the residual coupling B Q[:, :keep] in rows
+// keep..keep+bq, zero for the locked columns).
+static void ks_restart_blocks(KSState &S, int m, int bq, int keep, int ld, int64_t sH) {
+  S.Qk.assign((size_t)m * keep, zmk(0, 0));
+  S.Hk.assign((size_t)sH, zmk(0, 0));
+  for (int j = 0; j < keep; ++j)
+    for (int i = 0; i < m; ++i) {
+      const cplx q = S.Q[i + (size_t)j * m];
+      S.Qk[i + (size_t)j * m] = zmk(q.real(), q.imag());
+    }
+  for (int j = 0; j < keep; ++j) {
+    for (int i = 0; i <= j; ++i) {
+      const cplx t = S.T[i + (size_t)j * m];
+      S.Hk[i + (size_t)j * ld] = zmk(t.real(), t.imag());
+    }
+    for (int r = 0; r < bq; ++r) {
+      cplx acc = 0;
+      if (j >= S.nlock_next)  // locked: coupling deflated to zero
+        for (int i = 0; i < m; ++i) acc += S.H[(m + r) + (size_t)i * ld] * S.Q[i + (size_t)j * m];
+      S.Hk[(keep + r) + (size_t)j * ld] = zmk(acc.real(), acc.imag());
+    }
+  }
+}
 
 // Schur-decompose H[:m,:m], order the `keep` largest |theta| first and the
 // `nev` largest among them first of all, test the residuals
@@ -856,6 +883,9 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
           for (int64_t i = 0; i < sH; ++i) S.H[i] = cplx(Hh[(size_t)b * sH + i].x, Hh[(size_t)b * sH + i].y);
           // Ritz vectors of the basis are formed once the pairs pass the test
           convflag[b] = ks_project(S, m, bq, nev, keep, tol, &ritz[b], nullptr, false) ? 1 : 0;
+          // the restart's host part here, in parallel, not on the launching
+          // thread while it holds the GPU turn
+          if (!convflag[b] && !last) ks_restart_blocks(S, m, bq, keep, ld, sH);
         } catch (std::exception &e) {
           errs[b] = e.what();
         }
@@ -903,26 +933,13 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
     int a0 = keep;
     for (int b = 0; b < nact; ++b) a0 = std::min(a0, std::min(st[b].nlock, nev));
     std::vector<zd> Qh((size_t)nact * m * keep);
-    std::vector<zd> Hn((size_t)nact * sH, zmk(0, 0));
+    std::vector<zd> Hn((size_t)nact * sH);
     for (int b = 0; b < nact; ++b) {
       KSState &S = st[b];
-      for (int j = 0; j < keep; ++j)
-        for (int i = 0; i < m; ++i) {
-          cplx q = S.Q[i + (size_t)j * m];
-          Qh[(size_t)b * m * keep + i + (size_t)j * m] = zmk(q.real(), q.imag());
-        }
-      for (int j = 0; j < keep; ++j) {
-        for (int i = 0; i <= j; ++i) {
-          cplx t = S.T[i + (size_t)j * m];
-          Hn[(size_t)b * sH + i + (size_t)j * ld] = zmk(t.real(), t.imag());
-        }
-        for (int r = 0; r < bq; ++r) {
-          cplx acc = 0;
-          if (j >= S.nlock_next)  // locked: coupling deflated to zero
-            for (int i = 0; i < m; ++i) acc += S.H[(m + r) + (size_t)i * ld] * S.Q[i + (size_t)j * m];
-          Hn[(size_t)b * sH + (keep + r) + (size_t)j * ld] = zmk(acc.real(), acc.imag());
-        }
-      }
+      if (S.Qk.size() != (size_t)m * keep || S.Hk.size() != (size_t)sH)
+        throw std::logic_error("restart blocks missing for an active instance");
+      std::copy(S.Qk.begin(), S.Qk.end(), Qh.begin() + (size_t)b * m * keep);
+      std::copy(S.Hk.begin(), S.Hk.end(), Hn.begin() + (size_t)b * sH);
       S.nlock = S.nlock_next;
     }
     EMU_HIP(hipMemcpyAsync(Qd, Qh.data(), sizeof(zd) * Qh.size(), hipMemcpyHostToDevice, s));
