@@ -480,6 +480,7 @@ struct KSState {
   int nlock = 0;           // leading Schur vectors locked (their residual coupling is zero in H)
   int nlock_next = 0;      // locked after this projection (applied at the restart)
   std::vector<zd> Qk, Hk;  // restart blocks: Q[:, :keep] and the restarted H (filled by ks_restart_blocks)
+  std::vector<zd> Wz;      // Ritz vectors in the basis (m x nev) once converged
 };
 
 // The restart blocks of one instance, computed on its host thread right after
@@ -802,7 +803,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
     launch_swap(sH, Hd + a1 * sH, Hd + a2 * sH, s);
     launch_swap_i32(1, bad + a1, bad + a2, s);
   };
-  zd *Wd = c->buf<zd>("ks_W", (size_t)m * nev);
+  zd *Wd = c->buf<zd>("ks_W", (size_t)nb * m * nev);
   // The GPU phases of a cycle (restart combination + expansion) hold the
   // device's turn; the host Schur phase releases it, so that concurrent
   // sub-batches on other contexts alternate on the GPU instead of expanding
@@ -886,6 +887,11 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
           // the restart's host part here, in parallel, not on the launching
           // thread while it holds the GPU turn
           if (!convflag[b] && !last) ks_restart_blocks(S, m, bq, keep, ld, sH);
+          if (convflag[b]) {
+            const std::vector<cplx> &W = ritz[b];
+            S.Wz.resize(W.size());
+            for (size_t i = 0; i < W.size(); ++i) S.Wz[i] = zmk(W[i].real(), W[i].imag());
+          }
         } catch (std::exception &e) {
           errs[b] = e.what();
         }
@@ -900,18 +906,33 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
     // Retire the converged instances: their Ritz vectors X = V[:, :m] W are
     // formed now, and the slot is filled with the last active instance, so
     // that the next cycles run on the active instances only.
+    // All converged instances' Ritz vectors first, from the slots as they are
+    // (one copy of the coefficients, the products back to back), then the
+    // slot exchanges.
+    {
+      std::vector<int> conv;
+      for (int b = 0; b < nact; ++b)
+        if (convflag[b]) conv.push_back(b);
+      if (!conv.empty()) {
+        std::vector<zd> Wall(conv.size() * (size_t)m * nev);
+        for (size_t i = 0; i < conv.size(); ++i) {
+          const KSState &S = st[conv[i]];
+          if (S.Wz.size() != (size_t)m * nev) throw std::logic_error("Ritz vectors missing after convergence");
+          std::copy(S.Wz.begin(), S.Wz.end(), Wall.begin() + i * (size_t)m * nev);
+          const int o = slot_inst[conv[i]];
+          for (int q = 0; q < nev; ++q) res.mu[(size_t)o * nev + q] = S.theta[q];
+        }
+        EMU_HIP(hipMemcpyAsync(Wd, Wall.data(), sizeof(zd) * Wall.size(), hipMemcpyHostToDevice, s));
+        for (size_t i = 0; i < conv.size(); ++i) {
+          const int b = conv[i], o = slot_inst[b];
+          launch_gemm_nn(1, n, nev, m, zmk(1, 0), V + (int64_t)b * sV, n, sV, Wd + i * (size_t)m * nev, m,
+                         (int64_t)m * nev, zmk(0, 0), Xout + (int64_t)o * n * nev, n, (int64_t)n * nev, s);
+        }
+        EMU_HIP(hipStreamSynchronize(s));  // Wall is released below
+      }
+    }
     for (int b = nact - 1; b >= 0; --b) {
       if (!convflag[b]) continue;
-      const int o = slot_inst[b];
-      const std::vector<cplx> &W = ritz[b];
-      if (W.size() != (size_t)m * nev) throw std::logic_error("Ritz vectors missing after convergence");
-      std::vector<zd> Wh((size_t)m * nev);
-      for (size_t i = 0; i < W.size(); ++i) Wh[i] = zmk(W[i].real(), W[i].imag());
-      for (int i = 0; i < nev; ++i) res.mu[(size_t)o * nev + i] = st[b].theta[i];
-      EMU_HIP(hipMemcpyAsync(Wd, Wh.data(), sizeof(zd) * Wh.size(), hipMemcpyHostToDevice, s));
-      launch_gemm_nn(1, n, nev, m, zmk(1, 0), V + (int64_t)b * sV, n, sV, Wd, m, (int64_t)m * nev, zmk(0, 0),
-                     Xout + (int64_t)o * n * nev, n, (int64_t)n * nev, s);
-      EMU_HIP(hipStreamSynchronize(s));  // Wh is reused
       const int e = nact - 1;
       if (b != e) {
         swap_slots(b, e);
