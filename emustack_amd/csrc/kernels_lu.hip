@@ -515,45 +515,61 @@ __global__ __launch_bounds__(256) void k_wz(const int32_t *__restrict__ tasks, F
   const int r0 = tasks[4 * blockIdx.x + 2], c0 = tasks[4 * blockIdx.x + 3];
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
-  const int s = A.s[t], u = A.u[t], f = A.f[t];
+  const int s = A.s[t], f = A.f[t];
   const zd *F = pool + b * psz + A.off[t];
   zd *Ofw = inv + b * isz + A.ioff[t];
   zd *Obw = Ofw + (int64_t)f * s;
-  const int M = kind ? s : u, N = kind ? u : s;
+  const int M = kind ? s : A.u[t], N = kind ? A.u[t] : s;
   const int kbeg = kind ? r0 : c0;
-  __shared__ zd As[NB][TILE];
-  __shared__ zd Bs[NB][TILE];
-  const int tx = tid % 16, ty = tid / 16;
+  // K staged in rounds of GK (32 KB of LDS, four workgroups per CU); the next
+  // round's operands are fetched into registers while this one is used, from
+  // clamped addresses, and masked when stored to LDS (a select right after
+  // the load would wait for it at once)
+  __shared__ zd As[GK][TILE];
+  __shared__ zd Bs[GK][TILE];
+  const int tx = tid % 16, ty = tid / 16;  // rows tx + 16 i, columns ty + 16 j
   zd acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = zmk(0, 0);
-  for (int k0 = kbeg; k0 < s; k0 += NB) {
+  constexpr int NL = GK * TILE / 256;
+  zd ra[NL], rb[NL];
+  auto fetch = [&](int k0) {
 #pragma unroll
-    for (int u = 0; u < NB * TILE / 256; ++u) {
+    for (int u = 0; u < NL; ++u) {
       const int idx = tid + 256 * u, i = idx % TILE, kk = idx / TILE;
-      const int rr = r0 + i, k = k0 + kk;
-      const int rc = min(rr, M - 1), kc = min(k, s - 1);
-      const zd a = kind ? Obw[rc + (int64_t)kc * s] : F[(s + rc) + (int64_t)kc * f];
-      As[kk][i] = (rr < M && k < s) ? a : zmk(0, 0);
+      const int rc = min(r0 + i, M - 1), kc = min(k0 + kk, s - 1);
+      ra[u] = kind ? Obw[rc + (int64_t)kc * s] : F[(s + rc) + (int64_t)kc * f];
     }
 #pragma unroll
-    for (int u = 0; u < NB * TILE / 256; ++u) {
-      const int idx = tid + 256 * u, kk = idx % NB, j = idx / NB;
-      const int cc = c0 + j, k = k0 + kk;
-      const int ccc = min(cc, N - 1), kc = min(k, s - 1);
-      const zd v = kind ? F[kc + (int64_t)(s + ccc) * f] : Ofw[kc + (int64_t)ccc * f];
-      Bs[kk][j] = (cc < N && k < s) ? v : zmk(0, 0);
+    for (int u = 0; u < NL; ++u) {
+      const int idx = tid + 256 * u, kk = idx % GK, jj = idx / GK;
+      const int ccc = min(c0 + jj, N - 1), kc = min(k0 + kk, s - 1);
+      rb[u] = kind ? F[kc + (int64_t)(s + ccc) * f] : Ofw[kc + (int64_t)ccc * f];
+    }
+  };
+  if (kbeg < s) fetch(kbeg);
+  for (int k0 = kbeg; k0 < s; k0 += GK) {
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+      const int idx = tid + 256 * u, i = idx % TILE, kk = idx / TILE;
+      As[kk][i] = (r0 + i < M && k0 + kk < s) ? ra[u] : zmk(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < NL; ++u) {
+      const int idx = tid + 256 * u, kk = idx % GK, jj = idx / GK;
+      Bs[kk][jj] = (c0 + jj < N && k0 + kk < s) ? rb[u] : zmk(0, 0);
     }
     __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < NB; ++kk) {
+    if (k0 + GK < s) fetch(k0 + GK);
+#pragma unroll 2
+    for (int kk = 0; kk < GK; ++kk) {
       zd a[4], bb[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty + 16 * i];
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][tx + 16 * i];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bb[j] = Bs[kk][tx + 16 * j];
+      for (int j = 0; j < 4; ++j) bb[j] = Bs[kk][ty + 16 * j];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -563,15 +579,15 @@ __global__ __launch_bounds__(256) void k_wz(const int32_t *__restrict__ tasks, F
   }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    int cc = c0 + tx + 16 * j;
-    if (cc >= N) continue;
+    const int cc = c0 + ty + 16 * j;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      int rr = r0 + ty + 16 * i;
-      if (rr >= M) continue;
-      zd v = zmk(-acc[i][j].x, -acc[i][j].y);
-      if (kind) Obw[rr + (int64_t)(s + cc) * s] = v;
-      else Ofw[(s + rr) + (int64_t)cc * f] = v;
+      const int rr = r0 + tx + 16 * i;
+      if (rr < M && cc < N) {
+        const zd v = zmk(-acc[i][j].x, -acc[i][j].y);
+        if (kind) Obw[rr + (int64_t)(s + cc) * s] = v;
+        else Ofw[(s + rr) + (int64_t)cc * f] = v;
+      }
     }
   }
 }
