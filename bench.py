@@ -23,6 +23,7 @@ import json
 import os
 import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -159,8 +160,8 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=2)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--batch', type=int, default=48, help='wavelength points per step')
-    ap.add_argument('--groups', type=int, default=6,
+    ap.add_argument('--batch', type=int, default=64, help='wavelength points per step')
+    ap.add_argument('--groups', type=int, default=8,
                     help='concurrent sub-batches (host threads, each with its own context and stream)')
     ap.add_argument('--ncv', type=int, default=384,
                     help='Krylov dimension (0: the reference rule 2 num_BMs + num_BMs/2 + 3 = 503)')
@@ -168,7 +169,8 @@ def main():
                     help='host threads (contexts) for the layer sub-batches (0: groups + 1); each context '
                          'holds ~40 GB at 8 points per sub-batch, and 16 host cores serve the Schur steps')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline sample')
-    ap.add_argument('--workdir', default=os.path.join(ROOT, 'gpurun_out', 'bench_meshes'))
+    ap.add_argument('--workdir', default=os.path.join(tempfile.gettempdir(), 'emustack_amd_bench_meshes'),
+                    help='where the seeded bench meshes are written (regenerated when missing)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU; the workload is independent wavelength points and '
