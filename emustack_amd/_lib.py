@@ -39,6 +39,7 @@ EXPORTS = [
     'emu_mesh_analyze', 'emu_mesh_symbolic', 'emu_mesh_free', 'emu_mesh_info', 'emu_mesh_arrays',
     'emu_mesh_geometry', 'emu_calc_modes_2d', 'emu_calc_modes_2d_batch', 'emu_assemble', 'emu_lu_solve',
     'emu_profile', 'emu_mesh_create_1d', 'emu_calc_modes_1d', 'emu_calc_modes_1d_batch', 'emu_star_cascade_batch',
+    'emu_material_index',
 ]
 
 
@@ -142,6 +143,7 @@ def lib():
         L.emu_calc_modes_1d_batch.argtypes = [c_vp, c_vp, c_i64, c_dp, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp,
                                               c_i64, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp,
                                               c_i64p]
+        L.emu_material_index.argtypes = [c_vp, c_i64, c_i64p, c_i64p, c_dp, c_dp, c_i64, c_i64p, c_dp, c_dp]
         for f in EXPORTS:
             getattr(L, f)
         rc = L.emu_set_lapack(*_lapack_pointers())
@@ -258,7 +260,8 @@ class Context:
         """Batched star-product cascade (emu_star_cascade_batch).  Per layer
         (bottom to top) arrays with a leading batch axis: kz[l] (nb, dim),
         Z[l] (nb, 2P) for films, J[l] (nb, 2P, modes), Jd[l] (nb, modes, 2P)
-        for structured layers (None elsewhere).  Returns a_list (nb, L-1),
+        for structured layers (None elsewhere); heights (nb, L) over the
+        period (or (L,) shared by every stack).  Returns a_list (nb, L-1),
         r_list (nb,), t_list (nb, L-1) and, if asked, R_net, T_net."""
         L = len(kinds)
         nb = np.asarray(Zair).shape[0]
@@ -285,7 +288,12 @@ class Context:
         Jd_p = arr([fptr(j) for j in Jd])
         kinds = np.asarray(kinds, np.int64)
         nmodes = np.array([np.asarray(k).shape[1] for k in kz], np.int64)
-        heights = np.ascontiguousarray(heights, np.float64)
+        heights = np.asarray(heights, np.float64)
+        if heights.ndim == 1:
+            heights = np.broadcast_to(heights, (nb, L))
+        if heights.shape != (nb, L):
+            raise ValueError('heights must have shape (nbatch, nlayers) or (nlayers,)')
+        heights = np.ascontiguousarray(heights)
         nprop = np.ascontiguousarray(nprop, np.int64)
         a_l = np.zeros((nb, L - 1))
         r_l = np.zeros(nb)
@@ -300,6 +308,20 @@ class Context:
         if want_nets:  # stored per point column-major
             R, T = R.transpose(0, 2, 1), T.transpose(0, 2, 1)
         return a_l, r_l, t_l, R, T
+
+    def material_index(self, kind, offset, xs, ys, qmat, qwl):
+        """emu_material_index: complex n of every query (qmat[q], qwl[q])."""
+        kind = np.ascontiguousarray(kind, np.int64)
+        offset = np.ascontiguousarray(offset, np.int64)
+        xs = np.ascontiguousarray(xs, np.float64)
+        ys = np.ascontiguousarray(ys, np.complex128)
+        qmat = np.ascontiguousarray(qmat, np.int64)
+        qwl = np.ascontiguousarray(qwl, np.float64)
+        out = np.zeros(qmat.size, np.complex128)
+        check(lib().emu_material_index(self.h, kind.size, ptr(kind, c_i64p), ptr(offset, c_i64p),
+                                       ptr(xs) if xs.size else None, ptr(ys.view(np.float64)) if ys.size else None,
+                                       qmat.size, ptr(qmat, c_i64p), ptr(qwl), ptr(out.view(np.float64))))
+        return out
 
     PROFILE_CLASSES = ('solve_fwd', 'solve_bwd', 'gs_project', 'gs_update', 'spmv_M')
 

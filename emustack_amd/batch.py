@@ -183,7 +183,11 @@ def calc_scat_many(stacks, pol='TE', want_nets=False):
             Z.append(None)
             J.append(np.array([np.asarray(l.J) for l in lays]))
             Jd.append(np.array([np.asarray(l.J_dag) for l in lays]))
-    heights = [0.0] + [h / first.period for h in first.heights_nm()] + [0.0]
+    for st in stacks:
+        if st.period != first.period or st.period_y != first.period_y:
+            raise ValueError("all stacks of a batch need the same period")
+    # heights may differ from stack to stack (a sweep over layer thickness)
+    heights = np.array([[0.0] + [h / st.period for h in st.heights_nm()] + [0.0] for st in stacks])
     top = [st.layers[-1] for st in stacks]
     Zair = np.array([t.air_ref().Z() for t in top])
     d_minus = np.array([np.asarray(t.specular_incidence(pol)).reshape(-1) for t in top])

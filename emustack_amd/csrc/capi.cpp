@@ -300,6 +300,19 @@ int emu_star_cascade_batch(emu_ctx *ctx, int64_t nbatch, int64_t nlayers, int64_
   })
 }
 
+int emu_material_index(emu_ctx *ctx, int64_t nmat, const int64_t *kind, const int64_t *offset,
+                       const double *xs, const double *ys, int64_t nq, const int64_t *material,
+                       const double *wl_nm, double *n_out) {
+  if (!ctx || !kind || !offset || (nq > 0 && (!material || !wl_nm || !n_out)))
+    return fail(EMU_E_ARG, "emu_material_index: NULL argument");
+  if (nmat < 1 || nq < 0) return fail(EMU_E_ARG, "emu_material_index: invalid size");
+  if (offset[0] != 0) return fail(EMU_E_ARG, "emu_material_index: offset[0] must be 0");
+  for (int64_t m = 0; m < nmat; ++m)
+    if (offset[m + 1] < offset[m]) return fail(EMU_E_ARG, "emu_material_index: offsets must not decrease");
+  if (offset[nmat] > 0 && (!xs || !ys)) return fail(EMU_E_ARG, "emu_material_index: NULL table");
+  EMU_GUARD(emu::material_index(ctx->c, nmat, kind, offset, xs, ys, nq, material, wl_nm, n_out))
+}
+
 int emu_assemble(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, int64_t nb_typ_el, const double *pp,
                  const double *qq, const double *shift, const double *bloch, double *mat1, double *mat2) {
   if (!ctx || !mesh || !pp || !qq || !shift || !bloch) return fail(EMU_E_ARG, "emu_assemble: NULL argument");

@@ -134,11 +134,14 @@ __global__ void k_film_rt(int d, const zd *__restrict__ Z1, const zd *__restrict
   T21[o] = t;
 }
 
-// propagation phases exp(i kz h) per point
-__global__ void k_prop(int d, const zd *__restrict__ kz, double h, zd *__restrict__ P) {
+// propagation phases exp(i kz h) per point; h = hts[b * L + layer], the
+// layer's height (over the period) in stack b
+__global__ void k_prop(int d, const zd *__restrict__ kz, const double *__restrict__ hts, int L, int layer,
+                       zd *__restrict__ P) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (i >= d) return;
+  const double h = hts[(int64_t)b * L + layer];
   const zd k = kz[(int64_t)b * d + i];
   // exp(i (kr + i ki) h) = exp(-ki h) (cos kr h + i sin kr h)
   const double m = exp(-k.y * h);
@@ -211,6 +214,8 @@ void star_cascade(Ctx *c, const CascadeArgs &a) {
   };
   const zd one = zmk(1, 0), zero = zmk(0, 0);
   zd *Zair = up("cs_zair", a.Zair, (size_t)nb * d);
+  double *hts = c->buf<double>("cs_heights", (size_t)nb * L);
+  EMU_HIP(hipMemcpyAsync(hts, a.heights, sizeof(double) * nb * L, hipMemcpyHostToDevice, s));
   // 1. interface matrices of every layer against air
   std::vector<Lay> lay(L);
   for (int i = 0; i < L; ++i) {
@@ -226,7 +231,7 @@ void star_cascade(Ctx *c, const CascadeArgs &a) {
     l.P = c->buf<zd>(t + "p", (size_t)nb * m);
     zd *kz = up(t + "kz", a.kz[i], (size_t)nb * m);
     if (i > 0 && i < L - 1) {
-      hipLaunchKernelGGL(k_prop, dim3((m + 255) / 256, nb), dim3(256), 0, s, m, kz, a.heights[i], l.P);
+      hipLaunchKernelGGL(k_prop, dim3((m + 255) / 256, nb), dim3(256), 0, s, m, kz, hts, L, i, l.P);
       EMU_LAUNCH_CHECK();
     }
     if (l.kind == 0) {

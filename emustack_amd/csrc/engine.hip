@@ -511,7 +511,7 @@ static void ks_restart_blocks(KSState &S, int m, int bq, int keep, int ld, int64
 
 // Schur-decompose H[:m,:m], order the `keep` largest |theta| first and the
 // `nev` largest among them first of all, test the residuals
-// ||B Q y_i|| <= tol |theta_i| of the leading nev Ritz pairs (B = last bq rows
+// ||B Q y_i|| <= tol max(eps^(2/3), |theta_i|) of the leading nev Ritz pairs (B = last bq rows
 // of H).  Optionally return W = Q[:, :nev] Y (Ritz vectors in the basis),
 // always or only when converged.
 //
@@ -521,6 +521,11 @@ static void ks_restart_blocks(KSState &S, int m, int bq, int keep, int ld, int64
 // Schur decomposition (the expensive O(m^3) host step); T_LA = X Q_AA.  Schur
 // vectors of the wanted set whose own residual ||B q_i|| is below tol |theta_i|
 // (a leading run) are locked for the next cycle.
+// ARPACK's convergence scale (znconv): a Ritz value's residual is measured
+// against max(eps^(2/3), |theta|), so values near zero are not held to an
+// unreachable relative bound.
+static const double kEps23 = 3.6668528625e-11;  // (2^-53)^(2/3)
+
 static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol, std::vector<cplx> *ritzvec,
                        double *maxres, bool ritz_always = true) {
   const int ld = m + bq;
@@ -607,7 +612,7 @@ static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol
   for (int k = 0; k < nev; ++k) {
     double r2 = 0;
     for (int r = 0; r < bq; ++r) r2 += std::norm(BQ[r + (size_t)k * bq]);
-    if (!(std::sqrt(r2) <= tol * std::abs(st.theta[k]))) break;
+    if (!(std::sqrt(r2) <= tol * std::max(kEps23, std::abs(st.theta[k])))) break;
     st.nlock_next = k + 1;
   }
   bool conv = true;
@@ -620,7 +625,7 @@ static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol
       for (int j = 0; j <= k; ++j) acc += BQ[r + (size_t)j * bq] * VR[j + (size_t)k * nev];
       res2 += std::norm(acc);
     }
-    double rel = std::sqrt(res2 / ny) / std::max(std::abs(st.theta[k]), 1e-300);
+    double rel = std::sqrt(res2 / ny) / std::max(std::abs(st.theta[k]), kEps23);
     worst = std::max(worst, rel);
     if (!(rel <= tol)) conv = false;
   }

@@ -45,7 +45,7 @@ struct ModesArgs {
   const double *n_eff = nullptr;   // nl x ntyp complex
   const double *bloch = nullptr;   // nl x 2
   const double *shift = nullptr;   // nl complex
-  double tol = 1e-12;
+  double tol = 1.1102230246251565e-16;  // 2^-53: ARPACK's tol = 0 (machine precision)
   int64_t block = 8;               // Krylov block size (1, 2, 4 or 8)
   int64_t ncv = 0;                 // Krylov dimension (0: the reference's 2 nval + nval/2 + 3)
   double *beta = nullptr;          // nl x nval complex
@@ -73,7 +73,7 @@ struct CascadeArgs {
   const double *const *Z = nullptr;  // per film layer: nbatch x 2neq_pw impedances
   const double *const *J = nullptr;  // per structured layer: nbatch x (2neq_pw x modes), F-order
   const double *const *Jd = nullptr; // per structured layer: nbatch x (modes x 2neq_pw)
-  const double *heights = nullptr;   // per layer, normalised by the period
+  const double *heights = nullptr;   // [nbatch * nlayers]: per stack and layer, over the period
   const double *Zair = nullptr;      // nbatch x 2neq_pw: air reference impedances
   const double *d_minus = nullptr;   // nbatch x 2neq_pw incident amplitudes (top)
   const int64_t *nprop = nullptr;    // nbatch x 3: propagating orders of air, top film, bottom film
@@ -81,5 +81,9 @@ struct CascadeArgs {
   double *a_list = nullptr, *r_list = nullptr, *t_list = nullptr;  // nbatch x (nlayers-1), nbatch, nbatch x (nlayers-1)
 };
 void star_cascade(Ctx *c, const CascadeArgs &a);
+
+// n(lambda) of nq (material, wavelength) queries (kernels_material.hip)
+void material_index(Ctx *c, int64_t nmat, const int64_t *kind, const int64_t *off, const double *xs,
+                    const double *ys, int64_t nq, const int64_t *qm, const double *qw, double *out);
 
 }  // namespace emu

@@ -174,14 +174,21 @@ def ellipse_cell_ntri(period, period_y, a_nm, b_nm, ntri, rotation_deg=0.0, seed
 
 # --------------------------------------------------------------- 1D gratings
 
-def grating_1d(period, diameters, lc_bkg, small_space=None, split_touching_incs=False):
+def grating_1d(period, diameters, lc_bkg, small_space=None, split_touching_incs=False, inclusion_edge='code'):
     """Uniform P2 mesh of a 1D grating cell (objects.py:1054-1530 of the
     reference, one or two inclusions): nel = round(1/lc_bkg) elements on
     [0, 1]; nodes 2i-1 (left), 2i (mid), 2i+1 (right) of element i; the
     material type of each element follows from its end points exactly as the
     reference decides it (inclusion a centred, inclusion b at the cell edges
     or touching a).  Returns x_arr (2 nel + 1,), table_nod (3, nel) 1-based,
-    type_el (nel,) 1-based and the reference's mesh name."""
+    type_el (nel,) 1-based and the reference's mesh name.
+
+    inclusion_edge: 'code' applies the reference's current test for the
+    centre inclusion (objects.py:1502-1503: an element whose left end lies up
+    to one element width left of 0.5 - r still counts as inclusion); 'exact'
+    takes only the elements inside [0.5 - r, 0.5 + r], the rule with which
+    the reference's stacked-gratings golden R/T files (tests/ref/case_4)
+    were generated."""
     ds = [d for d in diameters if d > 0]
     if not ds:
         raise ValueError("Must have at least one grating of nonzero width.")
@@ -204,10 +211,12 @@ def grating_1d(period, diameters, lc_bkg, small_space=None, split_touching_incs=
     if len(ds) > 1:
         small = 0.5 - r1 - r2 if small_space is None else small_space
         large = 0.5 - r1 - r2 if small_space is None else 1.0 - small_space - 2 * r1 - 2 * r2
+    if inclusion_edge not in ('code', 'exact'):
+        raise ValueError("inclusion_edge must be 'code' or 'exact'")
     type_el = np.ones(nel, dtype=np.int64)
     for i in range(nel):
         x1, x2 = x[2 * i], x[2 * i + 2]
-        dx = x2 - x1
+        dx = x2 - x1 if inclusion_edge == 'code' else 0.0
         if (x1 <= 0.5 + r1 and 0.5 - r1 - dx <= x1 and x2 <= 0.5 + r1 and 0.5 - r1 - dx <= x2):
             type_el[i] = 2
         elif len(ds) == 2 and split_touching_incs:

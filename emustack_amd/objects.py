@@ -37,7 +37,7 @@ class NanoStruct(object):
                  xshift=None, make_mesh_now=True, force_mesh=True, mesh_file='NEED_FILE.mail', geo_file=None,
                  geo_params=None, lc_bkg=0.09, lc2=1.0, lc3=1.0, lc4=1.0, lc5=1.0, lc6=1.0, nb_typ_el=None,
                  plotting_fields=False, plot_real=1, plot_imag=0, plot_abs=0, plot_field_conc=False,
-                 plt_msh=True, rotation_deg=0.0, mesh_seed=0):
+                 plt_msh=True, rotation_deg=0.0, mesh_seed=0, inclusion_edge='code'):
         self.periodicity = periodicity
         self.period = float(period)
         self.diameter1 = diameter1
@@ -67,6 +67,9 @@ class NanoStruct(object):
         self.ellipticity = ellipticity
         self.rotation_deg = rotation_deg
         self.mesh_seed = mesh_seed
+        # 1D gratings: 'code' = the reference's current element rule for the
+        # centre inclusion, 'exact' = elements inside [0.5 - r, 0.5 + r] only
+        self.inclusion_edge = inclusion_edge
         if ellipticity > 1.0:
             raise ValueError("ellipticity must be less than 1.0")
         if nb_typ_el is None:  # objects.py:296-304
@@ -128,7 +131,7 @@ class NanoStruct(object):
             from . import mesher
             ds = [self.diameter1] + [getattr(self, 'diameter%d' % i) for i in range(2, 7)]
             x, table_nod, type_el, name = mesher.grating_1d(self.period, ds, self.lc, self.small_space,
-                                                            self.split_touching_incs)
+                                                            self.split_touching_incs, self.inclusion_edge)
             self.n_msh_el = type_el.size
             self.n_msh_pts = x.size
             self.table_nod = table_nod

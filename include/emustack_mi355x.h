@@ -106,8 +106,10 @@ int emu_calc_modes_2d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval,
  * instance-major: lambda[nbatch], n_eff[nbatch*2*nb_typ_el],
  * bloch_vec[nbatch*2], shift[nbatch*2]; outputs beta1[nbatch*2*nval],
  * overlap_J[nbatch*2*2neq_PW*nval], overlap_J_dagger[nbatch*2*nval*2neq_PW].
- * opts (may be NULL; 4 entries): opts[0] = eigen tolerance (<= 0: default
- * 1e-12); opts[1] = precision (0 = complex128 everywhere); opts[2] = Krylov
+ * opts (may be NULL; 4 entries): opts[0] = eigen tolerance: a Ritz pair is
+ * converged when its residual estimate is <= tol max(eps^(2/3), |mu|), the
+ * test of ARPACK's znconv (<= 0: machine precision 2^-53, as the reference's
+ * tol = 0, py_calc_modes.f:174 / zarpack.f:527); opts[1] = precision (0 = complex128 everywhere); opts[2] = Krylov
  * block size (1, 2, 4 or 8; <= 0: default 8); opts[3] = Krylov dimension
  * (<= 0: the reference's 2 nval + nval/2 + 3, rounded up to whole blocks).  stats[0..7] (may be NULL): OP applications,
  * restarts (max over instances), pivot perturbations, Arnoldi steps,
@@ -166,7 +168,8 @@ int emu_calc_modes_1d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const 
  * impedances, Anallo.Z; NULL for structured), J[l] / J_dagger[l] (structured:
  * nbatch x (2neq_PW x nmodes) and nbatch x (nmodes x 2neq_PW), F-order,
  * J already divided by area_norm as Simmo.J holds it; NULL for films),
- * heights[l] (height / period; ignored for layers 0 and nlayers-1).
+ * heights[b * nlayers + l] (height / period of layer l in stack b; the
+ * entries of layers 0 and nlayers-1 are ignored).
  * Zair[nbatch*2neq_PW]: impedances of the air reference; d_minus[nbatch *
  * 2neq_PW]: incident amplitudes in the top layer (specular_incidence);
  * nprop[nbatch*3]: propagating orders per polarisation of the air
@@ -179,6 +182,19 @@ int emu_star_cascade_batch(emu_ctx *ctx, int64_t nbatch, int64_t nlayers, int64_
                            const double *const *J, const double *const *J_dagger, const double *heights,
                            const double *Zair, const double *d_minus, const int64_t *nprop, double *R_net,
                            double *T_net, double *a_list, double *r_list, double *t_list);
+
+/* Refractive indices of nq (material, wavelength) queries in one launch
+ * (replaces Material.n of backend/materials.py:129-170 evaluated once per
+ * wavelength point: linear interpolation of tables with interp1d's segment
+ * choice, Drude n = sqrt(eps_inf - wp^2/(w^2 + i wg w)), w = 2 pi c/lambda).
+ * Material m (kind[m]: 0 constant, 1 table, 2 Drude) owns entries
+ * offset[m] .. offset[m+1]-1 of xs (table wavelengths in nm, increasing;
+ * Drude: wp, wg, eps_inf in rad/s) and ys (complex indices, interleaved;
+ * constant: ys[offset[m]] is the index).  Queries: material[q], wl_nm[q]
+ * (inside the table's range; the caller checks).  n_out[2*nq] complex. */
+int emu_material_index(emu_ctx *ctx, int64_t nmat, const int64_t *kind, const int64_t *offset,
+                       const double *xs, const double *ys, int64_t nq, const int64_t *material,
+                       const double *wl_nm, double *n_out);
 
 /* Live kernel timing with HIP events on the context's stream (benchmarks).
  * enable = 1 resets the counters and starts recording every `every`-th

@@ -4,10 +4,13 @@
 max_order_PWs 6, 46 Bloch modes, 200 elements).  Tolerances: beta 1e-9
 relative; J J^dagger (mode-subspace projector, invariant to the per-mode
 normalisation) 1e-7 of its largest entry; R/T/A 1e-8 against the oracle; the
-2D embedding (world_1d = 0) equal to the oracle's."""
+2D embedding (world_1d = 0) equal to the oracle's; and the reference's seven
+case-4 golden R/T/A files at 1e-8 through the drop-in objects, with the
+element rule those files were generated with (inclusion_edge='exact')."""
 import numpy as np
 import pytest
 
+from conftest import load_txt
 from oracle import fem1d, stack_ref
 
 pytestmark = pytest.mark.gpu
@@ -98,21 +101,41 @@ def test_batched_wavelengths_equal_single(gpu_ctx, gratings):
         np.testing.assert_allclose(bb[0][i], b1[0][0], rtol=1e-9)
 
 
-def test_dropin_api_case4(gratings):
-    """tests/test_case_4_stacked_gratings.py through the drop-in objects,
-    against the oracle (the reference's case-4 text files are not reproduced
-    by the restatement; see tests/test_oracle_1d.py)."""
+def _dropin_case4(edge):
+    """tests/test_case_4_stacked_gratings.py:44-81 of the reference through
+    the drop-in objects (NanoStruct('1D_array'), Stack.calc_scat)."""
     from emustack_amd import objects, materials, Stack
     light = objects.Light(WL, max_order_PWs=ORDER, theta=0.0, phi=0.0)
     sup = objects.ThinFilm(PERIOD, height_nm='semi_inf', world_1d=True, material=materials.Air, loss=False)
     sub = objects.ThinFilm(PERIOD, height_nm='semi_inf', world_1d=True, material=materials.Air, loss=False)
     g1 = objects.NanoStruct('1D_array', PERIOD, diameter1=190, diameter2=190, height_nm=150,
                             inclusion_a=materials.Material(1.46 + 0.0j), inclusion_b=materials.Material(1.46 + 0.0j),
-                            background=materials.Material(3.61 + 0.0j), loss=True, lc_bkg=0.005)
+                            background=materials.Material(3.61 + 0.0j), loss=True, lc_bkg=0.005,
+                            inclusion_edge=edge)
     g2 = objects.NanoStruct('1D_array', PERIOD, 190, height_nm=900, background=materials.Material(3.61 + 0.0j),
-                            inclusion_a=materials.Material(1.46 + 0.0j), loss=True, lc_bkg=0.005)
+                            inclusion_a=materials.Material(1.46 + 0.0j), loss=True, lc_bkg=0.005,
+                            inclusion_edge=edge)
     st = Stack((sub.calc_modes(light), g1.calc_modes(light), g2.calc_modes(light), sup.calc_modes(light)))
     st.calc_scat(pol='TE')
+    return st, light
+
+
+def test_dropin_case4_golden():
+    """All seven case-4 golden files (Reflectance, Transmittance,
+    Absorptance, Lay_Trans_0/1, Lay_Absorb_0/1) at 1e-8 absolute."""
+    st, _ = _dropin_case4('exact')
+    files = {'Reflectance': ('r', -1), 'Transmittance': ('t', -1), 'Absorptance': ('a', -1),
+             'Lay_Trans_0': ('t', 0), 'Lay_Trans_1': ('t', 1), 'Lay_Absorb_0': ('a', 0), 'Lay_Absorb_1': ('a', 1)}
+    for name, (q, i) in files.items():
+        ref = load_txt('case_4', name + '_stack0001.txt')[1]
+        v = float(getattr(st, q + '_list')[i])
+        assert abs(v - ref) < 1e-8, (name, v, ref)
+
+
+def test_dropin_api_case4(gratings):
+    """The reference's current element rule (inclusion_edge='code') through
+    the drop-in objects, against the oracle on the same element types."""
+    st, light = _dropin_case4('code')
     air = dict(kind='tf', n=1.0, h=None)
     lays = [dict(kind='ns', kz=gratings[n][1][0], J=gratings[n][1][1], Jd=gratings[n][1][2], h=h / PERIOD)
             for n, h in (('g1', 150.0), ('g2', 900.0))]

@@ -87,3 +87,51 @@ def test_case1_through_gpu_cascade(gpu_ctx):
         s.calc_scat(pol='TE')
         np.testing.assert_allclose(got, (float(s.a_list[-1]), float(s.r_list[-1]), float(s.t_list[-1])),
                                    rtol=0, atol=1e-10)
+
+
+def test_heights_sweep_with_shared_layers(gpu_ctx):
+    """Stacks built from the same layer objects with different heights_nm
+    (a thickness sweep): every stack keeps its own heights in the batched
+    cascade.  Against Stack.calc_scat at 1e-10."""
+    from emustack_amd import Stack
+    from emustack_amd.batch import calc_scat_many
+    base = _stacks(wls=(600.0,))[0]
+    sweeps = ([200, 50, 150, 50, 100], [120, 80, 60, 30, 240], [10, 5, 300, 5, 20])
+    stacks = [Stack(base.layers, heights_nm=h) for h in sweeps]
+    ref = []
+    for st in stacks:
+        st.calc_scat(pol='TE')
+        ref.append((float(st.a_list[-1]), float(st.r_list[-1]), float(st.t_list[-1])))
+    calc_scat_many(stacks, pol='TE')
+    got = [(float(st.a_list[-1]), float(st.r_list[-1]), float(st.t_list[-1])) for st in stacks]
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-10)
+    assert abs(ref[0][1] - ref[2][1]) > 1e-3  # the heights matter
+
+
+def test_case0_films_through_gpu_cascade(gpu_ctx, mats):
+    """The reference's case 0 (tests/test_case_0_thin_film_multistack.py:
+    five analytic films, period 1 nm, max_order_PWs 1, TM, 400/700/1000 nm)
+    through the batched GPU cascade, against its golden files at 1e-8."""
+    from conftest import load_txt
+    from emustack_amd import objects, materials, Stack
+    from emustack_amd.batch import calc_scat_many
+    stacks = []
+    for wl in (400.0, 700.0, 1000.0):
+        light = objects.Light(wl, max_order_PWs=1, theta=0.0, phi=0.0)
+        sup = objects.ThinFilm(period=1, height_nm='semi_inf', material=materials.Material(3.5 + 0.0j), loss=False)
+        f1 = objects.ThinFilm(period=1, height_nm=50, material=materials.Material(3.6 + 0.27j), loss=True)
+        f2 = objects.ThinFilm(period=1, height_nm=200, material=materials.Material(mats('Si_c', wl)), loss=True)
+        mirror = objects.ThinFilm(period=1, height_nm=100, material=materials.Material(mats('Ag', wl)), loss=True)
+        sub = objects.ThinFilm(period=1, height_nm='semi_inf', material=materials.Air, loss=False)
+        s1, s2 = f1.calc_modes(light), f2.calc_modes(light)
+        stacks.append(Stack((sub.calc_modes(light), mirror.calc_modes(light), s1, s2, s1, s2,
+                             sup.calc_modes(light))))
+    calc_scat_many(stacks, pol='TM')
+    checks = {'Absorptance': lambda s: s.a_list[-1], 'Reflectance': lambda s: s.r_list[-1],
+              'Transmittance': lambda s: s.t_list[-1]}
+    for i in range(5):
+        checks['Lay_Absorb_%d' % i] = (lambda i: lambda s: s.a_list[i])(i)
+    for name, get in checks.items():
+        ref = load_txt('case_0', name + '_stack0001.txt')[:, 1]
+        got = np.array([float(np.real(get(s))) for s in stacks])
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-8, err_msg=name)

@@ -1,0 +1,69 @@
+"""Material library (emustack_amd.materials): the reference's named
+tabulated materials, linear interpolation with interp1d's segment choice,
+Drude metals (reference backend/materials.py:129-330), and the batched GPU
+evaluation n_batch (emu_material_index).
+
+Pins: the committed fixture tests/golden/materials.json (indices at the
+wavelengths of the reference's test cases, interpolated from the
+reference's tables), scipy's interp1d on the packed tables (the routine the
+reference calls), and the Drude closed form at a wavelength where it
+reduces by hand (omega_gamma = 0: n^2 = eps_inf - (lambda/lambda_p)^2)."""
+import numpy as np
+import pytest
+from scipy.interpolate import interp1d
+
+from emustack_amd import materials
+
+
+def test_named_materials_match_fixture(mats):
+    for name, wls in (('Si_c', (400.0, 500.0, 700.0, 1000.0)), ('SiO2', (500.0, 1000.0)),
+                      ('Ag', (400.0, 700.0, 1000.0)), ('Au', (1043.4,))):
+        m = getattr(materials, name)
+        for wl in wls:
+            assert abs(m.n(wl) - mats(name, wl)) <= 1e-14 * abs(mats(name, wl)), (name, wl)
+
+
+def test_every_named_material_interpolates_like_interp1d():
+    rng = np.random.default_rng(0)
+    names = materials.named_materials()
+    assert len(names) == 59 and {'Si_c', 'Au', 'Ag', 'ITO', 'GaAs', 'Glass'} <= set(names)
+    for name in names:
+        m = getattr(materials, name)
+        f = interp1d(m.data_wls, m.data_ns)
+        wl = np.concatenate([rng.uniform(m.data_wls[0], m.data_wls[-1], 50), m.data_wls[[0, -1]],
+                             m.data_wls[1:-1:7]])
+        np.testing.assert_allclose(m.n(wl), f(wl), rtol=1e-14, atol=1e-300, err_msg=name)
+        with pytest.raises(ValueError):
+            m.n(m.data_wls[-1] * 1.01 + 1.0)
+
+
+def test_unsorted_and_repeated_tables_sorted_like_interp1d():
+    t = np.array([[900.0, 1.5, 0.1], [500.0, 2.0, 0.0], [700.0, 1.8, 0.02], [700.0, 1.9, 0.03], [400.0, 2.2, 0.2]])
+    m = materials.Material(t)
+    f = interp1d(t[:, 0], t[:, 1] + 1j * t[:, 2])
+    wl = np.linspace(400, 900, 101)
+    np.testing.assert_allclose(m.n(wl), f(wl), rtol=1e-15)
+
+
+def test_drude_reduces_to_plasma_formula():
+    wp = 1.36e16
+    m = materials.Material([wp, 0.0, 9.5])
+    lam_p = 2 * np.pi * 299792458.0 / wp * 1e9
+    for wl in (300.0, 1500.0):
+        assert abs(m.n(wl) ** 2 - (9.5 - (wl / lam_p) ** 2)) < 1e-12 * abs(m.n(wl)) ** 2
+    n = materials.Au_drude.n(np.array([500.0, 1000.0]))
+    assert np.all(n.imag > 0)
+
+
+@pytest.mark.gpu
+def test_n_batch_on_gpu_matches_host(gpu_ctx):
+    ms = [materials.Air, materials.Si_c, materials.Au, materials.Ag, materials.Material(3.5 + 0.01j),
+          materials.Au_drude, materials.Material([9.0 / 6.582119569e-16, 0.07 / 6.582119569e-16, 1.0]),
+          materials.SiO2, materials.ITO]
+    wl = np.linspace(400.0, 1000.0, 257)
+    got = materials.n_batch(ms, wl, ctx=gpu_ctx)
+    for i, m in enumerate(ms):
+        ref = np.broadcast_to(np.asarray(m.n(wl), complex), wl.shape)
+        np.testing.assert_allclose(got[i], ref, rtol=1e-14, atol=1e-300, err_msg=str(i))
+    with pytest.raises(ValueError):
+        materials.n_batch([materials.Au], [100.0], ctx=gpu_ctx)

@@ -24,12 +24,12 @@ def test_case0_thin_film_multistack(mats):
         R.append(out['r_list'][-1])
         T.append(out['t_list'][-1])
     A = np.array(A, float)
-    np.testing.assert_allclose(A[:, -1], load_txt('case_0', 'Absorptance_stack0001.txt')[:, 1], 1e-6, 1e-6)
-    np.testing.assert_allclose(R, load_txt('case_0', 'Reflectance_stack0001.txt')[:, 1], 1e-6, 1e-6)
-    np.testing.assert_allclose(T, load_txt('case_0', 'Transmittance_stack0001.txt')[:, 1], 1e-6, 1e-6)
+    np.testing.assert_allclose(A[:, -1], load_txt('case_0', 'Absorptance_stack0001.txt')[:, 1], 0, 1e-8)
+    np.testing.assert_allclose(R, load_txt('case_0', 'Reflectance_stack0001.txt')[:, 1], 0, 1e-8)
+    np.testing.assert_allclose(T, load_txt('case_0', 'Transmittance_stack0001.txt')[:, 1], 0, 1e-8)
     for i in range(5):
         ref = load_txt('case_0', 'Lay_Absorb_%d_stack0001.txt' % i)[:, 1]
-        np.testing.assert_allclose(A[:, i], ref, 1e-6, 1e-6)
+        np.testing.assert_allclose(A[:, i], ref, 0, 1e-8)
 
 
 @pytest.fixture(scope='module')
@@ -61,13 +61,13 @@ def test_case1_simple_struct(mesh600, mats):
     for name, vals in got.items():
         fn = name + '_stack0001.txt' if not name.startswith('Lay') else name + '_stack0001.txt'
         ref = load_txt('case_1', fn)[:, 1]
-        np.testing.assert_allclose(np.array(vals, float), ref, 1e-6, 1e-6, err_msg=name)
+        np.testing.assert_allclose(np.array(vals, float), ref, 0, 1e-8, err_msg=name)
 
 
 def test_case2_oblique(mesh600, mats):
     """Same mesh at theta=20, phi=40, 20 Bloch modes, order 1
     (tests/test_case_2_simple_non_norm_inc.py:40-97)."""
     o = _nanowire(mesh600, mats, 500.0, 20, 40, 20, 1, 'TE')
-    np.testing.assert_allclose(o['a_list'][-1], load_txt('case_2', 'Absorptance_stack0001.txt')[1], 1e-6, 1e-6)
-    np.testing.assert_allclose(o['r_list'][-1], load_txt('case_2', 'Reflectance_stack0001.txt')[1], 1e-6, 1e-6)
-    np.testing.assert_allclose(o['t_list'][-1], load_txt('case_2', 'Transmittance_stack0001.txt')[1], 1e-6, 1e-6)
+    np.testing.assert_allclose(o['a_list'][-1], load_txt('case_2', 'Absorptance_stack0001.txt')[1], 0, 1e-8)
+    np.testing.assert_allclose(o['r_list'][-1], load_txt('case_2', 'Reflectance_stack0001.txt')[1], 0, 1e-8)
+    np.testing.assert_allclose(o['t_list'][-1], load_txt('case_2', 'Transmittance_stack0001.txt')[1], 0, 1e-8)
