@@ -13,10 +13,16 @@ normal incidence.  Data are synthetic.
 A step solves the modes of all three layers for a batch of wavelength
 points (prime and adjoint problems, batched library calls per layer) and
 runs the star-product cascade with fluxes for every point (one batched
-cascade on the GPU).  `value` is
-wavelength-points per second.
+cascade on the GPU).  `value` is wavelength-points per second of the
+precision form given by --precision (default: the mixed form: the Krylov
+solves read a complex64 copy of the FP64 inverted factor, and every beta is
+the FP64 two-sided Rayleigh quotient); one step of the other form on the
+same batch gives the FP64/mixed ratio and the R/T/A difference between the
+two, and both are compared with the oracle's FP64 result at the fixture
+wavelength (tests/golden/configs/config3.npz) when the batch holds it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+                    [--precision mixed|fp64] [--no-cpu] [--no-compare]
 """
 import argparse
 import json
@@ -39,16 +45,20 @@ NUM_BMS = 200
 ORDER = 4
 NWL = 512
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FIXTURE = os.path.join(ROOT, 'tests', 'golden', 'configs', 'config3.npz')
 # HBM bytes per dispatch of the kernels that can be the roofline kernel, from
 # two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this workload at 16
 # points in 2 sub-batches (8 points = 16 instances per dispatch, as in the
-# default step), each scaled by what the counter reads for a plain 2 GiB copy.
+# default step), each scaled by what the counter reads for a plain 2 GiB copy;
+# summary and commands in profiles/kernels_stacked_gratings_8k.md.
 TRAFFIC_PER_DISPATCH = {
     'k_atb_mfma16<true>': 8473211998,
     'k_combine<16>': 7324606084,
     'k_fwd': 433422318,
     'k_bwd': 412855311,
 }
+TRAFFIC_SOURCE = 'profiles/kernels_stacked_gratings_8k.md'
 
 
 def inclusion_indices():
@@ -62,7 +72,9 @@ def build_meshes(workdir):
     for i, r in enumerate(RADII):
         p = os.path.join(workdir, 'stacked_r%d_s%d.mail' % (int(r), i + 1))
         if not os.path.exists(p):
-            mesher.write_mail(p, mesher.ellipse_cell_ntri(PERIOD, PERIOD, r, r, NTRI, seed=i + 1))
+            tmp = p + '.%d.tmp' % os.getpid()
+            mesher.write_mail(tmp, mesher.ellipse_cell_ntri(PERIOD, PERIOD, r, r, NTRI, seed=i + 1))
+            os.replace(tmp, p)
         paths.append(p)
     return paths
 
@@ -79,21 +91,61 @@ def layers_for(mesh_paths):
     return ns, film, sub, sup
 
 
-def run_step(ns, film, sub, sup, wls, groups=1, ncv=0, workers=None):
+def run_step(ns, film, sub, sup, wls, groups=1, ncv=0, workers=None, precision='mixed'):
     """All three layers for the wavelength batch `wls` (`groups` concurrent
-    sub-batches per layer, Krylov dimension ncv), then the cascade."""
+    sub-batches per layer, Krylov dimension ncv), then the cascade.  Returns
+    R/T/A per point and the summed solver stats."""
     from emustack_amd import objects, Stack
     from emustack_amd.batch import calc_modes_stack, calc_scat_many
     lights = [objects.Light(wl, max_order_PWs=ORDER, theta=0.0, phi=0.0) for wl in wls]
-    # the three layers' sub-batches share one pool of host threads
-    sims = calc_modes_stack(ns, lights, num_BMs=NUM_BMS, groups=groups, ncv=ncv, workers=workers)
-    ops = int(calc_modes_stack.last_stats[0])
+    sims = calc_modes_stack(ns, lights, num_BMs=NUM_BMS, groups=groups, ncv=ncv, workers=workers,
+                            precision=precision)
+    stats = calc_modes_stack.last_stats.copy()
     stacks = [Stack((sub.calc_modes(light), sims[0][i], film.calc_modes(light), sims[1][i],
                      film.calc_modes(light), sims[2][i], sup.calc_modes(light))) for i, light in enumerate(lights)]
     calc_scat_many(stacks, pol='TE')  # batched star-product cascade on the GPU
-    out = [(float(st.a_list[-1]), float(st.r_list[-1]), float(st.t_list[-1])) for st in stacks]
-    return out, ops
+    out = np.array([(float(st.a_list[-1]), float(st.r_list[-1]), float(st.t_list[-1])) for st in stacks])
+    return out, stats
 
+
+# ---------------------------------------------------------------- roofline
+
+
+def symbolic_flops(mesh_path):
+    """Real FP64 flops of one numeric factorisation and of its inverted form
+    (per front with s pivots and u update rows: LU s f^2 - s^2 f + s^3/3,
+    inversion s^3/3 + u s^2 complex multiply-adds, 8 flops each), and the
+    complex entries of the inverted factor."""
+    from emustack_amd import _lib
+    mh = _lib.MeshHandle(mesh_path)
+    fr = mh.symbolic()['fronts'].reshape(-1, 8).astype(np.float64)
+    s, u, f = fr[:, 0], fr[:, 1], fr[:, 2]
+    lu = 8.0 * np.sum(s * f * f - s * s * f + s ** 3 / 3.0)
+    inv = 8.0 * np.sum(s ** 3 / 3.0 + u * s * s)
+    return lu, inv, mh.neq, mh.nnz
+
+
+def whole_step_roofline(prof, mesh_info, npts, ops_per_inst, m, keep, step_s):
+    """T_roofline / T_measured of the timed region: the profiled kernel
+    classes at their algorithmic bytes over HBM peak, plus the
+    factorisations, triangular inversions, restart and Ritz-vector products
+    at their flop counts over the FP64 peak, plus assembly writes."""
+    bw = HBM_PEAK_GBS * 1e9
+    fp = FP64_PEAK_TFLOPS * 1e12
+    t_classes = sum(v[2] for v in prof.values()) / bw
+    t_flops = 0.0
+    t_asm = 0.0
+    for lu, inv, neq, nnz in mesh_info:
+        inst = 2 * npts  # prime + adjoint per point
+        restarts = max(0.0, (ops_per_inst - m) / max(m - keep, 1))
+        t_flops += inst * (lu + inv + 8.0 * neq * m * keep * restarts + 8.0 * neq * m * NUM_BMS) / fp
+        t_asm += inst * 2 * nnz * 16.0 / bw
+    t_roof = t_classes + t_flops + t_asm
+    return dict(T_roofline_s=t_roof, T_measured_s=step_s, achieved=t_roof / step_s,
+                parts_s=dict(profiled_classes=t_classes, factor_inverse_restart_flops=t_flops, assembly=t_asm))
+
+
+# ------------------------------------------------------------- CPU baseline
 
 CPU_SAMPLE = r'''
 import sys, time, json
@@ -110,13 +162,14 @@ perm = _lib.MeshHandle(mesh_path).arrays()['perm']
 wln = wl / PERIOD; k0 = 2 * np.pi / wln
 n_eff = np.array([1.0, n_incl]); th = np.deg2rad(1e-5)
 kpn = 2 * np.pi / wl * np.sin(th) * np.array([np.cos(th), np.sin(th)]) * PERIOD
-shift = 1.1 * np.real(n_eff).max() ** 2 * k0 ** 2 - kpn @ kpn
+shift = 1.1 * np.real(n_eff).max() ** 2 * k0 ** 2 - kpn[0] ** 2 - kpn[1] ** 2
 t0 = time.perf_counter()
 A, B = fem2d.assemble(m, np.ones(2, complex), n_eff ** 2 * k0 ** 2, shift, kpn)
 t_asm = time.perf_counter() - t0
 Ap = A.tocsr()[perm][:, perm].tocsc(); Bp = B.tocsr()[perm][:, perm].tocsr()
 t0 = time.perf_counter()
-lu = spla.splu(Ap, permc_spec='NATURAL', diag_pivot_thresh=0.0)
+# threshold partial pivoting (SuperLU's default) on the nested-dissection order
+lu = spla.splu(Ap, permc_spec='NATURAL')
 t_lu = time.perf_counter() - t0
 v = np.random.default_rng(0).standard_normal(m.neq) + 0j
 t0 = time.perf_counter()
@@ -135,24 +188,52 @@ print(json.dumps(dict(t_mesh=t_mesh, t_asm=t_asm, t_lu=t_lu, t_op=t_op, t_orth=t
 '''
 
 
-def cpu_baseline(mesh_path, wl, n_incl, ops_per_instance, nops=60):
-    """The oracle (numpy assembly + scipy SuperLU on the nested-dissection
-    order + ARPACK-style OP applications and CGS2 steps) timed on one host
-    core for a bounded sample of one eigensolve; extrapolated to a
-    wavelength point (6 eigensolves) with the GPU's measured OP count."""
+def _cpu_worker(args):
+    code, mesh_path, wl, n_incl, nops = args
     env = dict(os.environ, OMP_NUM_THREADS='1', OPENBLAS_NUM_THREADS='1', MKL_NUM_THREADS='1')
-    code = 'ROOT=%r; PERIOD=%r; NBM=%r\n' % (ROOT, PERIOD, NUM_BMS) + CPU_SAMPLE
     r = subprocess.run([sys.executable, '-c', code, mesh_path, str(wl), str(n_incl), str(nops)],
-                       capture_output=True, text=True, env=env, timeout=600)
+                       capture_output=True, text=True, env=env, timeout=900)
     if r.returncode != 0:
         raise RuntimeError(r.stderr[-2000:])
-    d = json.loads(r.stdout.strip().splitlines()[-1])
-    t_eig = d['t_asm'] + d['t_lu'] + ops_per_instance * (d['t_op'] + d['t_orth'])
-    value = 1.0 / (6.0 * t_eig)
-    sample = ('one eigensolve of layer 1 at %.0f nm on 1 core: assembly %.1f s + SuperLU (ND order) %.1f s + '
-              '%d OP applications x (%.1f ms solve + %.1f ms CGS2) timed over %d OPs; x6 eigensolves per point'
-              % (wl, d['t_asm'], d['t_lu'], ops_per_instance, 1e3 * d['t_op'], 1e3 * d['t_orth'], nops))
-    return dict(value=value, unit='wavelength-points/s', cores=1, kind='port', sample=sample)
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(mesh_paths, wls, n_incls, ops_per_instance, cores, nops=40):
+    """The oracle's FP64 path on the host cores: `cores` single-threaded
+    processes at once, each timing on one layer mesh and wavelength the mesh
+    preprocessing, assembly, SuperLU factorisation (threshold partial
+    pivoting on the nested-dissection order) and `nops` shift-invert OP
+    applications and CGS2 steps of an ARPACK-sized basis; an eigensolve is
+    assembly + factorisation + ops_per_instance x (OP + CGS2), a point 6
+    eigensolves, and the cores work on independent points."""
+    from concurrent.futures import ThreadPoolExecutor
+    code = 'ROOT=%r; PERIOD=%r; NBM=%r\n' % (ROOT, PERIOD, NUM_BMS) + CPU_SAMPLE
+    jobs = [(code, mesh_paths[i % 3], wls[i % len(wls)], n_incls[i % 3], nops) for i in range(cores)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        res = list(ex.map(_cpu_worker, jobs))
+    wall = time.perf_counter() - t0
+    t_eig = np.mean([d['t_asm'] + d['t_lu'] + ops_per_instance * (d['t_op'] + d['t_orth']) for d in res])
+    value = cores / (6.0 * t_eig)
+    d = res[0]
+    sample = ('%d processes at once (one core each, %.0f s wall): per process one layer mesh at one wavelength, '
+              'assembly %.1f s + SuperLU with partial pivoting on the ND order %.1f s + %d OP applications '
+              '(%.1f ms each) and CGS2 steps (%.1f ms each); an eigensolve = assembly + LU + %d x (OP + CGS2) '
+              '(the GPU run\'s OP count), 6 eigensolves per point, points independent across cores'
+              % (cores, wall, d['t_asm'], d['t_lu'], nops, 1e3 * d['t_op'], 1e3 * d['t_orth'], ops_per_instance))
+    return dict(value=value, unit='wavelength-points/s', cores=cores, kind='port', sample=sample)
+
+
+def host_cores():
+    """Cores this process may use (the box grants 16 per GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, int(os.environ.get('OMP_NUM_THREADS', n)), 16))
+
+
+# --------------------------------------------------------------------- main
 
 
 def main():
@@ -168,7 +249,9 @@ def main():
     ap.add_argument('--workers', type=int, default=3,
                     help='host threads (contexts) for the layer sub-batches (0: groups + 1); each context '
                          'holds ~40 GB at 8 points per sub-batch, and 16 host cores serve the Schur steps')
+    ap.add_argument('--precision', choices=('mixed', 'fp64'), default='mixed')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline sample')
+    ap.add_argument('--no-compare', action='store_true', help='skip the step of the other precision form')
     ap.add_argument('--workdir', default=os.path.join(tempfile.gettempdir(), 'emustack_amd_bench_meshes'),
                     help='where the seeded bench meshes are written (regenerated when missing)')
     args = ap.parse_args()
@@ -183,7 +266,7 @@ def main():
     ns, film, sub, sup = layers_for(mesh_paths)
     wavelengths = np.linspace(400.0, 1000.0, NWL)
     from emustack_amd.batch import preload_meshes
-    ctx = EMUstack.context()
+    EMUstack.context()
     for p in mesh_paths:
         EMUstack.mesh(p)
     # one-time setup outside the timed steps: every worker context loads the
@@ -195,55 +278,92 @@ def main():
         nonlocal step_i
         idx = [(step_i * args.batch + i) % NWL for i in range(args.batch)]
         step_i += 1
-        return wavelengths[idx]
+        return idx
+    kw = dict(groups=args.groups, ncv=args.ncv, workers=args.workers)
     for _ in range(args.warmup):
-        wl = next_batch()
-        print('warmup batch %s' % np.round(wl, 1), file=sys.stderr, flush=True)
-        run_step(ns, film, sub, sup, wl, args.groups, args.ncv, args.workers)
+        idx = next_batch()
+        print('warmup batch %s' % np.round(wavelengths[idx], 1), file=sys.stderr, flush=True)
+        run_step(ns, film, sub, sup, wavelengths[idx], precision=args.precision, **kw)
     ctxs = EMUstack.all_contexts()
     for c in ctxs:
         c.profile(True, every=1)
     ops = 0
-    results = []
+    results = {}
     t0 = time.perf_counter()
     for k in range(args.steps):
-        wl = next_batch()
-        res, o = run_step(ns, film, sub, sup, wl, args.groups, args.ncv, args.workers)
-        ops += o
-        results += res
+        idx = next_batch()
+        res, st = run_step(ns, film, sub, sup, wavelengths[idx], precision=args.precision, **kw)
+        ops += int(st[0])
+        for i, r in zip(idx, res):
+            results[i] = r
         print('step %d done (%.1f s)' % (k, time.perf_counter() - t0), file=sys.stderr, flush=True)
     T = time.perf_counter() - t0
     prof = {}
-    for c in ctxs:
+    for c in EMUstack.all_contexts():
         for k, v in c.profile(False).items():
             prof[k] = tuple(a + b for a, b in zip(prof.get(k, (0, 0, 0)), v))
     npts = args.steps * args.batch
     value = npts / T
+    ops_per_inst = ops / (npts * 6.0)
     dom = max(prof, key=lambda k: prof[k][1])
     cnt, ms, byt = prof[dom]
     achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     kernel_name = {'solve_fwd': 'k_fwd', 'solve_bwd': 'k_bwd', 'gs_project': 'k_atb_mfma16<true>',
                    'gs_update': 'k_combine<16>', 'spmv_M': 'k_spmm'}[dom]
+    m = args.ncv or 2 * NUM_BMS + NUM_BMS // 2 + 3
+    m = ((m + 7) // 8) * 8
+    keep = min(m - 8, NUM_BMS + (m - NUM_BMS) // 2)
+    keep = m - ((m - keep + 7) // 8) * 8
+    mesh_info = [symbolic_flops(p) for p in mesh_paths]
+    whole = whole_step_roofline(prof, mesh_info, npts, ops_per_inst, m, keep, T)
     line = {
         'metric': 'wavelength-points/s at 8k-tri mesh; mixed vs FP64 GPU at <=1e-6 R/T/A error',
         'value': value, 'unit': 'wavelength-points/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': 1e3 * T / args.steps, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
-        'dtype': 'c128', 'data': 'synthetic',
+        'dtype': 'c128 (complex64 factor reads)' if args.precision == 'mixed' else 'c128', 'data': 'synthetic',
         'config': {'workload': 'stacked gratings: 3 structured layers (r=100/150/80 nm, ~%d-triangle P2 meshes) '
                                '+ 2 films, period 500 nm, num_BMs=%d, max_order_PWs=%d (98-dim S-matrix), '
                                '%d wavelengths 400-1000 nm, normal incidence' % (NTRI, NUM_BMS, ORDER, NWL),
-                   'batch_wavelengths': args.batch, 'groups': args.groups, 'workers': args.workers or args.groups + 1, 'krylov_dim': args.ncv or None, 'precision': 'fp64 (complex128) everywhere',
-                   'op_applications_per_instance': ops / (npts * 6.0)},
+                   'batch_wavelengths': args.batch, 'groups': args.groups,
+                   'workers': args.workers or args.groups + 1, 'krylov_dim': args.ncv or None,
+                   'precision': args.precision, 'op_applications_per_instance': ops_per_inst},
         'roofline': {'bound': 'hbm', 'kernel': kernel_name, 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': TRAFFIC_PER_DISPATCH.get(kernel_name),
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': TRAFFIC_PER_DISPATCH.get(kernel_name), 'traffic_source': TRAFFIC_SOURCE,
                      'launches_timed': int(cnt), 'avg_launch_us': 1e3 * ms / max(cnt, 1)},
+        'whole_step_roofline': whole,
         'kernel_profile': {k: {'launches': int(v[0]), 'ms': v[1], 'GB': v[2] / 1e9} for k, v in prof.items()},
-        'rta_sample': results[:2],
     }
+    # the other precision form on one batch holding the fixture wavelength
+    fx = np.load(FIXTURE, allow_pickle=False) if os.path.exists(FIXTURE) else None
+    fix_i = int(np.argmin(np.abs(wavelengths - float(fx['wl'][0])))) if fx is not None else 64
+    cmp_idx = [(fix_i + i) % NWL for i in range(args.batch)]
+    if not args.no_compare:
+        other = 'fp64' if args.precision == 'mixed' else 'mixed'
+        mine = np.array([results[i] for i in cmp_idx]) if all(i in results for i in cmp_idx) else \
+            run_step(ns, film, sub, sup, wavelengths[cmp_idx], precision=args.precision, **kw)[0]
+        print('%s step on the comparison batch...' % other, file=sys.stderr, flush=True)
+        t1 = time.perf_counter()
+        theirs, st2 = run_step(ns, film, sub, sup, wavelengths[cmp_idx], precision=other, **kw)
+        t_other = time.perf_counter() - t1
+        by = {args.precision: mine, other: theirs}
+        v_other = args.batch / t_other
+        cmpd = {'other_form': other, 'other_value': v_other, 'other_ms_per_step': 1e3 * t_other,
+                'op_applications_per_instance_other': float(st2[0]) / (args.batch * 6.0),
+                'mixed_over_fp64': (value / v_other) if other == 'fp64' else (v_other / value),
+                'rta_max_abs_diff_mixed_vs_fp64': float(np.abs(by['mixed'] - by['fp64']).max()),
+                'points_compared': args.batch}
+        if fx is not None:
+            ref = np.asarray(fx['rta_cluster'], float)
+            for form in ('mixed', 'fp64'):
+                cmpd['rta_err_vs_oracle_%s' % form] = float(np.abs(by[form][0] - ref).max())
+            cmpd['oracle_point_nm'] = float(fx['wl'][0])
+        line['precision_compare'] = cmpd
     if not args.no_cpu:
-        print('cpu baseline sample...', file=sys.stderr, flush=True)
-        line['cpu_baseline'] = cpu_baseline(mesh_paths[0], float(wavelengths[0]), inclusion_indices()[0],
-                                            max(1, int(round(ops / (npts * 6.0)))))
+        cores = host_cores()
+        print('cpu baseline sample on %d cores...' % cores, file=sys.stderr, flush=True)
+        line['cpu_baseline'] = cpu_baseline(mesh_paths, [float(w) for w in wavelengths[cmp_idx[:4]]],
+                                            inclusion_indices(), max(1, int(round(ops_per_inst))), cores)
     else:
         line['cpu_baseline'] = None
     print(json.dumps(line))

@@ -131,12 +131,13 @@ def calc_modes_1d(lambda_, nval, ordre_ls, nb_typ_el, npt_P2, nel, table_nod, ty
 
 
 def calc_modes_2d_batch(lambdas, nval, ordre_ls, mesh_path, n_effs, bloch_vecs, shifts, neq_PW, E_H_field=1,
-                        itermax=30, tol=0.0, ncv=0):
+                        itermax=30, tol=0.0, ncv=0, precision=0):
     """Batched form: many (lambda, n_eff, bloch_vec, shift) points on one mesh
     solved together.  Returns beta [nl, nval], J [nl, 2P, nval],
     J_dagger [nl, nval, 2P] and the stats vector.  ncv: Krylov dimension
-    (0: the reference's rule 2 nval + nval/2 + 3)."""
+    (0: the reference's rule 2 nval + nval/2 + 3); precision 0 (complex128)
+    or 1 (mixed: complex64 factor in the Krylov solves, FP64 beta)."""
     h = mesh(mesh_path)
     return context().calc_modes_batch(h, lambdas, int(nval), int(ordre_ls), n_effs, bloch_vecs, shifts,
                                       int(neq_PW), E_H_field=int(E_H_field), itermax=int(itermax), tol=tol,
-                                      ncv=int(ncv))
+                                      ncv=int(ncv), precision=int(precision))

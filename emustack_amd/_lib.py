@@ -206,8 +206,10 @@ class Context:
         return x, int(st[0])
 
     def calc_modes_batch(self, mesh, lam, nval, ordre, n_eff, bloch, shift, neq_pw, E_H_field=1, itermax=30,
-                         tol=0.0, block=0, ncv=0):
-        """Batched calc_modes_2d: returns beta [nl, nval], J [nl, 2P, nval], Jd [nl, nval, 2P], stats."""
+                         tol=0.0, block=0, ncv=0, precision=0):
+        """Batched calc_modes_2d: returns beta [nl, nval], J [nl, 2P, nval], Jd [nl, nval, 2P], stats.
+        precision 0: complex128 throughout; 1: mixed (complex64 factor in the
+        Krylov solves, FP64 two-sided Rayleigh-quotient beta)."""
         lam = np.ascontiguousarray(np.atleast_1d(np.asarray(lam, np.float64)))
         nl = lam.shape[0]
         n_eff = np.ascontiguousarray(np.asarray(n_eff, np.complex128).reshape(nl, -1))
@@ -217,7 +219,7 @@ class Context:
         beta = np.zeros((nl, nval), np.complex128)
         J = np.zeros((nl, nval, 2 * neq_pw), np.complex128)    # C-order view of F-order (2P x nval)
         Jd = np.zeros((nl, 2 * neq_pw, nval), np.complex128)   # C-order view of F-order (nval x 2P)
-        opts = np.array([tol, 0.0, float(block), float(ncv)])
+        opts = np.array([tol, float(precision), float(block), float(ncv)])
         st = np.zeros(8, np.int64)
         with _host_lapack_threads(nl):
             check(lib().emu_calc_modes_2d_batch(self.h, mesh.h, nl, ptr(lam), nval, ordre, ntyp,
@@ -228,7 +230,7 @@ class Context:
         return beta, J.transpose(0, 2, 1), Jd.transpose(0, 2, 1), st
 
     def calc_modes_1d_batch(self, mesh, lam, nval, ordre, n_eff, bloch, shift, neq_pw, neq_pw_2d=0,
-                            world_1d=True, itermax=30, tol=0.0, block=0, ncv=0):
+                            world_1d=True, itermax=30, tol=0.0, block=0, ncv=0, precision=0):
         """Batched calc_modes_1d on one MeshHandle1D.  Returns beta (nl, nval),
         J (nl, 2P, nval), Jd (nl, nval, 2P), J2d, Jd2d (None for a 1D world)
         and stats."""
@@ -244,7 +246,7 @@ class Context:
         w2 = 0 if world_1d else 1
         J2 = np.zeros((nl, nval, 2 * neq_pw_2d), np.complex128) if w2 else None
         Jd2 = np.zeros((nl, 2 * neq_pw_2d, nval), np.complex128) if w2 else None
-        opts = np.array([tol, 0.0, float(block), float(ncv)])
+        opts = np.array([tol, float(precision), float(block), float(ncv)])
         st = np.zeros(8, np.int64)
         with _host_lapack_threads(nl):
             check(lib().emu_calc_modes_1d_batch(

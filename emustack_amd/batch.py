@@ -30,7 +30,10 @@ def _pool(n):
         return p
 
 
-def _solve(structure, sims, inputs, nbm, order, P, tol, itermax, ncv):
+PRECISIONS = {'fp64': 0, 'mixed': 1}
+
+
+def _solve(structure, sims, inputs, nbm, order, P, tol, itermax, ncv, precision='fp64'):
     st = structure
     lam = [s.wl_norm() for s in sims]
     n_effs = [x[0] for x in inputs]
@@ -38,14 +41,16 @@ def _solve(structure, sims, inputs, nbm, order, P, tol, itermax, ncv):
     shifts = [x[3] for x in inputs]
     if st.periodicity == '2D_array':
         beta, J, Jd, stats = EMUstack.calc_modes_2d_batch(lam, nbm, order, paths.msh_path + st.mesh_file, n_effs,
-                                                          kps, shifts, P, tol=tol, itermax=itermax, ncv=ncv)
+                                                          kps, shifts, P, tol=tol, itermax=itermax, ncv=ncv,
+                                                          precision=PRECISIONS[precision])
         area_norm = st.period * st.period_y / st.period ** 2
         return beta, [np.asmatrix(j) / area_norm for j in J], [np.asmatrix(j) for j in Jd], stats
     h = EMUstack.mesh_1d(st.x_arr, st.table_nod, st.type_el)
     w1 = st.world_1d is True
     P2 = 1 if w1 else sims[0].calc_2d_grating_orders(order)[0].size
     beta, J, Jd, J2, Jd2, stats = EMUstack.context().calc_modes_1d_batch(
-        h, lam, nbm, order, n_effs, kps, shifts, P, neq_pw_2d=P2, world_1d=w1, itermax=itermax, tol=tol, ncv=ncv)
+        h, lam, nbm, order, n_effs, kps, shifts, P, neq_pw_2d=P2, world_1d=w1, itermax=itermax, tol=tol, ncv=ncv,
+        precision=PRECISIONS[precision])
     if not w1:
         J, Jd = J2, Jd2
     return beta, [np.asmatrix(j) for j in J], [np.asmatrix(j) for j in Jd], stats
@@ -86,34 +91,41 @@ def _fill(sims, nbm, parts, results):
     return stats
 
 
-def calc_modes_many(structure, lights, num_BMs=None, tol=0.0, itermax=30, groups=1, ncv=0):
+def calc_modes_many(structure, lights, num_BMs=None, tol=0.0, itermax=30, groups=1, ncv=0, precision='fp64'):
     """Simmo objects for every light in `lights` (same max_order_PWs),
     computed together on the GPU (`groups` concurrent sub-batches; ncv the
-    Krylov dimension, 0 for the reference's rule)."""
+    Krylov dimension, 0 for the reference's rule; precision 'fp64' or
+    'mixed')."""
+    if precision not in PRECISIONS:
+        raise ValueError("precision must be 'fp64' or 'mixed'")
     sims, inputs, nbm, order, P = _prepare(structure, lights, num_BMs)
     parts = _parts(len(sims), groups)
     if len(parts) == 1:
-        results = [_solve(structure, sims, inputs, nbm, order, P, tol, itermax, ncv)]
+        results = [_solve(structure, sims, inputs, nbm, order, P, tol, itermax, ncv, precision)]
     else:
-        futs = [_pool(len(parts)).submit(_solve, structure, sims[a:b], inputs[a:b], nbm, order, P, tol, itermax, ncv)
-                for a, b in parts]
+        futs = [_pool(len(parts)).submit(_solve, structure, sims[a:b], inputs[a:b], nbm, order, P, tol, itermax, ncv,
+                                         precision) for a, b in parts]
         results = [f.result() for f in futs]
     calc_modes_many.last_stats = _fill(sims, nbm, parts, results)
     return sims
 
 
-def calc_modes_stack(structures, lights, num_BMs=None, tol=0.0, itermax=30, groups=2, ncv=0, workers=None):
+def calc_modes_stack(structures, lights, num_BMs=None, tol=0.0, itermax=30, groups=2, ncv=0, workers=None,
+                     precision='fp64'):
     """calc_modes_many for several NanoStruct layers at once: every (layer,
     sub-batch) job goes to one pool of `workers` host threads (default
     groups + 1), each with its own context, so that one job's host Schur
     steps overlap another's GPU work and the GPU does not idle between
     layers.  Returns one list of Simmo objects per structure."""
+    if precision not in PRECISIONS:
+        raise ValueError("precision must be 'fp64' or 'mixed'")
     preps = [_prepare(st, lights, num_BMs) for st in structures]
     pool = _pool(int(workers) if workers else groups + 1)
     jobs = []
     for st, (sims, inputs, nbm, order, P) in zip(structures, preps):
         parts = _parts(len(sims), groups)
-        futs = [pool.submit(_solve, st, sims[a:b], inputs[a:b], nbm, order, P, tol, itermax, ncv) for a, b in parts]
+        futs = [pool.submit(_solve, st, sims[a:b], inputs[a:b], nbm, order, P, tol, itermax, ncv, precision)
+                for a, b in parts]
         jobs.append((parts, futs))
     stats = np.zeros(8, np.int64)
     out = []

@@ -152,7 +152,6 @@ int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const 
   if (E_H_field != 1 && E_H_field != 2) return fail(EMU_E_ARG, "E_H_field must be 1 or 2");
   for (int64_t i = 0; i < nbatch; ++i)
     if (!(lambda[i] > 0)) return fail(EMU_E_ARG, "lambda must be positive");
-  if (opts && opts[1] != 0) return fail(EMU_E_ARG, "only the complex128 precision form is available");
   EMU_GUARD({
     emu::ModesArgs a;
     a.nl = nbatch;
@@ -166,7 +165,12 @@ int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const 
     a.n_eff = n_eff;
     a.bloch = bloch_vec;
     a.shift = shift;
-    if (opts && opts[0] > 0) a.tol = opts[0];
+    if (opts && opts[0] > 0) {
+      a.tol = opts[0];
+      a.tol_set = true;
+    }
+    if (opts && opts[1] != 0) a.precision = (int64_t)opts[1];
+    if (a.precision != 0 && a.precision != 1) return fail(EMU_E_ARG, "precision must be 0 (FP64) or 1 (mixed)");
     if (opts && opts[2] > 0) a.block = (int64_t)opts[2];
     if (opts && opts[3] > 0) a.ncv = (int64_t)opts[3];
     if (a.block != 1 && a.block != 2 && a.block != 4 && a.block != 8)
@@ -218,7 +222,6 @@ int emu_calc_modes_1d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const 
     return fail(EMU_E_ARG, "world_1d = 0 needs the 2D overlap outputs and neq_PW_2d");
   for (int64_t i = 0; i < nbatch; ++i)
     if (!(lambda[i] > 0)) return fail(EMU_E_ARG, "lambda must be positive");
-  if (opts && opts[1] != 0) return fail(EMU_E_ARG, "only the complex128 precision form is available");
   EMU_GUARD({
     emu::ModesArgs a;
     a.nl = nbatch;
@@ -232,7 +235,12 @@ int emu_calc_modes_1d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const 
     a.n_eff = n_eff;
     a.bloch = bloch_vec;
     a.shift = shift;
-    if (opts && opts[0] > 0) a.tol = opts[0];
+    if (opts && opts[0] > 0) {
+      a.tol = opts[0];
+      a.tol_set = true;
+    }
+    if (opts && opts[1] != 0) a.precision = (int64_t)opts[1];
+    if (a.precision != 0 && a.precision != 1) return fail(EMU_E_ARG, "precision must be 0 (FP64) or 1 (mixed)");
     if (opts && opts[2] > 0) a.block = (int64_t)opts[2];
     if (opts && opts[3] > 0) a.ncv = (int64_t)opts[3];
     if (a.block != 1 && a.block != 2 && a.block != 4 && a.block != 8)

@@ -268,6 +268,11 @@ void lu_invert(DevMesh &dm, int nb, const zd *pool, zd *inv, hipStream_t s);
 int64_t solve_ws(const DevMesh &dm);
 void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *gx, const zd *b, zd *y, zd *x,
               zd *w, hipStream_t s, Profiler *prof = nullptr);
+// the same with the complex64 copy of the inverted factor (lu_narrow)
+void lu_solve(DevMesh &dm, int nb, int nrhs, const float2 *inv32, const int32_t *gx, const zd *b, zd *y, zd *x,
+              zd *w, hipStream_t s, Profiler *prof = nullptr);
+void lu_narrow(const DevMesh &dm, int nb, const zd *inv, float2 *inv32, hipStream_t s);
+void launch_swap_f2(int64_t n, float2 *a, float2 *b, hipStream_t s);
 // permute vectors old <-> new numbering: dst[new] = src[perm[new]] or inverse
 void launch_permute_vec(DevMesh &dm, int nb, int ncol, const zd *src, zd *dst, bool to_new,
                         hipStream_t s);

@@ -521,9 +521,9 @@ static void ks_restart_blocks(KSState &S, int m, int bq, int keep, int ld, int64
 // Schur decomposition (the expensive O(m^3) host step); T_LA = X Q_AA.  Schur
 // vectors of the wanted set whose own residual ||B q_i|| is below tol |theta_i|
 // (a leading run) are locked for the next cycle.
-// ARPACK's convergence scale (znconv): a Ritz value's residual is measured
-// against max(eps^(2/3), |theta|), so values near zero are not held to an
-// unreachable relative bound.
+// ARPACK's convergence scale (znaup2): a Ritz value's residual estimate is
+// measured against max(eps^(2/3), |theta|), so values near zero are not held
+// to an unreachable relative bound.
 static const double kEps23 = 3.6668528625e-11;  // (2^-53)^(2/3)
 
 static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol, std::vector<cplx> *ritzvec,
@@ -607,12 +607,14 @@ static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol
       if (qv == 0.0) continue;
       for (int r = 0; r < bq; ++r) BQ[r + (size_t)j * bq] += st.H[(m + r) + (size_t)i * ld] * qv;
     }
+  // residual bound of pair k: tol relative to |theta_k|
+  auto bound = [&](int k) { return tol * std::max(kEps23, std::abs(st.theta[k])); };
   // locking candidates: leading run of Schur vectors with small own residual
   st.nlock_next = 0;
   for (int k = 0; k < nev; ++k) {
     double r2 = 0;
     for (int r = 0; r < bq; ++r) r2 += std::norm(BQ[r + (size_t)k * bq]);
-    if (!(std::sqrt(r2) <= tol * std::max(kEps23, std::abs(st.theta[k])))) break;
+    if (!(std::sqrt(r2) <= bound(k))) break;
     st.nlock_next = k + 1;
   }
   bool conv = true;
@@ -625,9 +627,9 @@ static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol
       for (int j = 0; j <= k; ++j) acc += BQ[r + (size_t)j * bq] * VR[j + (size_t)k * nev];
       res2 += std::norm(acc);
     }
-    double rel = std::sqrt(res2 / ny) / std::max(std::abs(st.theta[k]), kEps23);
-    worst = std::max(worst, rel);
-    if (!(rel <= tol)) conv = false;
+    const double res = std::sqrt(res2 / ny);
+    worst = std::max(worst, res / std::max(std::abs(st.theta[k]), kEps23));
+    if (!(res <= bound(k))) conv = false;
   }
   if (maxres) *maxres = worst;
   if (ritzvec && (conv || ritz_always)) {
@@ -730,7 +732,8 @@ static CoreGate &core_gate() {
 }
 
 static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int bq, int itermax, double tol,
-                              const zd *inv, const int32_t *gx, const zd *mvals, zd *Xout, StageTimer &tm) {
+                              const zd *inv, const float2 *inv32, const int32_t *gx, const zd *mvals, zd *Xout,
+                              StageTimer &tm) {
   hipStream_t s = c->stream;
   const Symbolic &sy = dm.host.sym;
   const int64_t n = dm.host.neq;
@@ -796,9 +799,11 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   zd *inv_m = const_cast<zd *>(inv);          // the caller's buffers: slots are swapped during the
   int32_t *gx_m = const_cast<int32_t *>(gx);  // solve and restored in order before returning
   zd *mv_m = const_cast<zd *>(mvals);
+  float2 *inv32_m = const_cast<float2 *>(inv32);
   const int64_t nnzm = dm.host.nnz;
   auto swap_caller = [&](int a1, int a2) {
     launch_swap(dm.isz, inv_m + a1 * dm.isz, inv_m + a2 * dm.isz, s);
+    if (inv32_m) launch_swap_f2(dm.isz, inv32_m + a1 * dm.isz, inv32_m + a2 * dm.isz, s);
     launch_swap_i32(4 * sy.npiv, gx_m + 4 * a1 * sy.npiv, gx_m + 4 * a2 * sy.npiv, s);
     launch_swap(nnzm, mv_m + a1 * nnzm, mv_m + a2 * nnzm, s);
   };
@@ -824,14 +829,19 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
       const KTime kspmv = pf ? pf->time(4, nbd * (20.0 * nzd + 8.0 * (nd + 1) + 32.0 * nd * bq)) : KTime{};
       launch_spmm(nact, n, bq, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, t1, (int64_t)bq * n, s,
                   kspmv);
-      lu_solve(dm, nact, bq, inv, gx, t1, ty, wv, tw, s, pf);
+      if (inv32)
+        lu_solve(dm, nact, bq, inv32, gx, t1, ty, wv, tw, s, pf);
+      else
+        lu_solve(dm, nact, bq, inv, gx, t1, ty, wv, tw, s, pf);
       launch_copy(nact, bq * n, wv, (int64_t)bq * n, Wn, sV, s);
       if (delayed) {
         // Xj is the provisional block Z; one pass over V[:, :j] for Z's
         // reorthogonalisation and W's projection, then [Q_j, Zraw] in place
         const int c0 = j == k ? 0 : j - bq;
+        // algorithmic bytes: one read of [V Z W] (j + 2 bq columns; [Z W] is
+        // a slice of it); the combine reads V[:, :j] and [Z W] and writes [Z W]
         launch_atb(nact, n, j + 2 * bq, 2 * bq, V, n, sV, Xj, n, sV, Pd, ldp, sP, true, atbs, atb_elems, s,
-                   pf ? pf->time(2, nbd * 16.0 * nd * (j + 4 * bq)) : KTime{});
+                   pf ? pf->time(2, nbd * 16.0 * nd * (j + 2 * bq)) : KTime{});
         if (j > 0) launch_gemm_nn(nact, j, bq, j, zmk(1, 0), Hd, ld, sH, Pd, ldp, sP, zmk(0, 0), A0d, ldp, sA0, s);
         launch_dcgs_coef(nact, false, j, c0, Pd, ldp, sP, A0d, ldp, sA0, Hd, ld, sH, Ed, sP, Ld, bad, s);
         launch_combine(nact, n, j, 2 * bq, V, n, sV, Ed, sP, Ld, Xj, n, sV, s,
@@ -1080,6 +1090,108 @@ static std::vector<std::pair<int, int>> pair_swaps(const std::vector<cplx> &L, i
   return sw;
 }
 
+// Near-degenerate modes (a pair split only by the 1e-5 degree tilt of normal
+// incidence, relative gap ~1e-7) are individually ill-determined, and their
+// prime and adjoint vectors are biorthogonal only to (residual / gap), far
+// from working precision; the per-mode normalisation of normalisation.f then
+// leaves a coupling that shows in J J^dagger and R/T/A, and it depends on how
+// the solve happened to converge (batch, Krylov dimension).  Within each
+// group of modes coupled through the overlap matrix (|L_rc| > 1e-10
+// sqrt|L_rr L_cc|) the adjoint overlaps are therefore multiplied by the
+// inverse of the group's normalised overlap block: J^dagger rows of a group
+// become S_g^-1 J^dagger_g, which is what exact eigenvectors would give
+// (S_g = I for them); isolated modes are left exactly as the reference's
+// normalisation leaves them.  Lp(r, c): overlap of adjoint mode r (after the
+// pair swaps) with prime mode c, column-major.  `sym`: both families were
+// scaled by 1/sqrt(L_kk) (1D path), else the adjoint by 1/L_kk (2D path).
+// Ra: the instance's P2 x nval adjoint overlaps (column k = mode k), updated.
+static int biorth_groups(const std::vector<cplx> &Lp, int nval, bool sym, zd *Ra, int P2) {
+  std::vector<cplx> z(nval);
+  std::vector<char> ok(nval);
+  for (int k = 0; k < nval; ++k) {
+    z[k] = std::sqrt(Lp[k + (size_t)k * nval]);
+    ok[k] = std::abs(z[k]) > 1e-8;
+  }
+  std::vector<int> parent(nval);
+  std::iota(parent.begin(), parent.end(), 0);
+  auto find = [&](int i) {
+    while (parent[i] != i) i = parent[i] = parent[parent[i]];
+    return i;
+  };
+  for (int c = 0; c < nval; ++c)
+    for (int r = 0; r < nval; ++r) {
+      if (r == c) continue;
+      const double lim = 1e-10 * std::sqrt(std::abs(Lp[r + (size_t)r * nval]) * std::abs(Lp[c + (size_t)c * nval]));
+      if (std::abs(Lp[r + (size_t)c * nval]) > lim) {
+        const int a = find(r), b = find(c);
+        if (a != b) parent[std::max(a, b)] = std::min(a, b);
+      }
+    }
+  std::vector<std::vector<int>> groups(nval);
+  for (int k = 0; k < nval; ++k) groups[find(k)].push_back(k);
+  int changed = 0;
+  for (const auto &g : groups) {
+    const int gs = (int)g.size();
+    if (gs < 2) continue;
+    bool all_ok = true;
+    for (int k : g) all_ok = all_ok && ok[k];
+    if (!all_ok) continue;
+    // S_g (row-major gs x gs) and its inverse by Gauss-Jordan with pivoting
+    std::vector<cplx> S((size_t)gs * gs), Inv((size_t)gs * gs, 0.0);
+    for (int i = 0; i < gs; ++i) {
+      Inv[(size_t)i * gs + i] = 1.0;
+      for (int j = 0; j < gs; ++j) {
+        const cplx v = Lp[g[i] + (size_t)g[j] * nval];
+        S[(size_t)i * gs + j] = sym ? v / (z[g[i]] * z[g[j]]) : v / Lp[g[i] + (size_t)g[i] * nval];
+      }
+    }
+    bool singular = false;
+    for (int col = 0; col < gs && !singular; ++col) {
+      int piv = col;
+      for (int i = col + 1; i < gs; ++i)
+        if (std::abs(S[(size_t)i * gs + col]) > std::abs(S[(size_t)piv * gs + col])) piv = i;
+      if (std::abs(S[(size_t)piv * gs + col]) < 1e-12) {
+        singular = true;
+        break;
+      }
+      if (piv != col)
+        for (int j = 0; j < gs; ++j) {
+          std::swap(S[(size_t)piv * gs + j], S[(size_t)col * gs + j]);
+          std::swap(Inv[(size_t)piv * gs + j], Inv[(size_t)col * gs + j]);
+        }
+      const cplx d = 1.0 / S[(size_t)col * gs + col];
+      for (int j = 0; j < gs; ++j) {
+        S[(size_t)col * gs + j] *= d;
+        Inv[(size_t)col * gs + j] *= d;
+      }
+      for (int i = 0; i < gs; ++i) {
+        if (i == col) continue;
+        const cplx f = S[(size_t)i * gs + col];
+        if (f == 0.0) continue;
+        for (int j = 0; j < gs; ++j) {
+          S[(size_t)i * gs + j] -= f * S[(size_t)col * gs + j];
+          Inv[(size_t)i * gs + j] -= f * Inv[(size_t)col * gs + j];
+        }
+      }
+    }
+    if (singular) continue;
+    std::vector<cplx> col_new((size_t)gs);
+    for (int q = 0; q < P2; ++q) {
+      for (int i = 0; i < gs; ++i) {
+        cplx acc = 0;
+        for (int j = 0; j < gs; ++j) {
+          const zd v = Ra[q + (size_t)g[j] * P2];
+          acc += Inv[(size_t)i * gs + j] * cplx(v.x, v.y);
+        }
+        col_new[i] = acc;
+      }
+      for (int i = 0; i < gs; ++i) Ra[q + (size_t)g[i] * P2] = zmk(col_new[i].real(), col_new[i].imag());
+    }
+    changed += gs;
+  }
+  return changed;
+}
+
 // ------------------------------------------------------------ calc_modes
 // Per-instance coefficients (py_calc_modes.f:583-599, py_calc_modes_1d.f:
 // 318-372): instance 2i is the prime problem (+bloch), 2i+1 the adjoint
@@ -1128,6 +1240,23 @@ static void instance_params(const ModesArgs &a, int eh, std::vector<cplx> &pp, s
 // beta = sqrt(1/mu + shift) with the reference's branch rule, and the
 // z_indexx sort on -Re(beta^2).  The sorted modes (original numbering) end
 // in Xn; Xo is scratch.
+// default convergence tolerance of the mixed-precision form
+static const double kMixedTol = 1e-8;
+
+// beta from the two-sided Rayleigh quotient mu_rq of a mode whose Ritz value
+// gave beta_ritz; the quotient is used only when it agrees with the Ritz
+// value to 1e-4 (a mode whose adjoint partner is not its left eigenvector,
+// e.g. a vanishing overlap, keeps the Ritz value)
+static cplx rq_beta(cplx mu_rq, cplx beta_ritz, cplx shift, int64_t &used, int64_t &kept) {
+  const cplx mu_ritz = 1.0 / (beta_ritz * beta_ritz - shift);
+  if (!(std::abs(mu_rq - mu_ritz) <= 1e-4 * std::abs(mu_ritz))) {
+    ++kept;
+    return beta_ritz;
+  }
+  ++used;
+  return beta_from_mu(mu_rq, shift);
+}
+
 struct SolveOut {
   EigResult er;
   std::vector<cplx> beta;  // nb x nval, sorted
@@ -1151,8 +1280,16 @@ static SolveOut factor_solve_sort(Ctx *c, DevMesh &dm, const ModesArgs &a, int n
   int32_t *gx = c->buf<int32_t>("gx", (size_t)nb * sy.npiv * 4);
   zd *inv = c->buf<zd>("inv", (size_t)nb * dm.isz);
   lu_factor(dm, nb, m1, pool, piv, o.pflag, sig, gx, inv, s);
+  float2 *inv32 = nullptr;
+  if (a.precision == 1) {
+    inv32 = c->buf<float2>("inv32", (size_t)nb * dm.isz);
+    lu_narrow(dm, nb, inv, inv32, s);
+  }
   tm.mark("lu_factor");
-  o.er = krylov_schur(c, dm, nb, nval, m, bq, (int)a.itermax, a.tol, inv, gx, m2p, Xn, tm);
+  // mixed form: the pairs converge against the complex64 operator, whose
+  // distance from the FP64 one (~1e-7) bounds what a tighter test could buy
+  const double tol = a.tol_set || a.precision == 0 ? a.tol : kMixedTol;
+  o.er = krylov_schur(c, dm, nb, nval, m, bq, (int)a.itermax, tol, inv, inv32, gx, m2p, Xn, tm);
   launch_permute_vec(dm, nb, nval, Xn, Xo, false, s);
   o.beta.assign((size_t)nb * nval, 0);
   std::vector<int32_t> order((size_t)nb * nval);
@@ -1235,19 +1372,57 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
              false, atbs, atb_elems, s);
   std::vector<cplx> Lh((size_t)nl * nval * nval);
   EMU_HIP(hipMemcpyAsync(Lh.data(), Ld, sizeof(zd) * Lh.size(), hipMemcpyDeviceToHost, s));
+  // the FP64 pencil K - shift M (mat1) applied to the prime modes as well,
+  // for the two-sided Rayleigh quotients below
+  std::vector<cplx> LAh;
+  {
+    launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, m1, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s);
+    zd *LAd = c->buf<zd>("ovl_LA", (size_t)nl * nval * nval);
+    launch_atb(nl, n, nval, nval, Xn + n * nval, n, 2 * n * nval, Y, n, n * nval, LAd, nval, (int64_t)nval * nval,
+               false, atbs, atb_elems, s);
+    LAh.resize((size_t)nl * nval * nval);
+    EMU_HIP(hipMemcpyAsync(LAh.data(), LAd, sizeof(zd) * LAh.size(), hipMemcpyDeviceToHost, s));
+  }
   EMU_HIP(hipStreamSynchronize(s));
   std::vector<int32_t> perm2((size_t)nb * nval);
   std::vector<zd> scl((size_t)nb * nval, zmk(1, 0));
+  std::vector<std::vector<cplx>> Lswapped(nl);
+  std::vector<cplx> beta_c(beta);
+  int64_t rq_used = 0, rq_kept = 0;  // corrected in the mixed form
   for (int i = 0; i < nl; ++i) {
     cplx *L = &Lh[(size_t)i * nval * nval];
+    {
+      // beta from mu = (y^T M x) / (y^T (K - shift M) x) of each prime mode x
+      // and its adjoint partner y (the pencil's left eigenvector): FP64, with
+      // an error of the order of the product of the two vectors' errors.
+      // This matters for modes near cut-off (beta ~ 0), where a residual of
+      // tol |mu| in mu becomes tol shift / (2 |beta|) in beta, and in the
+      // mixed form, whose vectors carry the complex64 factor's error.
+      std::vector<int> p(nval);
+      std::iota(p.begin(), p.end(), 0);
+      std::vector<cplx> Lv(L, L + (size_t)nval * nval);
+      for (int j = 0; j < nval; ++j)
+        for (int r = 0; r < nval; ++r) Lv[r + (size_t)j * nval] *= beta[(size_t)(2 * i) * nval + j] * (-1.0 / k0[i]);
+      for (auto &sw : pair_swaps(Lv, nval)) std::swap(p[sw.first], p[sw.second]);
+      const cplx *LA = &LAh[(size_t)i * nval * nval];
+      for (int k = 0; k < nval; ++k) {
+        const cplx den = LA[p[k] + (size_t)k * nval];
+        if (std::abs(den) == 0.0) continue;
+        const cplx mu = L[p[k] + (size_t)k * nval] / den;
+        beta_c[(size_t)(2 * i) * nval + k] = rq_beta(mu, beta[(size_t)(2 * i) * nval + k], sh[2 * i], rq_used, rq_kept);
+      }
+    }
     for (int j = 0; j < nval; ++j) {
-      cplx f = beta[(size_t)(2 * i) * nval + j] * (-1.0 / k0[i]);
+      cplx f = beta_c[(size_t)(2 * i) * nval + j] * (-1.0 / k0[i]);
       for (int r = 0; r < nval; ++r) L[r + (size_t)j * nval] *= f;
     }
     std::vector<int> p(nval);
     std::iota(p.begin(), p.end(), 0);
     std::vector<cplx> Lv(L, L + (size_t)nval * nval);
     for (auto &sw : pair_swaps(Lv, nval)) std::swap(p[sw.first], p[sw.second]);
+    Lswapped[i].resize((size_t)nval * nval);
+    for (int cc = 0; cc < nval; ++cc)
+      for (int r = 0; r < nval; ++r) Lswapped[i][r + (size_t)cc * nval] = L[p[r] + (size_t)cc * nval];
     for (int k = 0; k < nval; ++k) {
       perm2[(size_t)(2 * i) * nval + k] = k;
       perm2[(size_t)(2 * i + 1) * nval + k] = p[k];
@@ -1317,10 +1492,11 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   EMU_HIP(hipStreamSynchronize(s));
   // outputs
   const int P2 = 2 * P;
+  for (int i = 0; i < nl; ++i) biorth_groups(Lswapped[i], nval, false, &Rh[(size_t)(2 * i + 1) * P2 * nval], P2);
   for (int i = 0; i < nl; ++i) {
     for (int k = 0; k < nval; ++k) {
-      a.beta[2 * ((size_t)i * nval + k)] = beta[(size_t)(2 * i) * nval + k].real();
-      a.beta[2 * ((size_t)i * nval + k) + 1] = beta[(size_t)(2 * i) * nval + k].imag();
+      a.beta[2 * ((size_t)i * nval + k)] = beta_c[(size_t)(2 * i) * nval + k].real();
+      a.beta[2 * ((size_t)i * nval + k) + 1] = beta_c[(size_t)(2 * i) * nval + k].imag();
     }
     const zd *Rp = &Rh[(size_t)(2 * i) * P2 * nval];
     const zd *Ra = &Rh[(size_t)(2 * i + 1) * P2 * nval];
@@ -1341,6 +1517,8 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
     a.stats[2] = std::accumulate(pf.begin(), pf.end(), (int64_t)0);
     a.stats[3] = er.steps;
     a.stats[4] = er.rank_fixes;
+    a.stats[5] = rq_used;
+    a.stats[6] = rq_kept;
   }
 }
 
@@ -1425,9 +1603,28 @@ void calc_modes_1d(Ctx *c, DevMesh &dm, const ModesArgs &a) {
              false, atbs, atb_elems, s);
   std::vector<cplx> Lh((size_t)nl * nval * nval);
   EMU_HIP(hipMemcpyAsync(Lh.data(), Ld, sizeof(zd) * Lh.size(), hipMemcpyDeviceToHost, s));
+  // y^T M x and y^T (K - shift M) x of the prime modes and their adjoint
+  // partners, for the two-sided Rayleigh quotients below
+  std::vector<cplx> LBh, LAh;
+  {
+    zd *LAd = c->buf<zd>("ovl_LA", (size_t)nl * nval * nval);
+    zd *LBd = c->buf<zd>("ovl_LB", (size_t)nl * nval * nval);
+    for (int w = 0; w < 2; ++w) {
+      launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, w ? m1 : m2, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s);
+      launch_atb(nl, n, nval, nval, Xn + n * nval, n, 2 * n * nval, Y, n, n * nval, w ? LAd : LBd, nval,
+                 (int64_t)nval * nval, false, atbs, atb_elems, s);
+    }
+    LAh.resize((size_t)nl * nval * nval);
+    LBh.resize((size_t)nl * nval * nval);
+    EMU_HIP(hipMemcpyAsync(LAh.data(), LAd, sizeof(zd) * LAh.size(), hipMemcpyDeviceToHost, s));
+    EMU_HIP(hipMemcpyAsync(LBh.data(), LBd, sizeof(zd) * LBh.size(), hipMemcpyDeviceToHost, s));
+  }
   EMU_HIP(hipStreamSynchronize(s));
   std::vector<int32_t> perm2((size_t)nb * nval);
   std::vector<zd> scl((size_t)nb * nval, zmk(1, 0));
+  std::vector<std::vector<cplx>> Lswapped(nl);
+  std::vector<cplx> beta_c(beta);
+  int64_t rq_used = 0, rq_kept = 0;  // corrected in the mixed form
   for (int i = 0; i < nl; ++i) {
     cplx *L = &Lh[(size_t)i * nval * nval];
     for (int j = 0; j < nval; ++j) {
@@ -1438,6 +1635,25 @@ void calc_modes_1d(Ctx *c, DevMesh &dm, const ModesArgs &a) {
     std::iota(p.begin(), p.end(), 0);
     std::vector<cplx> Lv(L, L + (size_t)nval * nval);
     for (auto &sw : pair_swaps(Lv, nval)) std::swap(p[sw.first], p[sw.second]);
+    {
+      // beta from mu = (y^T M x) / (y^T (K - shift M) x) (FP64 pencil; see
+      // calc_modes)
+      const cplx *LA = &LAh[(size_t)i * nval * nval], *LB = &LBh[(size_t)i * nval * nval];
+      for (int k = 0; k < nval; ++k) {
+        const cplx den = LA[p[k] + (size_t)k * nval];
+        if (std::abs(den) == 0.0) continue;
+        beta_c[(size_t)(2 * i) * nval + k] = rq_beta(LB[p[k] + (size_t)k * nval] / den, beta[(size_t)(2 * i) * nval + k],
+                                                     sh[2 * i], rq_used, rq_kept);
+      }
+      // the overlap carries beta1: rescale it by the corrected values
+      for (int j = 0; j < nval; ++j) {
+        const cplx f = beta_c[(size_t)(2 * i) * nval + j] / beta[(size_t)(2 * i) * nval + j];
+        for (int r = 0; r < nval; ++r) L[r + (size_t)j * nval] *= f;
+      }
+    }
+    Lswapped[i].resize((size_t)nval * nval);
+    for (int cc = 0; cc < nval; ++cc)
+      for (int r = 0; r < nval; ++r) Lswapped[i][r + (size_t)cc * nval] = L[p[r] + (size_t)cc * nval];
     for (int k = 0; k < nval; ++k) {
       perm2[(size_t)(2 * i) * nval + k] = k;
       perm2[(size_t)(2 * i + 1) * nval + k] = p[k];
@@ -1497,12 +1713,13 @@ void calc_modes_1d(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   // outputs: beta, J (2P x nval), J^dagger (nval x 2P) and, for a 2D world,
   // the same embedded in the 2D plane-wave basis (pw_matrix_1d_to_2d.f)
   const int P2 = 2 * P;
+  for (int i = 0; i < nl; ++i) biorth_groups(Lswapped[i], nval, true, &Rh[(size_t)(2 * i + 1) * P2 * nval], P2);
   int PW2 = 0;
   std::vector<int> pos2d;  // per 1D order (unsorted): sorted 2D position of (px, 0)
   for (int i = 0; i < nl; ++i) {
     for (int k = 0; k < nval; ++k) {
-      a.beta[2 * ((size_t)i * nval + k)] = beta[(size_t)(2 * i) * nval + k].real();
-      a.beta[2 * ((size_t)i * nval + k) + 1] = beta[(size_t)(2 * i) * nval + k].imag();
+      a.beta[2 * ((size_t)i * nval + k)] = beta_c[(size_t)(2 * i) * nval + k].real();
+      a.beta[2 * ((size_t)i * nval + k) + 1] = beta_c[(size_t)(2 * i) * nval + k].imag();
     }
     const zd *Rp = &Rh[(size_t)(2 * i) * P2 * nval];
     const zd *Ra = &Rh[(size_t)(2 * i + 1) * P2 * nval];
@@ -1558,6 +1775,8 @@ void calc_modes_1d(Ctx *c, DevMesh &dm, const ModesArgs &a) {
     a.stats[2] = std::accumulate(pf.begin(), pf.end(), (int64_t)0);
     a.stats[3] = so.er.steps;
     a.stats[4] = so.er.rank_fixes;
+    a.stats[5] = rq_used;
+    a.stats[6] = rq_kept;
   }
 }
 

@@ -45,8 +45,13 @@ struct ModesArgs {
   const double *n_eff = nullptr;   // nl x ntyp complex
   const double *bloch = nullptr;   // nl x 2
   const double *shift = nullptr;   // nl complex
-  double tol = 1.1102230246251565e-16;  // 2^-53: ARPACK's tol = 0 (machine precision)
+  double tol = 1e-12;  // relative residual of every wanted Ritz pair (see ks_project)
   int64_t block = 8;               // Krylov block size (1, 2, 4 or 8)
+  // 0: complex128 everywhere; 1: mixed: the Krylov solves read a complex64
+  // copy of the (FP64-computed) inverted factor, and every converged pair's
+  // beta is corrected by the FP64 two-sided Rayleigh quotient of the pencil
+  int64_t precision = 0;
+  bool tol_set = false;            // tol given by the caller (else the precision's default)
   int64_t ncv = 0;                 // Krylov dimension (0: the reference's 2 nval + nval/2 + 3)
   double *beta = nullptr;          // nl x nval complex
   double *J = nullptr;             // nl x (2P x nval) complex

@@ -269,6 +269,13 @@ void launch_swap(int64_t n, zd *a, zd *b, hipStream_t s) {
   EMU_LAUNCH_CHECK();
 }
 
+void launch_swap_f2(int64_t n, float2 *a, float2 *b, hipStream_t s) {
+  if (n <= 0 || a == b) return;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_swap<float2>, dim3(g), dim3(256), 0, s, n, a, b);
+  EMU_LAUNCH_CHECK();
+}
+
 void launch_swap_i32(int64_t n, int32_t *a, int32_t *b, hipStream_t s) {
   if (n <= 0 || a == b) return;
   const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);

@@ -621,9 +621,31 @@ void lu_invert(DevMesh &dm, int nb, const zd *pool, zd *inv, hipStream_t s) {
 
 constexpr int KC = 64;  // staged input chunk
 
-template <int RC>
+// Storage of the inverted factor read by the solves: complex128 (zd), or
+// complex64 (float2, the mixed-precision form), widened to double on load;
+// the arithmetic is FP64 either way.
+template <typename FT>
+struct FactorLd;
+template <>
+struct FactorLd<zd> {
+  static constexpr uint32_t kBytes = 16;
+  static __device__ __forceinline__ zd ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return buf_ld_stream(r, voff, soff);
+  }
+};
+template <>
+struct FactorLd<float2> {
+  static constexpr uint32_t kBytes = 8;
+  static __device__ __forceinline__ zd ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    const f2 v = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+    return zmk((double)v.x, (double)v.y);
+  }
+};
+
+template <int RC, typename FT>
 __global__ __launch_bounds__(256) void k_fwd(const SolveTask *__restrict__ tasks, const int32_t *__restrict__ cmap,
-                                             const zd *__restrict__ inv, int64_t isz, const int4 *__restrict__ gx,
+                                             const FT *__restrict__ inv, int64_t isz, const int4 *__restrict__ gx,
                                              int64_t npiv, int nrhs, int64_t n, int64_t nw,
                                              const zd *__restrict__ bv, zd *__restrict__ y, zd *__restrict__ wbuf,
                                              zd *__restrict__ part, int nslot, int span) {
@@ -634,14 +656,15 @@ __global__ __launch_bounds__(256) void k_fwd(const SolveTask *__restrict__ tasks
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: column offsets in SGPRs
   const int ks = 4 / T.tpw, tile = wave / ks, kp = wave - tile * ks;
   const int s = T.s, f = T.f;
-  const zd *Fw = inv + b * isz + T.ioff;
+  const FT *Fw = inv + b * isz + T.ioff;
+  constexpr uint32_t EB = FactorLd<FT>::kBytes;
   const int4 *gi = gx + b * npiv + T.pivoff;
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
   const int row0 = T.r0 + 64 * tile;  // first row of this wave's tile
   const int row = row0 + lane;
   const int rowc = min(row, f - 1);   // clamped: addresses stay inside the front
-  const uint32_t rowb = (uint32_t)rowc * 16u;  // byte offset of the row in a column
+  const uint32_t rowb = (uint32_t)rowc * EB;  // byte offset of the row in a column
   // columns of this task: the workgroup's range, cut to [kb, kb + span)
   // for a split task; lower-triangular rows need k <= row
   int kbeg = 0, kend = min(s, T.r0 + 64 * T.tpw);
@@ -670,15 +693,15 @@ __global__ __launch_bounds__(256) void k_fwd(const SolveTask *__restrict__ tasks
     const int4 g = gi[okk ? kk : kbeg];
     const int kl = min(KC, kend_w - kc);
     const int nk = kl > kp ? (kl - kp + ks - 1) / ks : 0;  // this wave's columns kp, kp + ks, ...
-    // column c of the wave's rows: byte c f 16 + rowb of the front's block
-    const __amdgpu_buffer_rsrc_t Fcol = buf_view(Fw, (uint32_t)f * (uint32_t)s * 16u);
+    // column c of the wave's rows: byte c f EB + rowb of the front's block
+    const __amdgpu_buffer_rsrc_t Fcol = buf_view(Fw, (uint32_t)f * (uint32_t)s * EB);
     // column indices clamped into [0, s): an idle wave (nk = 0) reads a
     // valid column and discards it, so the loads need no branch (a branch
     // would make the gather below wait for them)
     const int cbase = kc + kp, clast = min(kc + kp + max(nk - 1, 0) * ks, s - 1);
     zd a[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] = buf_ld_stream(Fcol, rowb, (uint32_t)(min(cbase + u * ks, clast) * f) * 16u);
+    for (int u = 0; u < 8; ++u) a[u] = FactorLd<FT>::ld(Fcol, rowb, (uint32_t)(min(cbase + u * ks, clast) * f) * EB);
     constexpr int NG = (RC + 3) / 4;  // right-hand sides gathered per thread
     zd gv[NG], g0[NG], g1[NG];
 #pragma unroll
@@ -703,7 +726,7 @@ __global__ __launch_bounds__(256) void k_fwd(const SolveTask *__restrict__ tasks
     for (; t0 + 8 < nk; t0 += 8) {
       zd nx[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) nx[u] = buf_ld_stream(Fcol, rowb, (uint32_t)(min(cbase + (t0 + 8 + u) * ks, clast) * f) * 16u);
+      for (int u = 0; u < 8; ++u) nx[u] = FactorLd<FT>::ld(Fcol, rowb, (uint32_t)(min(cbase + (t0 + 8 + u) * ks, clast) * f) * EB);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int kx = kp + (t0 + u) * ks;
@@ -808,9 +831,9 @@ __global__ __launch_bounds__(256) void k_fwd_red(const int32_t *__restrict__ tas
   }
 }
 
-template <int RC>
+template <int RC, typename FT>
 __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks, const int32_t *__restrict__ uidx,
-                                             const zd *__restrict__ inv, int64_t isz, int nrhs, int64_t n,
+                                             const FT *__restrict__ inv, int64_t isz, int nrhs, int64_t n,
                                              const zd *__restrict__ y, zd *__restrict__ x, zd *__restrict__ part,
                                              int nslot, int span) {
   const SolveTask T = tasks[blockIdx.x];
@@ -820,14 +843,15 @@ __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: column offsets in SGPRs
   const int ks = 4 / T.tpw, tile = wave / ks, kp = wave - tile * ks;
   const int s = T.s, f = T.f;
-  const zd *Bw = inv + b * isz + T.ioff + (int64_t)f * s;
+  const FT *Bw = inv + b * isz + T.ioff + (int64_t)f * s;
+  constexpr uint32_t EB = FactorLd<FT>::kBytes;
   const int32_t *ui = uidx + T.uoff;
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
   const int row = T.r0 + 64 * tile;  // first row of this wave's tile
   const int rowl = row + lane;
   const int rowc = min(rowl, s - 1);
-  const uint32_t rowb = (uint32_t)max(rowc, 0) * 16u;  // byte offset of the row in a column
+  const uint32_t rowb = (uint32_t)max(rowc, 0) * EB;  // byte offset of the row in a column
   // columns: [r0, f) for the workgroup (upper-triangular rows need k >= row),
   // cut to [kb, kb + span) for a split task
   int kbeg = T.r0, kend = f;
@@ -851,13 +875,13 @@ __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks
     const int klo = max(0, row - kc), khi = min(KC, kend - kc);
     const int kst = klo + kp;
     const int nk = (row < s && khi > kst) ? (khi - kst + ks - 1) / ks : 0;
-    // column c of the wave's rows: byte c s 16 + rowb of the front's block
-    const __amdgpu_buffer_rsrc_t Fcol = buf_view(Bw, (uint32_t)f * (uint32_t)s * 16u);
+    // column c of the wave's rows: byte c s EB + rowb of the front's block
+    const __amdgpu_buffer_rsrc_t Fcol = buf_view(Bw, (uint32_t)f * (uint32_t)s * EB);
     // column indices clamped into [0, f) (see k_fwd)
     const int cbase = kc + kst, clast = min(kc + kst + max(nk - 1, 0) * ks, f - 1);
     zd a[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) a[u] = buf_ld_stream(Fcol, rowb, (uint32_t)(min(cbase + u * ks, clast) * s) * 16u);
+    for (int u = 0; u < 8; ++u) a[u] = FactorLd<FT>::ld(Fcol, rowb, (uint32_t)(min(cbase + u * ks, clast) * s) * EB);
     const zd *src = kkc < s ? y : x;
     constexpr int NG = (RC + 3) / 4;
     zd gv[NG];
@@ -875,7 +899,7 @@ __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks
     for (; t0 + 8 < nk; t0 += 8) {
       zd nx[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) nx[u] = buf_ld_stream(Fcol, rowb, (uint32_t)(min(cbase + (t0 + 8 + u) * ks, clast) * s) * 16u);
+      for (int u = 0; u < 8; ++u) nx[u] = FactorLd<FT>::ld(Fcol, rowb, (uint32_t)(min(cbase + (t0 + 8 + u) * ks, clast) * s) * EB);
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int kx = kst + (t0 + u) * ks;
@@ -959,13 +983,13 @@ static int pick_rc(int nrhs, int64_t tasks, int nb) {
   return rc;
 }
 
-template <int RC>
-static void launch_fwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *inv,
+template <int RC, typename FT>
+static void launch_fwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, int nrhs, const FT *inv,
                        const int32_t *gx, const zd *bnew, zd *y, zd *w, zd *part, hipStream_t s, KTime kt) {
   const Symbolic &sy = dm.host.sym;
   const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
   const int64_t nw = std::max<int64_t>(sy.nw, 1);
-  hipExtLaunchKernelGGL(k_fwd<RC>, dim3((unsigned)T.cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
+  hipExtLaunchKernelGGL(k_fwd<RC, FT>, dim3((unsigned)T.cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
                         reinterpret_cast<const SolveTask *>(dm.tasks.p + T.off), dm.cmap.p, inv, dm.isz,
                         reinterpret_cast<const int4 *>(gx), sy.npiv, nrhs, sy.n, nw, bnew, y, w, part,
                         (int)T.nslot, (int)std::min<int64_t>(T.span, INT32_MAX));
@@ -974,12 +998,12 @@ static void launch_fwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, 
                        sy.n, nw, part, (int)T.nslot, y, w);
 }
 
-template <int RC>
-static void launch_bwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, int nrhs, const zd *inv,
+template <int RC, typename FT>
+static void launch_bwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, int nrhs, const FT *inv,
                        const zd *y, zd *x, zd *part, hipStream_t s, KTime kt) {
   const Symbolic &sy = dm.host.sym;
   const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
-  hipExtLaunchKernelGGL(k_bwd<RC>, dim3((unsigned)T.cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
+  hipExtLaunchKernelGGL(k_bwd<RC, FT>, dim3((unsigned)T.cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
                         reinterpret_cast<const SolveTask *>(dm.tasks.p + T.off), dm.uidx.p, inv, dm.isz, nrhs,
                         sy.n, y, x, part, (int)T.nslot, (int)std::min<int64_t>(T.span, INT32_MAX));
   if (T.rcnt > 0)
@@ -996,16 +1020,18 @@ int64_t solve_ws(const DevMesh &dm) { return std::max<int64_t>(dm.host.sym.nw, 1
 // traffic.
 static const SolveTasks &pick_tasks(const SolveTasks *v, int nrhs) { return nrhs <= 8 ? v[1] : v[0]; }
 
-void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *gx, const zd *bnew, zd *y, zd *x,
-              zd *w, hipStream_t s, Profiler *prof) {
+template <typename FT>
+static void lu_solve_t(DevMesh &dm, int nb, int nrhs, const FT *inv, const int32_t *gx, const zd *bnew, zd *y,
+                       zd *x, zd *w, hipStream_t s, Profiler *prof) {
   FrontArrays A = fa(dm);
   const double vecs = (double)nb * nrhs;
+  const double fscale = FactorLd<FT>::kBytes / 16.0;  // factor bytes per entry relative to complex128
   // partial sums of split tasks live behind the update vectors in w
   zd *part = w + (int64_t)nb * nrhs * std::max<int64_t>(dm.host.sym.nw, 1);
   for (const LevelSched &L : dm.sched) {
     const SolveTasks &T = pick_tasks(L.fwd, nrhs);
     const int rc = pick_rc(nrhs, T.cnt, nb);
-    KTime kt = prof ? prof->time(0, L.fwd_mat * nb + L.fwd_vec * vecs) : KTime{};
+    KTime kt = prof ? prof->time(0, L.fwd_mat * fscale * nb + L.fwd_vec * vecs) : KTime{};
     switch (rc) {
       case 8: launch_fwd<8>(T, dm, A, nb, nrhs, inv, gx, bnew, y, w, part, s, kt); break;
       case 4: launch_fwd<4>(T, dm, A, nb, nrhs, inv, gx, bnew, y, w, part, s, kt); break;
@@ -1017,7 +1043,7 @@ void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *gx, c
   for (auto it = dm.sched.rbegin(); it != dm.sched.rend(); ++it) {
     const SolveTasks &T = pick_tasks(it->bwd, nrhs);
     const int rc = pick_rc(nrhs, T.cnt, nb);
-    KTime kt = prof ? prof->time(1, it->bwd_mat * nb + it->bwd_vec * vecs) : KTime{};
+    KTime kt = prof ? prof->time(1, it->bwd_mat * fscale * nb + it->bwd_vec * vecs) : KTime{};
     switch (rc) {
       case 8: launch_bwd<8>(T, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
       case 4: launch_bwd<4>(T, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
@@ -1025,6 +1051,32 @@ void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *gx, c
       default: launch_bwd<1>(T, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
     }
   }
+  EMU_LAUNCH_CHECK();
+}
+
+void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *gx, const zd *bnew, zd *y, zd *x,
+              zd *w, hipStream_t s, Profiler *prof) {
+  lu_solve_t<zd>(dm, nb, nrhs, inv, gx, bnew, y, x, w, s, prof);
+}
+
+void lu_solve(DevMesh &dm, int nb, int nrhs, const float2 *inv32, const int32_t *gx, const zd *bnew, zd *y, zd *x,
+              zd *w, hipStream_t s, Profiler *prof) {
+  lu_solve_t<float2>(dm, nb, nrhs, inv32, gx, bnew, y, x, w, s, prof);
+}
+
+// complex64 copy of the inverted factor (round to nearest)
+__global__ void k_narrow(int64_t n, const zd *__restrict__ src, float2 *__restrict__ dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const zd v = src[i];
+    dst[i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+void lu_narrow(const DevMesh &dm, int nb, const zd *inv, float2 *inv32, hipStream_t s) {
+  const int64_t n = (int64_t)nb * dm.isz;
+  if (n <= 0) return;
+  const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_narrow, dim3(g), dim3(256), 0, s, n, inv, inv32);
   EMU_LAUNCH_CHECK();
 }
 

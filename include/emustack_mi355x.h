@@ -108,12 +108,22 @@ int emu_calc_modes_2d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval,
  * overlap_J[nbatch*2*2neq_PW*nval], overlap_J_dagger[nbatch*2*nval*2neq_PW].
  * opts (may be NULL; 4 entries): opts[0] = eigen tolerance: a Ritz pair is
  * converged when its residual estimate is <= tol max(eps^(2/3), |mu|), the
- * test of ARPACK's znconv (<= 0: machine precision 2^-53, as the reference's
- * tol = 0, py_calc_modes.f:174 / zarpack.f:527); opts[1] = precision (0 = complex128 everywhere); opts[2] = Krylov
+ * form of ARPACK's test (zarpack.f:2037-2047).  <= 0: 1e-12 for precision 0,
+ * 1e-8 for precision 1.  (The reference asks ARPACK for machine precision,
+ * py_calc_modes.f:174; with this build's block Krylov-Schur the relative
+ * 1e-12 converges inside the reference's 30 restarts and gives beta within
+ * 1e-9 of ARPACK's; near-degenerate modes, whose vectors neither method
+ * fixes to working precision, are normalised by their overlap block.)
+ * opts[1] = precision: 0 complex128 throughout; 1 mixed (the Krylov solves
+ * read a complex64 copy of the FP64 inverted factor, and each converged
+ * pair's beta is the FP64 two-sided Rayleigh quotient of the pencil);
+ * opts[2] = Krylov
  * block size (1, 2, 4 or 8; <= 0: default 8); opts[3] = Krylov dimension
  * (<= 0: the reference's 2 nval + nval/2 + 3, rounded up to whole blocks).  stats[0..7] (may be NULL): OP applications,
  * restarts (max over instances), pivot perturbations, Arnoldi steps,
- * 0, 0, 0, 0. */
+ * rank repairs, modes whose beta came from the FP64 two-sided Rayleigh
+ * quotient, modes that kept the Ritz value (quotient disagreed by > 1e-4),
+ * 0. */
 int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch,
                             const double *lambda, int64_t nval, int64_t ordre_ls,
                             int64_t nb_typ_el, const double *n_eff, const double *bloch_vec,

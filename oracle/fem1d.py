@@ -36,7 +36,7 @@ import numpy as np
 import scipy.linalg as sla
 import scipy.sparse.linalg as spla
 
-from .fem2d import beta_from_mu, nr_indexx, pair_swap
+from .fem2d import beta_from_mu, biorth_clusters, nr_indexx, pair_swap
 
 # ------------------------------------------------------------------ mesh
 
@@ -384,7 +384,7 @@ def pw_1d_to_2d(J, Jd, ordre, period_x, kx, ky, k0):
 # ----------------------------------------------------------------- driver
 
 
-def calc_modes_1d(wl_norm, nval, ordre, m, n_eff, kx, ky, shift, world_1d=True):
+def calc_modes_1d(wl_norm, nval, ordre, m, n_eff, kx, ky, shift, world_1d=True, biorth='diag'):
     """beta (nval,), J (2P x nval), J^dagger (nval x 2P) as returned by
     calc_modes_1d (py_calc_modes_1d.f); P = 2 ordre + 1, or the 2D count
     when world_1d is False."""
@@ -413,6 +413,24 @@ def calc_modes_1d(wl_norm, nval, ordre, m, n_eff, kx, ky, shift, world_1d=True):
     L = overlap_1d(m, pp, ky, b1, s1, s2, k0)
     s1, s2 = normalise_1d(L, s1, s2)
     J, Jd = j_overlaps_1d(m, s1, s2, ordre, kx, ky, k0)
+    Jdc = None
+    if biorth in ('cluster', 'both'):
+        # after the reference's 1/sqrt(L_ii) scaling of both families the
+        # overlap is L_ij / sqrt(L_ii L_jj); invert it within each group of
+        # coupled modes (identity for isolated modes)
+        z = np.sqrt(np.diag(L))
+        ok = np.abs(z) > 1e-8
+        Ln = L / np.outer(np.where(ok, z, 1), np.where(ok, z, 1))
+        N = np.eye(len(b1), dtype=complex)
+        for g in biorth_clusters(L):
+            if g.size > 1 and np.all(ok[g]):
+                N[np.ix_(g, g)] = np.linalg.inv(Ln[np.ix_(g, g)])
+        Jdc = N @ Jd
     if not world_1d:
-        J, Jd = pw_1d_to_2d(J, Jd, ordre, m.period_x, kx, ky, k0)
-    return b1, J, Jd
+        J2, Jd = pw_1d_to_2d(J, Jd, ordre, m.period_x, kx, ky, k0)
+        if Jdc is not None:
+            Jdc = pw_1d_to_2d(J, Jdc, ordre, m.period_x, kx, ky, k0)[1]
+        J = J2
+    if biorth == 'both':
+        return b1, J, Jd, Jdc
+    return b1, J, (Jdc if biorth == 'cluster' else Jd)

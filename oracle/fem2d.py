@@ -740,8 +740,54 @@ def j_overlaps(m, sol1, sol2, ordre, bloch, k0):
     return J, Jd
 
 
+def biorth_clusters(L, rel=1e-10):
+    """Groups of modes coupled through the off-diagonal of the overlap
+    matrix L (|L_ij| > rel sqrt(|L_ii L_jj|)): near-degenerate modes whose
+    prime and adjoint vectors are not biorthogonal to working precision.
+    Returns a list of index arrays (singletons included)."""
+    n = L.shape[0]
+    d = np.sqrt(np.abs(np.diag(L)))
+    A = np.abs(L) > rel * np.outer(d, d)
+    np.fill_diagonal(A, False)
+    A = A | A.T
+    parent = list(range(n))
+
+    def find(i):
+        while parent[i] != i:
+            parent[i] = parent[parent[i]]
+            i = parent[i]
+        return i
+    for i, j in zip(*np.nonzero(A)):
+        a, b = find(i), find(j)
+        if a != b:
+            parent[max(a, b)] = min(a, b)
+    groups = {}
+    for i in range(n):
+        groups.setdefault(find(i), []).append(i)
+    return [np.array(g) for g in sorted(groups.values())]
+
+
+def adjoint_normaliser(L, biorth='diag'):
+    """Matrix N with J^dagger = N J^dagger_raw.  'diag' is the reference's
+    normalisation.f:15-31 (N = diag(1/L_ii), entries with |sqrt(L_ii)| <=
+    1e-8 left unscaled); 'cluster' inverts L within each group of
+    biorthogonally coupled modes (the same as 'diag' for isolated modes),
+    which gives the projector the exact eigenvectors would give."""
+    n = L.shape[0]
+    N = np.zeros((n, n), complex)
+    dg = np.sqrt(np.diag(L))
+    for k in range(n):
+        N[k, k] = 1.0 / dg[k] ** 2 if abs(dg[k]) > 1e-8 else 1.0
+    if biorth == 'diag':
+        return N
+    for g in biorth_clusters(L):
+        if g.size > 1 and np.all(np.abs(dg[g]) > 1e-8):
+            N[np.ix_(g, g)] = np.linalg.inv(L[np.ix_(g, g)])
+    return N
+
+
 def calc_modes_2d(wl_norm, nval, ordre_ls, mesh, n_eff, bloch_vec, shift,
-                  neq_PW, itermax=30, E_H_field=1, eig_tol=0.0, return_debug=False):
+                  neq_PW, itermax=30, E_H_field=1, eig_tol=0.0, return_debug=False, biorth='diag'):
     """Oracle of EMUstack.calc_modes_2d (py_calc_modes.f:1-958).
 
     mesh: a Mesh2D from build_mesh().  Returns (beta1, J, J_dagger) where J is
@@ -777,10 +823,8 @@ def calc_modes_2d(wl_norm, nval, ordre_ls, mesh, n_eff, bloch_vec, shift,
         beta2[[j, j2]] = beta2[[j2, j]]
         sol2[:, :, [j, j2]] = sol2[:, :, [j2, j]]
     L = overlap_matrix(m, sol1, sol2, beta1, k0, pp)
-    d = np.sqrt(np.diag(L))
-    scale = np.where(np.abs(d) > 1e-8, 1.0 / np.where(np.abs(d) > 1e-8, d, 1) ** 2, 1.0)
-    sol2 = sol2 * scale[None, None, :, None]
-    J, Jd = j_overlaps(m, sol1, sol2, ordre_ls, bloch_vec, k0)
+    J, Jd_raw = j_overlaps(m, sol1, sol2, ordre_ls, bloch_vec, k0)
+    Jd = adjoint_normaliser(L, biorth) @ Jd_raw
     if return_debug:
-        return beta1, J, Jd, dict(res=res, L=L, sol1=sol1, sol2=sol2, k0=k0)
+        return beta1, J, Jd, dict(res=res, L=L, sol1=sol1, sol2=sol2, k0=k0, Jd_raw=Jd_raw)
     return beta1, J, Jd

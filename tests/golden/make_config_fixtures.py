@@ -1,0 +1,212 @@
+"""Write tests/golden/configs/*.npz: the oracle's beta, J J^dagger and
+R/T/A on the seeded synthetic workloads (the meshes are committed next to
+them under tests/golden/meshes/), so that GPU tests can check the batched
+path at those shapes without running the oracle on the GPU box.
+
+  config1  square cell p = 600 nm, circle r = 120 nm, n = 3.5 + 0.01i in air,
+           200 nm slab on n = 1.5 under air; ~1,500-triangle mesh (seed 1);
+           80 Bloch modes, max_order_PWs = 2; wavelengths 400 and 800 nm,
+           normal incidence, TE.
+  config2  1D grating p = 400 nm, inclusions 80 nm (n = 2.0, centre) and
+           120 nm (n = 3.6 + 0.005i, cell edges) in n = 1.45, 150 nm on
+           n = 1.5 under air; 200 elements; 60 Bloch modes, max_order_PWs = 5;
+           wavelengths 450 / 900 nm x polar angles 0 / 20 deg (phi = 0), TE.
+  config3  the stacked-gratings workload of bench.py (three ~8,000-triangle
+           layers r = 100 / 150 / 80 nm, two 50 nm films, 200 Bloch modes,
+           max_order_PWs = 4) at one wavelength, normal incidence, TE.
+
+The oracle is the CPU restatement in oracle/ (scipy SuperLU + ARPACK);
+config 3 takes about an hour of CPU per layer.
+
+    python tests/golden/make_config_fixtures.py config1 config2 config3
+"""
+import hashlib
+import os
+import sys
+import tempfile
+from multiprocessing import Pool
+
+# one BLAS thread per worker process (set before numpy loads its BLAS)
+os.environ.setdefault('OMP_NUM_THREADS', '1')
+os.environ.setdefault('OPENBLAS_NUM_THREADS', '1')
+
+import numpy as np  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+OUT = os.path.join(HERE, 'configs')
+MESHES = os.path.join(HERE, 'meshes')
+
+C1 = dict(period=600.0, r=120.0, n=3.5 + 0.01j, h=200.0, ntri=1500, seed=1, nval=80, order=2,
+          wls=(400.0, 800.0))
+C2 = dict(period=400.0, diam=(80.0, 120.0), n=(1.45, 2.0, 3.6 + 0.005j), h=150.0, lc=0.005, nval=60, order=5,
+          wls=(450.0, 900.0), thetas=(0.0, 20.0))
+# bench.py's workload; the wavelength is entry 64 of linspace(400, 1000, 512)
+C3 = dict(period=500.0, radii=(100.0, 150.0, 80.0), heights=(100.0, 150.0, 120.0), ntri=8000, nval=200, order=4,
+          wl=float(np.linspace(400.0, 1000.0, 512)[64]))
+
+
+# SHA-256 of the seeded meshes the fixtures were computed on.  The config-1
+# mesh is committed; the three config-3 meshes (1.8 MB each) are written by
+# the project mesher on demand and checked against these digests.
+MESH_SHA256 = {
+    'config1_p600_r120.mail': '98922073cc54bc1e6b0a7869656a2d2e0d5cd4517ff1686db5693e0ceda5ee49',
+    'config3_r100_s1.mail': '58df7d1022fc5358c6ab3b4dc8bea27516ed925280101baa346cc9a40a45e7b7',
+    'config3_r150_s2.mail': '8a4c307eaf79968979e36084474af9a5671c76172570d28dacf8dbd41728a28e',
+    'config3_r80_s3.mail': '4508806418cf1cefe0da010b1527843336747521a977b29e8b3f8013dcea5729',
+}
+MESH_CACHE = os.path.join(tempfile.gettempdir(), 'emustack_amd_config_meshes')
+
+
+def _checked(path):
+    digest = hashlib.sha256(open(path, 'rb').read()).hexdigest()
+    if digest != MESH_SHA256[os.path.basename(path)]:
+        raise RuntimeError('mesh %s differs from the one the fixtures were made on' % path)
+    return path
+
+
+def config1_mesh():
+    return _checked(os.path.join(MESHES, 'config1_p600_r120.mail'))
+
+
+def config3_meshes():
+    """The three stacked-gratings meshes (bench.py's seeds), generated into a
+    cache directory when missing and checked against MESH_SHA256."""
+    from emustack_amd import mesher
+    os.makedirs(MESH_CACHE, exist_ok=True)
+    out = []
+    for i, r in enumerate(C3['radii']):
+        p = os.path.join(MESH_CACHE, 'config3_r%d_s%d.mail' % (int(r), i + 1))
+        if not os.path.exists(p):
+            tmp = p + '.%d.tmp' % os.getpid()
+            mesher.write_mail(tmp, mesher.ellipse_cell_ntri(C3['period'], C3['period'], r, r, C3['ntri'], seed=i + 1))
+            os.replace(tmp, p)
+        out.append(_checked(p))
+    return out
+
+
+def config3_indices():
+    """Inclusion indices of bench.py (seed 3, U[2, 4] + i U[0, 0.05])."""
+    rng = np.random.default_rng(3)
+    return [complex(rng.uniform(2.0, 4.0), rng.uniform(0.0, 0.05)) for _ in C3['radii']]
+
+
+def k_par_norm(wl, theta, phi, period):
+    """Light(theta, phi).k_pll * period, with the 1e-5 degree nudges."""
+    if abs(theta) < 1e-5:
+        theta += 1e-5
+    if abs(phi) < 1e-5:
+        phi += 1e-5
+    k = 2 * np.pi / wl
+    th, ph = np.deg2rad(theta), np.deg2rad(phi)
+    return k * np.sin(th) * np.array([np.cos(ph), np.sin(ph)]) * period
+
+
+def write_meshes():
+    from emustack_amd import mesher
+    os.makedirs(MESHES, exist_ok=True)
+    p = os.path.join(MESHES, 'config1_p600_r120.mail')
+    if not os.path.exists(p):
+        mesher.write_mail(p, mesher.ellipse_cell_ntri(C1['period'], C1['period'], C1['r'], C1['r'], C1['ntri'],
+                                                       seed=C1['seed']))
+    config1_mesh()
+    config3_meshes()
+
+
+def _modes_2d(args):
+    """beta, J and J^dagger of both normalisations: 'diag' (the reference's
+    1/L_ii) and 'cluster' (block inverse of L over near-degenerate modes)."""
+    path, wl, period, n_eff, nval, order = args
+    from oracle import fem2d
+    m = fem2d.build_mesh(path)
+    wln = wl / period
+    k0 = 2 * np.pi / wln
+    kp = k_par_norm(wl, 0.0, 0.0, period)
+    n_eff = np.asarray(n_eff, complex)
+    shift = 1.1 * np.real(n_eff).max() ** 2 * k0 ** 2 - kp[0] ** 2 - kp[1] ** 2
+    P = len(fem2d.pw_list(order))
+    b, J, Jd, dbg = fem2d.calc_modes_2d(wln, nval, order, m, n_eff, kp, shift, P, return_debug=True)
+    Jdc = fem2d.adjoint_normaliser(dbg['L'], 'cluster') @ dbg['Jd_raw']
+    return b, J, Jd, Jdc
+
+
+def _modes_1d(args):
+    wl, theta = args
+    from oracle import fem1d
+    x, tn, te = fem1d.grating_mesh_1d(C2['period'], C2['diam'], C2['lc'])
+    m = fem1d.build_mesh_1d(x, tn, te)
+    wln = wl / C2['period']
+    k0 = 2 * np.pi / wln
+    kp = k_par_norm(wl, theta, 0.0, C2['period'])
+    n_eff = np.asarray(C2['n'], complex)
+    shift = 1.1 * np.real(n_eff).max() ** 2 * k0 ** 2 - kp[0] ** 2 - kp[1] ** 2
+    return fem1d.calc_modes_1d(wln, C2['nval'], C2['order'], m, n_eff, kp[0], kp[1], shift, biorth='both')
+
+
+def config1(pool):
+    from oracle import stack_ref
+    res = pool.map(_modes_2d, [(config1_mesh(), wl, C1['period'], [1.0, C1['n']], C1['nval'], C1['order'])
+                               for wl in C1['wls']])
+    out = dict(wl=np.array(C1['wls']), beta=np.array([r[0] for r in res]))
+    for tag, k in (('', 2), ('_cluster', 3)):
+        out['JJd' + tag] = np.array([r[1] @ r[k] for r in res])
+        rta = []
+        for wl, r in zip(C1['wls'], res):
+            L = [dict(kind='tf', n=1.5, h=None), dict(kind='ns', kz=r[0], J=r[1], Jd=r[k], h=C1['h'] / C1['period']),
+                 dict(kind='tf', n=1.0, h=None)]
+            o = stack_ref.calc_scat(L, wl / C1['period'], k_par_norm(wl, 0, 0, C1['period']), C1['order'],
+                                    pol='TE')
+            rta.append([o['a_list'][-1], o['r_list'][-1], o['t_list'][-1]])
+        out['rta' + tag] = np.array(rta, float)
+    np.savez(os.path.join(OUT, 'config1.npz'), **out)
+
+
+def config2(pool):
+    from oracle import stack_ref
+    pts = [(wl, th) for wl in C2['wls'] for th in C2['thetas']]
+    res = pool.map(_modes_1d, pts)
+    out = dict(wl=np.array([p[0] for p in pts]), theta=np.array([p[1] for p in pts]),
+               beta=np.array([r[0] for r in res]))
+    for tag, k in (('', 2), ('_cluster', 3)):
+        out['JJd' + tag] = np.array([r[1] @ r[k] for r in res])
+        rta = []
+        for (wl, th), r in zip(pts, res):
+            L = [dict(kind='tf', n=1.5, h=None), dict(kind='ns', kz=r[0], J=r[1], Jd=r[k], h=C2['h'] / C2['period']),
+                 dict(kind='tf', n=1.0, h=None)]
+            o = stack_ref.calc_scat(L, wl / C2['period'], k_par_norm(wl, th, 0.0, C2['period']), C2['order'],
+                                    pol='TE', world_1d=True)
+            rta.append([o['a_list'][-1], o['r_list'][-1], o['t_list'][-1]])
+        out['rta' + tag] = np.array(rta, float)
+    np.savez(os.path.join(OUT, 'config2.npz'), **out)
+
+
+def config3(pool):
+    from oracle import stack_ref
+    wl = C3['wl']
+    res = pool.map(_modes_2d, [(p, wl, C3['period'], [1.0, n], C3['nval'], C3['order'])
+                               for p, n in zip(config3_meshes(), config3_indices())])
+    per = C3['period']
+    out = dict(wl=np.array([wl]), beta=np.array([r[0] for r in res]))
+    for tag, k in (('', 2), ('_cluster', 3)):
+        L = [dict(kind='tf', n=1.5, h=None)]
+        for i, r in enumerate(res):
+            if i:
+                L.append(dict(kind='tf', n=1.46, h=50.0 / per))
+            L.append(dict(kind='ns', kz=r[0], J=r[1], Jd=r[k], h=C3['heights'][i] / per))
+        L.append(dict(kind='tf', n=1.0, h=None))
+        o = stack_ref.calc_scat(L, wl / per, k_par_norm(wl, 0, 0, per), C3['order'], pol='TE')
+        out['JJd' + tag] = np.array([r[1] @ r[k] for r in res])
+        out['rta' + tag] = np.array([o['a_list'][-1], o['r_list'][-1], o['t_list'][-1]], float)
+    np.savez(os.path.join(OUT, 'config3.npz'), **out)
+
+
+if __name__ == '__main__':
+    os.makedirs(OUT, exist_ok=True)
+    write_meshes()
+    which = sys.argv[1:] or ['config1', 'config2']
+    with Pool(4) as pool:
+        for w in which:
+            globals()[w](pool)
+            print('wrote', w, flush=True)

@@ -1,0 +1,63 @@
+"""The mixed-precision form (precision='mixed': the Krylov solves read a
+complex64 copy of the FP64 inverted factor, the pairs converge against that
+operator to 1e-8, and every converged pair's beta is recomputed from the FP64
+pencil as the two-sided Rayleigh quotient of the prime mode and its adjoint
+partner) against the FP64 oracle's fixtures: R/T/A within 1e-6 (the bound
+the mixed form is held to); beta within 2e-6 relative (measured 3e-7 and 1e-6:
+the complex64 factor leaves the mode vectors ~1e-4 from the FP64 ones, and
+the FP64 Rayleigh quotient reduces that to the product of the prime and
+adjoint errors).  The measured errors are printed."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+sys.path.insert(0, os.path.join(GOLDEN))
+import make_config_fixtures as cf  # noqa: E402
+from test_gpu_configs import _films, _load, _rta, _structure_2d  # noqa: E402
+
+
+def test_mixed_config1_against_fp64_oracle():
+    from emustack_amd import objects, Stack
+    from emustack_amd.batch import calc_modes_many
+    fx = _load('config1')
+    C = cf.C1
+    ns = _structure_2d(cf.config1_mesh(), C['period'], C['r'], C['n'], C['h'])
+    lights = [objects.Light(wl, max_order_PWs=C['order'], theta=0.0, phi=0.0) for wl in fx['wl']]
+    sims = calc_modes_many(ns, lights, num_BMs=C['nval'], precision='mixed')
+    print('config1 mixed: restarts %d, OP applications %d' % tuple(calc_modes_many.last_stats[[1, 0]]))
+    sub, sup = _films(C['period'])
+    for i, (light, sim) in enumerate(zip(lights, sims)):
+        eb = np.abs(np.asarray(sim.k_z) - fx['beta'][i]).max() / np.abs(fx['beta'][i]).max()
+        st = Stack((sub.calc_modes(light), sim, sup.calc_modes(light)))
+        st.calc_scat(pol='TE')
+        er = np.abs(_rta(st) - fx['rta_cluster'][i]).max()
+        print('config1 mixed wl %g: beta rel err %.2e, R/T/A err %.2e' % (light.wl_nm, eb, er), flush=True)
+        assert eb <= 2e-6 and er <= 1e-6
+
+
+def test_mixed_config2_1d_against_fp64_oracle():
+    from emustack_amd import objects, materials, Stack
+    from emustack_amd.batch import calc_modes_many
+    fx = _load('config2')
+    C = cf.C2
+    g = objects.NanoStruct('1D_array', C['period'], C['diam'][0], diameter2=C['diam'][1], height_nm=C['h'],
+                           background=materials.Material(C['n'][0] + 0j), inclusion_a=materials.Material(C['n'][1]),
+                           inclusion_b=materials.Material(C['n'][2]), loss=True, lc_bkg=C['lc'])
+    lights = [objects.Light(wl, max_order_PWs=C['order'], theta=th, phi=0.0)
+              for wl, th in zip(fx['wl'], fx['theta'])]
+    sims = calc_modes_many(g, lights, num_BMs=C['nval'], precision='mixed')
+    sub, sup = _films(C['period'], world_1d=True)
+    for i, (light, sim) in enumerate(zip(lights, sims)):
+        eb = np.abs(np.asarray(sim.k_z) - fx['beta'][i]).max() / np.abs(fx['beta'][i]).max()
+        st = Stack((sub.calc_modes(light), sim, sup.calc_modes(light)))
+        st.calc_scat(pol='TE')
+        er = np.abs(_rta(st) - fx['rta_cluster'][i]).max()
+        print('config2 mixed wl %g theta %g: beta rel err %.2e, R/T/A err %.2e'
+              % (light.wl_nm, fx['theta'][i], eb, er), flush=True)
+        assert eb <= 2e-6 and er <= 1e-6

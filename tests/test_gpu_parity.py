@@ -1,9 +1,10 @@
 """GPU parity tests: every stage of the HIP path against the oracle on the
 same inputs, and the full drop-in API against the reference's golden R/T/A.
 Tolerances: assembly 1e-13 relative (same FP64 arithmetic, different
-summation order); LU residual 1e-8; beta 1e-9 relative; R/T/A 1e-6 against
-the reference files (their own tolerance) and 1e-7 against the oracle (two
-FP64 eigensolvers agree to the conditioning of clustered modes)."""
+summation order); LU residual 1e-8; beta 1e-9 relative; J J^dagger 1e-8 of
+its largest entry; R/T/A 1e-8 against the reference files (the reference
+itself compares at 1e-6) and 1e-9 against the oracle with the same
+normalisation of near-degenerate mode groups."""
 import os
 
 import numpy as np
@@ -77,7 +78,8 @@ def case1_500(gpu_ctx, mesh600):
     wln, k0, n_eff, kpn, shift = _params(500.0, 600., nsi)
     mh, m = mesh600
     g = gpu_ctx.calc_modes_batch(mh, [wln], 40, 2, [n_eff], [kpn], [shift], 13)
-    o = fem2d.calc_modes_2d(wln, 40, 2, m, n_eff, kpn, shift, 13)
+    # the oracle with the GPU's normalisation of near-degenerate groups
+    o = fem2d.calc_modes_2d(wln, 40, 2, m, n_eff, kpn, shift, 13, biorth='cluster')
     return g, o, (wln, kpn)
 
 
@@ -93,7 +95,7 @@ def test_mode_projectors_match_oracle(case1_500):
     (beta, J, Jd, st), (bo, Jo, Jdo), _ = case1_500
     Pg = J[0] @ Jd[0]
     Po = Jo @ Jdo
-    assert np.abs(Pg - Po).max() <= 1e-6 * np.abs(Po).max()
+    assert np.abs(Pg - Po).max() <= 1e-8 * np.abs(Po).max()
     # non-degenerate modes agree column by column (eigenvector accuracy of a
     # 1e-12 Ritz residual divided by the eigenvalue gap)
     gaps = np.abs(np.diff(bo))
@@ -113,9 +115,10 @@ def test_rta_matches_reference_and_oracle(case1_500, mats):
         res[name] = np.array([o['a_list'][-1], o['r_list'][-1], o['t_list'][-1]], float)
     ref = np.array([load_txt('case_1', f)[0, 1] for f in ('Absorptance_stack0001.txt', 'Reflectance_stack0001.txt',
                                                          'Transmittance_stack0001.txt')])
-    np.testing.assert_allclose(res['gpu'], ref, rtol=1e-6, atol=1e-6)
-    # two FP64 solvers agree to the conditioning of the clustered modes (~1e-8)
-    np.testing.assert_allclose(res['gpu'], res['oracle'], rtol=0, atol=1e-7)
+    # the golden files carry the per-mode normalisation's ~5e-9 ambiguity of
+    # this normal-incidence cell (its near-degenerate pairs)
+    np.testing.assert_allclose(res['gpu'], ref, rtol=0, atol=1e-8)
+    np.testing.assert_allclose(res['gpu'], res['oracle'], rtol=0, atol=1e-9)
 
 
 def test_batched_equals_single(gpu_ctx, mesh600, mats):
@@ -163,7 +166,7 @@ def test_dropin_api_case1():
                     ('Reflectance_stack0001.txt', lambda s: s.r_list[-1]),
                     ('Transmittance_stack0001.txt', lambda s: s.t_list[-1])):
         ref = load_txt('case_1', fn)[:, 1]
-        np.testing.assert_allclose([float(get(s)) for s in stacks], ref, rtol=1e-6, atol=1e-6, err_msg=fn)
+        np.testing.assert_allclose([float(np.real(get(s))) for s in stacks], ref, rtol=0, atol=1e-8, err_msg=fn)
 
 
 def test_dropin_api_case2():
@@ -171,7 +174,7 @@ def test_dropin_api_case2():
     s = _api_stack(500.0, 20, 40, 20, 1, 'TE')
     for fn, v in (('Absorptance_stack0001.txt', s.a_list[-1]), ('Reflectance_stack0001.txt', s.r_list[-1]),
                   ('Transmittance_stack0001.txt', s.t_list[-1])):
-        np.testing.assert_allclose(float(v), load_txt('case_2', fn)[1], rtol=1e-6, atol=1e-6, err_msg=fn)
+        np.testing.assert_allclose(float(np.real(v)), load_txt('case_2', fn)[1], rtol=0, atol=1e-8, err_msg=fn)
 
 
 def test_stack_batch_equals_per_layer(gpu_ctx):
