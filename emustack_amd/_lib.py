@@ -39,7 +39,7 @@ EXPORTS = [
     'emu_mesh_analyze', 'emu_mesh_symbolic', 'emu_mesh_free', 'emu_mesh_info', 'emu_mesh_arrays',
     'emu_mesh_geometry', 'emu_calc_modes_2d', 'emu_calc_modes_2d_batch', 'emu_assemble', 'emu_lu_solve',
     'emu_profile', 'emu_mesh_create_1d', 'emu_calc_modes_1d', 'emu_calc_modes_1d_batch', 'emu_star_cascade_batch',
-    'emu_material_index',
+    'emu_material_index', 'emu_assemble_noalloc',
 ]
 
 
@@ -144,6 +144,7 @@ def lib():
                                               c_i64, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp,
                                               c_i64p]
         L.emu_material_index.argtypes = [c_vp, c_i64, c_i64p, c_i64p, c_dp, c_dp, c_i64, c_i64p, c_dp, c_dp]
+        L.emu_assemble_noalloc.argtypes = [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]
         for f in EXPORTS:
             getattr(L, f)
         rc = L.emu_set_lapack(*_lapack_pointers())

@@ -328,6 +328,16 @@ int emu_assemble(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, int64_t nb_typ_el
   EMU_GUARD(emu::assemble(ctx->c, *mesh->m, (int)nbatch, (int)nb_typ_el, pp, qq, shift, bloch, mat1, mat2))
 }
 
+int emu_assemble_noalloc(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, int64_t nb_typ_el, const double *pp,
+                         const double *qq, const double *shift, const double *bloch, double *mat1, double *mat2,
+                         void *stream) {
+  if (!ctx || !mesh || !pp || !qq || !shift || !bloch || !mat1 || !mat2)
+    return fail(EMU_E_ARG, "emu_assemble_noalloc: NULL argument");
+  if (nbatch < 1 || nb_typ_el < 1) return fail(EMU_E_ARG, "emu_assemble_noalloc: invalid size");
+  EMU_GUARD(emu::assemble_noalloc(ctx->c, *mesh->m, (int)nbatch, (int)nb_typ_el, pp, qq, shift, bloch, mat1, mat2,
+                                  stream))
+}
+
 int emu_lu_solve(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *mat1, int64_t nrhs, const double *b,
                  double *x, int64_t *stats) {
   if (!ctx || !mesh || !mat1 || !b || !x) return fail(EMU_E_ARG, "emu_lu_solve: NULL argument");

@@ -358,6 +358,7 @@ Ctx *ctx_create(int device) {
   Ctx *c = new Ctx();
   c->device = device;
   EMU_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  noalloc_scratch(device);  // per-device address of the fixed scratch, with the device current
   return c;
 }
 void ctx_destroy(Ctx *c) { delete c; }

@@ -87,6 +87,15 @@ struct CascadeArgs {
 };
 void star_cascade(Ctx *c, const CascadeArgs &a);
 
+// Allocation-free assembly (noalloc.hip): scratch in a fixed device array
+constexpr size_t kNoallocScratchBytes = (size_t)16 << 20;  // 16 MiB per GPU
+constexpr int kMaxDevices = 64;
+void *noalloc_scratch(int device);
+size_t assemble_noalloc_scratch(const DevMesh &dm, int64_t nb);
+// device pointers in and out (complex interleaved), stream NULL = the context's
+void assemble_noalloc(Ctx *c, DevMesh &dm, int nb, int ntyp, const double *pp, const double *qq,
+                      const double *shift, const double *bloch, double *mat1, double *mat2, void *stream);
+
 // n(lambda) of nq (material, wavelength) queries (kernels_material.hip)
 void material_index(Ctx *c, int64_t nmat, const int64_t *kind, const int64_t *off, const double *xs,
                     const double *ys, int64_t nq, const int64_t *qm, const double *qw, double *out);

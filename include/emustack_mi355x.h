@@ -227,6 +227,21 @@ int emu_assemble(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, int64_t nb_typ_el
                  const double *pp, const double *qq, const double *shift,
                  const double *bloch, double *mat1, double *mat2);
 
+/* Allocation-free variant of emu_assemble.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as emu_assemble, except that every
+ * array is a DEVICE pointer (pp, qq [nbatch*2*nb_typ_el], shift [nbatch*2],
+ * bloch [nbatch*2] in; mat1, mat2 [nbatch*2*nnz] out, allocated by the
+ * caller) and the call is asynchronous on `stream` (NULL: the context's
+ * stream).  The kernels' only scratch (per-element Bloch phases, nbatch x
+ * nel x 224 bytes for 2D meshes, none for 1D) lives in a fixed 16 MiB
+ * device array per GPU; larger requests fail with EMU_E_ARG before any
+ * launch.  Calls that overlap on one GPU share that array and race. */
+int emu_assemble_noalloc(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, int64_t nb_typ_el, const double *pp,
+                         const double *qq, const double *shift, const double *bloch, double *mat1, double *mat2,
+                         void *stream);
+
 /* Factor mat1 (host, nbatch*2*nnz, CSR order) with the GPU multifrontal LU and
  * solve mat1 x = b for nrhs right-hand sides per instance
  * (b, x: nbatch*nrhs*2*neq, column-major per instance).  stats[0] = pivot
