@@ -168,8 +168,10 @@ A, B = fem2d.assemble(m, np.ones(2, complex), n_eff ** 2 * k0 ** 2, shift, kpn)
 t_asm = time.perf_counter() - t0
 Ap = A.tocsr()[perm][:, perm].tocsc(); Bp = B.tocsr()[perm][:, perm].tocsr()
 t0 = time.perf_counter()
-# threshold partial pivoting (SuperLU's default) on the nested-dissection order
-lu = spla.splu(Ap, permc_spec='NATURAL')
+# threshold partial pivoting with diagonal preference on the nested-dissection
+# order (the symmetric strategy UMFPACK applies to this structurally
+# symmetric pencil, pivot tolerance 0.001)
+lu = spla.splu(Ap, permc_spec='NATURAL', diag_pivot_thresh=0.001, options=dict(SymmetricMode=True))
 t_lu = time.perf_counter() - t0
 v = np.random.default_rng(0).standard_normal(m.neq) + 0j
 t0 = time.perf_counter()
@@ -202,7 +204,8 @@ def cpu_baseline(mesh_paths, wls, n_incls, ops_per_instance, cores, nops=40):
     """The oracle's FP64 path on the host cores: `cores` single-threaded
     processes at once, each timing on one layer mesh and wavelength the mesh
     preprocessing, assembly, SuperLU factorisation (threshold partial
-    pivoting on the nested-dissection order) and `nops` shift-invert OP
+    pivoting, tolerance 0.001 with diagonal preference, on the
+    nested-dissection order) and `nops` shift-invert OP
     applications and CGS2 steps of an ARPACK-sized basis; an eigensolve is
     assembly + factorisation + ops_per_instance x (OP + CGS2), a point 6
     eigensolves, and the cores work on independent points."""
@@ -217,7 +220,7 @@ def cpu_baseline(mesh_paths, wls, n_incls, ops_per_instance, cores, nops=40):
     value = cores / (6.0 * t_eig)
     d = res[0]
     sample = ('%d processes at once (one core each, %.0f s wall): per process one layer mesh at one wavelength, '
-              'assembly %.1f s + SuperLU with partial pivoting on the ND order %.1f s + %d OP applications '
+              'assembly %.1f s + SuperLU (threshold pivoting 0.001, ND order) %.1f s + %d OP applications '
               '(%.1f ms each) and CGS2 steps (%.1f ms each); an eigensolve = assembly + LU + %d x (OP + CGS2) '
               '(the GPU run\'s OP count), 6 eigensolves per point, points independent across cores'
               % (cores, wall, d['t_asm'], d['t_lu'], nops, 1e3 * d['t_op'], 1e3 * d['t_orth'], ops_per_instance))
