@@ -60,26 +60,35 @@ def calc_modes_2d(lambda_, nval, ordre_ls, debug, msh_path, mesh_file, npt, nel,
     """Bloch modes of a 2D-periodic layer (py_calc_modes.f:1-958).
 
     Returns (beta1[nval], overlap_J[2 neq_PW, nval], overlap_J_dagger[nval, 2 neq_PW],
-    sol1, mode_pol, table_nod[6, nel], type_el[nel], x_arr[2, npt]).  The field
-    arrays sol1 and mode_pol are only used for plotting by the reference
-    (mode_calcs.py:396-408) and are returned as None.
+    sol1, mode_pol, table_nod[6, nel], type_el[nel], x_arr[2, npt]).  With
+    plot_modes == 1, sol1[3, 13, nval, nel] holds the prime modes' fields at
+    the 6 P2 nodes and Ez at the 7 further P3 nodes (array_sol.f:120-339),
+    and mode_pol[4, nval] the x / y / z / core fractions of eps |E|^2 of the
+    adjoint modes (mode_energy.f:60-141); otherwise both are None (the
+    reference fills them with values only plotting reads,
+    mode_calcs.py:396-408).  plot_real / plot_imag / plot_abs select the
+    reference's gmsh post-processing files and are not used.
     cmplx_max, real_max and int_max size the reference's Fortran work arrays
     and are accepted and ignored.
     """
     if i_cond != 2:
         raise ValueError("only periodic boundary conditions (i_cond=2) are supported")
-    if plot_modes == 1:
-        raise NotImplementedError("field plotting output is not produced by the GPU path")
     h = mesh(msh_path + mesh_file)
     if h.npt != npt or h.nel != nel:
         raise ValueError("mesh header (%d, %d) does not match npt, nel = (%d, %d)" % (h.npt, h.nel, npt, nel))
     n_eff = np.asarray(n_eff, np.complex128).reshape(-1)
     if n_eff.size != nb_typ_el:
         raise ValueError("n_eff must have nb_typ_el entries")
+    table_nod, type_el, x_arr = h.geometry()
+    if plot_modes == 1:
+        beta, J, Jd, sol1, pol = context().calc_modes_2d_fields(
+            h, lambda_, int(nval), int(ordre_ls), n_eff, np.asarray(bloch_vec, float), complex(shift), int(neq_PW),
+            E_H_field=int(E_H_field), itermax=int(itermax))
+        return (beta, np.asfortranarray(J), np.asfortranarray(Jd), np.asfortranarray(sol1), np.asfortranarray(pol),
+                table_nod, type_el, x_arr)
     beta, J, Jd, _ = context().calc_modes_batch(h, [lambda_], int(nval), int(ordre_ls), [n_eff],
                                                 [np.asarray(bloch_vec, float)], [complex(shift)], int(neq_PW),
                                                 E_H_field=int(E_H_field), itermax=int(itermax))
-    table_nod, type_el, x_arr = h.geometry()
     return (beta[0], np.asfortranarray(J[0]), np.asfortranarray(Jd[0]), None, None, table_nod, type_el, x_arr)
 
 
@@ -105,10 +114,10 @@ def calc_modes_1d(lambda_, nval, ordre_ls, nb_typ_el, npt_P2, nel, table_nod, ty
 
     Returns (beta_1[nval], overlap_J[2 neq_PW, nval], overlap_J_dagger[nval, 2 neq_PW],
     overlap_J_2d, overlap_J_dagger_2d, sol_P2); the 2D overlaps are filled when
-    world_1d is 0 (else zero arrays of the reference's shapes), sol_P2 (plotting
-    only) is None."""
-    if plot_modes == 1:
-        raise NotImplementedError("field plotting output is not produced by the GPU path")
+    world_1d is 0 (else zero arrays of the reference's shapes).  With
+    plot_modes == 1, sol_P2[3, 3, nval, nel] holds the prime modes after
+    normalisation_1d at each element's two ends and mid-node
+    (array_sol_P2_1d.f); otherwise it is None."""
     tn = np.asarray(table_nod, np.int64)
     te = np.asarray(type_el, np.int64).reshape(-1)
     x = np.asarray(x_P2, np.float64).reshape(-1)
@@ -119,15 +128,25 @@ def calc_modes_1d(lambda_, nval, ordre_ls, nb_typ_el, npt_P2, nel, table_nod, ty
     if n_eff.size != nb_typ_el:
         raise ValueError("n_eff must have nb_typ_el entries")
     w1 = bool(world_1d)
-    beta, J, Jd, J2, Jd2, _ = context().calc_modes_1d_batch(
-        h, [lambda_], int(nval), int(ordre_ls), [n_eff], [[bloch_vec_x, bloch_vec_y]], [complex(shift)],
-        int(neq_PW), neq_pw_2d=int(neq_PW_2d), world_1d=w1, itermax=int(itermax))
+    if plot_modes == 1:
+        beta, J, Jd, J2, Jd2, sol = context().calc_modes_1d_fields(
+            h, lambda_, int(nval), int(ordre_ls), n_eff, [bloch_vec_x, bloch_vec_y], complex(shift), int(neq_PW),
+            neq_pw_2d=int(neq_PW_2d), world_1d=w1, itermax=int(itermax))
+        sol = np.asfortranarray(sol)
+    else:
+        beta, J, Jd, J2, Jd2, _ = context().calc_modes_1d_batch(
+            h, [lambda_], int(nval), int(ordre_ls), [n_eff], [[bloch_vec_x, bloch_vec_y]], [complex(shift)],
+            int(neq_PW), neq_pw_2d=int(neq_PW_2d), world_1d=w1, itermax=int(itermax))
+        beta, J, Jd = beta[0], J[0], Jd[0]
+        if not w1:
+            J2, Jd2 = J2[0], Jd2[0]
+        sol = None
     if w1:
         J2 = np.zeros((2 * neq_PW_2d, nval), np.complex128, order='F')
         Jd2 = np.zeros((nval, 2 * neq_PW_2d), np.complex128, order='F')
     else:
-        J2, Jd2 = np.asfortranarray(J2[0]), np.asfortranarray(Jd2[0])
-    return beta[0], np.asfortranarray(J[0]), np.asfortranarray(Jd[0]), J2, Jd2, None
+        J2, Jd2 = np.asfortranarray(J2), np.asfortranarray(Jd2)
+    return beta, np.asfortranarray(J), np.asfortranarray(Jd), J2, Jd2, sol
 
 
 def calc_modes_2d_batch(lambdas, nval, ordre_ls, mesh_path, n_effs, bloch_vecs, shifts, neq_PW, E_H_field=1,

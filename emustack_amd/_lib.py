@@ -39,7 +39,7 @@ EXPORTS = [
     'emu_mesh_analyze', 'emu_mesh_symbolic', 'emu_mesh_free', 'emu_mesh_info', 'emu_mesh_arrays',
     'emu_mesh_geometry', 'emu_calc_modes_2d', 'emu_calc_modes_2d_batch', 'emu_assemble', 'emu_lu_solve',
     'emu_profile', 'emu_mesh_create_1d', 'emu_calc_modes_1d', 'emu_calc_modes_1d_batch', 'emu_star_cascade_batch',
-    'emu_material_index', 'emu_assemble_noalloc',
+    'emu_material_index', 'emu_assemble_noalloc', 'emu_calc_modes_2d_fields', 'emu_calc_modes_1d_fields',
 ]
 
 
@@ -145,6 +145,11 @@ def lib():
                                               c_i64p]
         L.emu_material_index.argtypes = [c_vp, c_i64, c_i64p, c_i64p, c_dp, c_dp, c_i64, c_i64p, c_dp, c_dp]
         L.emu_assemble_noalloc.argtypes = [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]
+        L.emu_calc_modes_2d_fields.argtypes = [c_vp, c_vp, ctypes.c_double, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp,
+                                               c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_dp]
+        L.emu_calc_modes_1d_fields.argtypes = [c_vp, c_vp, ctypes.c_double, c_i64, c_i64, c_i64, c_dp,
+                                               ctypes.c_double, ctypes.c_double, c_dp, c_i64, c_i64, c_i64, c_i64,
+                                               c_dp, c_dp, c_dp, c_dp, c_dp, c_dp]
         for f in EXPORTS:
             getattr(L, f)
         rc = L.emu_set_lapack(*_lapack_pointers())
@@ -229,6 +234,49 @@ class Context:
                                                 ptr(beta.view(np.float64)), ptr(J.view(np.float64)),
                                                 ptr(Jd.view(np.float64)), ptr(st, c_i64p)))
         return beta, J.transpose(0, 2, 1), Jd.transpose(0, 2, 1), st
+
+    def calc_modes_2d_fields(self, mesh, lam, nval, ordre, n_eff, bloch, shift, neq_pw, E_H_field=1, itermax=30):
+        """One point with the plotting fields: beta (nval,), J (2P, nval),
+        Jd (nval, 2P), sol1 (3, 13, nval, nel) and mode_pol (4, nval)."""
+        n_eff = np.ascontiguousarray(np.asarray(n_eff, np.complex128).reshape(-1))
+        bloch = np.ascontiguousarray(np.asarray(bloch, np.float64).reshape(2))
+        sh = np.array([complex(shift)])
+        beta = np.zeros(nval, np.complex128)
+        J = np.zeros((nval, 2 * neq_pw), np.complex128)
+        Jd = np.zeros((2 * neq_pw, nval), np.complex128)
+        sol = np.zeros((mesh.nel, nval, 13, 3), np.complex128)
+        pol = np.zeros((nval, 4), np.complex128)
+        with _host_lapack_threads(1):
+            check(lib().emu_calc_modes_2d_fields(self.h, mesh.h, float(lam), nval, ordre, n_eff.size,
+                                                 ptr(n_eff.view(np.float64)), ptr(bloch), ptr(sh.view(np.float64)),
+                                                 E_H_field, itermax, neq_pw, ptr(beta.view(np.float64)),
+                                                 ptr(J.view(np.float64)), ptr(Jd.view(np.float64)),
+                                                 ptr(sol.view(np.float64)), ptr(pol.view(np.float64))))
+        return beta, J.T, Jd.T, sol.transpose(3, 2, 1, 0), pol.T
+
+    def calc_modes_1d_fields(self, mesh, lam, nval, ordre, n_eff, bloch, shift, neq_pw, neq_pw_2d=1, world_1d=True,
+                             itermax=30):
+        """One 1D point with sol_P2 (3, 3, nval, nel): beta, J, Jd, J2d, Jd2d,
+        sol_P2 (J2d / Jd2d None for a 1D world)."""
+        n_eff = np.ascontiguousarray(np.asarray(n_eff, np.complex128).reshape(-1))
+        sh = np.array([complex(shift)])
+        beta = np.zeros(nval, np.complex128)
+        J = np.zeros((nval, 2 * neq_pw), np.complex128)
+        Jd = np.zeros((2 * neq_pw, nval), np.complex128)
+        J2 = np.zeros((nval, 2 * neq_pw_2d), np.complex128)
+        Jd2 = np.zeros((2 * neq_pw_2d, nval), np.complex128)
+        sol = np.zeros((mesh.nel, nval, 3, 3), np.complex128)
+        with _host_lapack_threads(1):
+            check(lib().emu_calc_modes_1d_fields(self.h, mesh.h, float(lam), nval, ordre, n_eff.size,
+                                                 ptr(n_eff.view(np.float64)), float(bloch[0]), float(bloch[1]),
+                                                 ptr(sh.view(np.float64)), itermax, neq_pw, neq_pw_2d,
+                                                 1 if world_1d else 0, ptr(beta.view(np.float64)),
+                                                 ptr(J.view(np.float64)), ptr(Jd.view(np.float64)),
+                                                 ptr(J2.view(np.float64)), ptr(Jd2.view(np.float64)),
+                                                 ptr(sol.view(np.float64))))
+        if world_1d:
+            return beta, J.T, Jd.T, None, None, sol.transpose(3, 2, 1, 0)
+        return beta, J.T, Jd.T, J2.T, Jd2.T, sol.transpose(3, 2, 1, 0)
 
     def calc_modes_1d_batch(self, mesh, lam, nval, ordre, n_eff, bloch, shift, neq_pw, neq_pw_2d=0,
                             world_1d=True, itermax=30, tol=0.0, block=0, ncv=0, precision=0):

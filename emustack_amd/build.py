@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, 'csrc')
 OBJ = os.path.join(HERE, '_build')
 LIB = os.path.join(HERE, 'libemu_mi355x.so')
 SOURCES = ['mesh.cpp', 'mesh1d.cpp', 'symbolic.cpp', 'capi.cpp', 'kernels_fem.hip', 'kernels_lu.hip',
-           'kernels_krylov.hip', 'kernels_post.hip', 'kernels_1d.hip', 'cascade.hip', 'kernels_material.hip', 'noalloc.hip', 'engine.hip']
+           'kernels_krylov.hip', 'kernels_post.hip', 'kernels_1d.hip', 'cascade.hip', 'kernels_material.hip', 'kernels_fields.hip', 'noalloc.hip', 'engine.hip']
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 ARCH = os.environ.get('EMU_OFFLOAD_ARCH', 'gfx950')
 # packed-fp32 instructions (v_pk_*_f32) are disabled for all device code:

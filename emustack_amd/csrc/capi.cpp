@@ -140,11 +140,13 @@ int emu_mesh_geometry(const emu_mesh *mesh, int64_t *table_nod, int64_t *type_el
   EMU_GUARD(emu::mesh_geometry(mesh->m, table_nod, type_el, x_arr))
 }
 
-int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *lambda, int64_t nval,
+}  // extern "C"
+
+static int modes2d_impl(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *lambda, int64_t nval,
                             int64_t ordre_ls, int64_t nb_typ_el, const double *n_eff, const double *bloch_vec,
                             const double *shift, int64_t E_H_field, int64_t itermax, int64_t neq_PW,
                             const double *opts, double *beta1, double *overlap_J, double *overlap_J_dagger,
-                            int64_t *stats) {
+                            int64_t *stats, double *sol, double *mode_pol) {
   if (!ctx || !mesh || !lambda || !n_eff || !bloch_vec || !shift || !beta1 || !overlap_J || !overlap_J_dagger)
     return fail(EMU_E_ARG, "emu_calc_modes_2d_batch: NULL argument");
   if (nbatch < 1 || nval < 1 || nb_typ_el < 1 || neq_PW < 1 || itermax < 0)
@@ -179,8 +181,21 @@ int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const 
     a.J = overlap_J;
     a.Jd = overlap_J_dagger;
     a.stats = stats;
+    a.sol = sol;
+    a.mode_pol = mode_pol;
     emu::calc_modes(ctx->c, *mesh->m, a);
   })
+}
+
+extern "C" {
+
+int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *lambda, int64_t nval,
+                            int64_t ordre_ls, int64_t nb_typ_el, const double *n_eff, const double *bloch_vec,
+                            const double *shift, int64_t E_H_field, int64_t itermax, int64_t neq_PW,
+                            const double *opts, double *beta1, double *overlap_J, double *overlap_J_dagger,
+                            int64_t *stats) {
+  return modes2d_impl(ctx, mesh, nbatch, lambda, nval, ordre_ls, nb_typ_el, n_eff, bloch_vec, shift, E_H_field,
+                      itermax, neq_PW, opts, beta1, overlap_J, overlap_J_dagger, stats, nullptr, nullptr);
 }
 
 int emu_calc_modes_2d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval, int64_t ordre_ls,
@@ -189,6 +204,15 @@ int emu_calc_modes_2d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval,
                       double *overlap_J_dagger) {
   return emu_calc_modes_2d_batch(ctx, mesh, 1, &lambda, nval, ordre_ls, nb_typ_el, n_eff, bloch_vec, shift,
                                  E_H_field, itermax, neq_PW, nullptr, beta1, overlap_J, overlap_J_dagger, nullptr);
+}
+
+int emu_calc_modes_2d_fields(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval, int64_t ordre_ls,
+                             int64_t nb_typ_el, const double *n_eff, const double *bloch_vec, const double *shift,
+                             int64_t E_H_field, int64_t itermax, int64_t neq_PW, double *beta1, double *overlap_J,
+                             double *overlap_J_dagger, double *sol1, double *mode_pol) {
+  if (!sol1 && !mode_pol) return fail(EMU_E_ARG, "emu_calc_modes_2d_fields: no field output requested");
+  return modes2d_impl(ctx, mesh, 1, &lambda, nval, ordre_ls, nb_typ_el, n_eff, bloch_vec, shift, E_H_field, itermax,
+                      neq_PW, nullptr, beta1, overlap_J, overlap_J_dagger, nullptr, sol1, mode_pol);
 }
 
 int emu_mesh_create_1d(emu_ctx *ctx, int64_t npt_P2, int64_t nel, const int64_t *table_nod,
@@ -207,12 +231,14 @@ int emu_mesh_create_1d(emu_ctx *ctx, int64_t npt_P2, int64_t nel, const int64_t 
   })
 }
 
-int emu_calc_modes_1d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *lambda, int64_t nval,
+}  // extern "C"
+
+static int modes1d_impl(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *lambda, int64_t nval,
                             int64_t ordre_ls, int64_t nb_typ_el, const double *n_eff, const double *bloch_vec,
                             const double *shift, int64_t itermax, int64_t neq_PW, int64_t neq_PW_2d,
                             int64_t world_1d, const double *opts, double *beta1, double *overlap_J,
                             double *overlap_J_dagger, double *overlap_J_2d, double *overlap_J_dagger_2d,
-                            int64_t *stats) {
+                            int64_t *stats, double *sol) {
   if (!ctx || !mesh || !lambda || !n_eff || !bloch_vec || !shift || !beta1 || !overlap_J || !overlap_J_dagger)
     return fail(EMU_E_ARG, "emu_calc_modes_1d_batch: NULL argument");
   if (nbatch < 1 || nval < 1 || nb_typ_el < 1 || neq_PW < 1 || itermax < 0 || ordre_ls < 0)
@@ -253,8 +279,22 @@ int emu_calc_modes_1d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const 
     a.J2d = overlap_J_2d;
     a.Jd2d = overlap_J_dagger_2d;
     a.stats = stats;
+    a.sol = sol;
     emu::calc_modes_1d(ctx->c, *mesh->m, a);
   })
+}
+
+extern "C" {
+
+int emu_calc_modes_1d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *lambda, int64_t nval,
+                            int64_t ordre_ls, int64_t nb_typ_el, const double *n_eff, const double *bloch_vec,
+                            const double *shift, int64_t itermax, int64_t neq_PW, int64_t neq_PW_2d,
+                            int64_t world_1d, const double *opts, double *beta1, double *overlap_J,
+                            double *overlap_J_dagger, double *overlap_J_2d, double *overlap_J_dagger_2d,
+                            int64_t *stats) {
+  return modes1d_impl(ctx, mesh, nbatch, lambda, nval, ordre_ls, nb_typ_el, n_eff, bloch_vec, shift, itermax, neq_PW,
+                      neq_PW_2d, world_1d, opts, beta1, overlap_J, overlap_J_dagger, overlap_J_2d, overlap_J_dagger_2d,
+                      stats, nullptr);
 }
 
 int emu_calc_modes_1d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval, int64_t ordre_ls,
@@ -266,6 +306,18 @@ int emu_calc_modes_1d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval,
   return emu_calc_modes_1d_batch(ctx, mesh, 1, &lambda, nval, ordre_ls, nb_typ_el, n_eff, bl, shift, itermax,
                                  neq_PW, neq_PW_2d, world_1d, nullptr, beta1, overlap_J, overlap_J_dagger,
                                  overlap_J_2d, overlap_J_dagger_2d, nullptr);
+}
+
+int emu_calc_modes_1d_fields(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval, int64_t ordre_ls,
+                             int64_t nb_typ_el, const double *n_eff, double bloch_vec_x, double bloch_vec_y,
+                             const double *shift, int64_t itermax, int64_t neq_PW, int64_t neq_PW_2d,
+                             int64_t world_1d, double *beta1, double *overlap_J, double *overlap_J_dagger,
+                             double *overlap_J_2d, double *overlap_J_dagger_2d, double *sol_P2) {
+  if (!sol_P2) return fail(EMU_E_ARG, "emu_calc_modes_1d_fields: sol_P2 is NULL");
+  const double bl[2] = {bloch_vec_x, bloch_vec_y};
+  return modes1d_impl(ctx, mesh, 1, &lambda, nval, ordre_ls, nb_typ_el, n_eff, bl, shift, itermax, neq_PW, neq_PW_2d,
+                      world_1d, nullptr, beta1, overlap_J, overlap_J_dagger, overlap_J_2d, overlap_J_dagger_2d,
+                      nullptr, sol_P2);
 }
 
 int emu_star_cascade_batch(emu_ctx *ctx, int64_t nbatch, int64_t nlayers, int64_t neq_PW, const int64_t *kind,

@@ -340,6 +340,13 @@ void launch_gather_1d(DevMesh &dm, int nb, int ntyp, const zd *pp, const zd *qq,
 // prime k_y, phase from bloch[b][0]
 void launch_pw_load_1d(DevMesh &dm, int nb, int P, const double *alpha, const double *ky, const int32_t *sign,
                        const double *bloch, zd *G, hipStream_t s);
+// plotting fields (kernels_fields.hip): 13-node fields of one instance's
+// modes; their mode_pol fractions (beta: nval, eps: per material type;
+// part: 4 nval nel scratch); sol_P2 of a 1D instance
+void launch_fields_2d(DevMesh &dm, int nval, const zd *X, int64_t ldx, const zd *ve, zd *sol, hipStream_t s);
+void launch_mode_pol_2d(DevMesh &dm, int nval, const zd *X, int64_t ldx, const zd *ve, const zd *beta,
+                        const zd *eps, int core_t, zd *part, zd *pol, hipStream_t s);
+void launch_sol_p2_1d(DevMesh &dm, int nval, const zd *X, int64_t ldx, double kx, zd *out, hipStream_t s);
 // array_sol_1d.f normalisation value per mode: den[b][k]
 void launch_field_max_1d(DevMesh &dm, int nb, int nval, const zd *X, int64_t sX, const double *bloch, double xmid,
                          double dq, zd *den, hipStream_t s);

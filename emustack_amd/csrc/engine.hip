@@ -1363,6 +1363,43 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   double *fms = c->buf<double>("fmax_scratch", (size_t)nb * nval * nparts * 4);
   launch_field_max(dm, nb, nval, Xn, n, n * nval, ve, false, zmax, fms, s);
   launch_scale_cols(nb, n, nval, Xn, n, n * nval, zmax, nval, true, s);
+  if (a.sol || a.mode_pol) {
+    // plotting fields: sol1 of the prime modes as array_sol.f leaves them,
+    // and mode_pol of the adjoint modes in their own sorted order (the
+    // reference's last writer of mode_pol is mode_energy on the adjoint
+    // family, py_calc_modes.f:709-714); the core material is type 1 when
+    // Re eps_1 > Re eps_2, else type 2 (py_calc_modes.f:546-551)
+    const size_t nsol = (size_t)39 * nval * nel;
+    zd *fsol = a.sol ? c->buf<zd>("plot_sol", nsol) : nullptr;
+    zd *fpart = a.mode_pol ? c->buf<zd>("plot_part", (size_t)4 * nval * nel) : nullptr;
+    zd *fpol = c->buf<zd>("plot_pol", (size_t)4 * nval);
+    zd *fbe = c->buf<zd>("plot_beta", (size_t)nval + ntyp);
+    std::vector<zd> hbe((size_t)nval + ntyp);
+    for (int i = 0; i < nl; ++i) {
+      for (int k = 0; k < nval; ++k) {
+        const cplx b = beta[(size_t)(2 * i + 1) * nval + k];
+        hbe[k] = zmk(b.real(), b.imag());
+      }
+      for (int t = 0; t < ntyp; ++t) {
+        const cplx ne(a.n_eff[2 * (i * ntyp + t)], a.n_eff[2 * (i * ntyp + t) + 1]);
+        const cplx ep = ne * ne;
+        hbe[nval + t] = zmk(ep.real(), ep.imag());
+      }
+      const int core_t = ntyp > 1 && hbe[nval].x <= hbe[nval + 1].x ? 1 : 0;
+      if (a.sol) {
+        launch_fields_2d(dm, nval, Xn + (int64_t)(2 * i) * n * nval, n, ve + (int64_t)(2 * i) * nel * 14, fsol, s);
+        EMU_HIP(hipMemcpyAsync(a.sol + (size_t)i * 2 * nsol, fsol, sizeof(zd) * nsol, hipMemcpyDeviceToHost, s));
+      }
+      if (a.mode_pol) {
+        EMU_HIP(hipMemcpyAsync(fbe, hbe.data(), sizeof(zd) * hbe.size(), hipMemcpyHostToDevice, s));
+        launch_mode_pol_2d(dm, nval, Xn + (int64_t)(2 * i + 1) * n * nval, n, ve + (int64_t)(2 * i + 1) * nel * 14,
+                           fbe, fbe + nval, core_t, fpart, fpol, s);
+        EMU_HIP(hipMemcpyAsync(a.mode_pol + (size_t)i * 8 * nval, fpol, sizeof(zd) * 4 * nval,
+                               hipMemcpyDeviceToHost, s));
+      }
+      EMU_HIP(hipStreamSynchronize(s));
+    }
+  }
   // 6. orthogonality overlap L = -(1/k0) X2^T M_prime X1 diag(beta1) (orthogonal.f:95-248)
   zd *Y = c->buf<zd>("ovl_Y", (size_t)nl * n * nval);
   launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, m2, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s);
@@ -1671,6 +1708,16 @@ void calc_modes_1d(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   EMU_HIP(hipStreamSynchronize(s));
   launch_permute_cols(nb, n, nval, Xn, n * nval, so.dord, Xo, n * nval, s);
   launch_scale_cols(nb, n, nval, Xo, n, n * nval, den, nval, false, s);
+  if (a.sol) {
+    // sol_P2 of the prime modes after normalisation_1d (array_sol_P2_1d.f)
+    const size_t nsol = (size_t)9 * nval * nel;
+    zd *fsol = c->buf<zd>("plot_sol", nsol);
+    for (int i = 0; i < nl; ++i) {
+      launch_sol_p2_1d(dm, nval, Xo + (int64_t)(2 * i) * n * nval, n, a.bloch[2 * i], fsol, s);
+      EMU_HIP(hipMemcpyAsync(a.sol + (size_t)i * 2 * nsol, fsol, sizeof(zd) * nsol, hipMemcpyDeviceToHost, s));
+      EMU_HIP(hipStreamSynchronize(s));
+    }
+  }
   tm.mark("normalise_orthogonality");
   // 7. plane-wave order (pw_ordering_1d.f) and J / J^dagger
   std::vector<double> al((size_t)nb * P), kyv(nb);

@@ -61,6 +61,10 @@ struct ModesArgs {
   int64_t world_1d = 1, neq_pw_2d = 0;
   double *J2d = nullptr;           // nl x (2 P2d x nval) complex
   double *Jd2d = nullptr;          // nl x (nval x 2 P2d) complex
+  // plotting fields (host, may be null): 2D sol1 nl x (3 x 13 x nval x nel)
+  // and mode_pol nl x (4 x nval); 1D sol_P2 nl x (3 x 3 x nval x nel)
+  double *sol = nullptr;
+  double *mode_pol = nullptr;
 };
 void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a);
 DevMesh *mesh_create_1d(Ctx *c, int64_t npt, int64_t nel, const int64_t *table_nod, const int64_t *type_el,

@@ -296,11 +296,13 @@ class Simmo(Modes):
         st = self.structure
         with open(paths.msh_path + st.mesh_file) as f:
             self.n_msh_pts, self.n_msh_el = (int(v) for v in f.readline().split()[:2])
+        # the reference returns mode_pol from every call; here the field
+        # arrays are produced on request, so field-concentration plots ask
         out = EMUstack.calc_modes_2d(
             self.wl_norm(), req.num_BMs, self.max_order_PWs, 0, paths.msh_path, st.mesh_file, self.n_msh_pts,
             self.n_msh_el, st.nb_typ_el, req.n_effs, self.k_pll_norm(), req.shift, E_FIELD_FORMULATION,
-            PERIODIC_BOUNDARIES, MAX_RESTARTS, st.plotting_fields, st.plot_real, st.plot_imag, st.plot_abs,
-            req.num_pw_per_pol, 0, 0, 0)
+            PERIODIC_BOUNDARIES, MAX_RESTARTS, 1 if (st.plotting_fields == 1 or st.plot_field_conc) else 0,
+            st.plot_real, st.plot_imag, st.plot_abs, req.num_pw_per_pol, 0, 0, 0)
         (self.k_z, J, J_dag, self.sol1, self.mode_pol, self.table_nod, self.type_el, self.x_arr) = out
         # J is an overlap per unit cell area; the reference divides by the
         # area in units of period^2 (1 for square cells)

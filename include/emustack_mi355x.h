@@ -100,6 +100,20 @@ int emu_calc_modes_2d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval,
                       int64_t itermax, int64_t neq_PW, double *beta1, double *overlap_J,
                       double *overlap_J_dagger);
 
+/* emu_calc_modes_2d plus the plotting fields the reference fills when
+ * plot_modes = 1 (py_calc_modes.f:1-11 outputs sol1, mode_pol; array_sol.f:
+ * 120-339): sol1[2*(3*13*nval*nel)] complex, Fortran order (component, node,
+ * mode, element): the field of every prime mode at the 6 P2 nodes (Ex, Ey,
+ * Ez) and Ez at the 7 remaining P3 nodes, normalised as array_sol.f leaves
+ * it (largest P2-node component = 1); mode_pol[2*(4*nval)] complex (real
+ * values): fractions of |E|^2 in x, y, z and in the core material, of the
+ * adjoint modes as the reference's last array_sol call leaves them.  Either
+ * output may be NULL (not both). */
+int emu_calc_modes_2d_fields(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval, int64_t ordre_ls,
+                             int64_t nb_typ_el, const double *n_eff, const double *bloch_vec, const double *shift,
+                             int64_t E_H_field, int64_t itermax, int64_t neq_PW, double *beta1, double *overlap_J,
+                             double *overlap_J_dagger, double *sol1, double *mode_pol);
+
 /* Batched form: nbatch independent (lambda, n_eff, bloch, shift) instances on
  * one mesh, solved together (one launch sequence; prime and adjoint problems
  * of every instance share the symbolic factorisation).  Arrays are
@@ -167,6 +181,16 @@ int emu_calc_modes_1d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const 
                             int64_t neq_PW, int64_t neq_PW_2d, int64_t world_1d, const double *opts,
                             double *beta1, double *overlap_J, double *overlap_J_dagger,
                             double *overlap_J_2d, double *overlap_J_dagger_2d, int64_t *stats);
+
+/* emu_calc_modes_1d plus sol_P2 (py_calc_modes_1d.f output when plot_modes =
+ * 1; array_sol_P2_1d.f:20-60): sol_P2[2*(3*3*nval*nel)] complex, Fortran order
+ * (component, node, mode, element), the prime modes after
+ * normalisation_1d.f at the element's left, right and mid nodes. */
+int emu_calc_modes_1d_fields(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval, int64_t ordre_ls,
+                             int64_t nb_typ_el, const double *n_eff, double bloch_vec_x, double bloch_vec_y,
+                             const double *shift, int64_t itermax, int64_t neq_PW, int64_t neq_PW_2d,
+                             int64_t world_1d, double *beta1, double *overlap_J, double *overlap_J_dagger,
+                             double *overlap_J_2d, double *overlap_J_dagger_2d, double *sol_P2);
 
 /* Star-product cascade with energy fluxes for nbatch stacks of identical
  * layer structure (Stack.calc_scat, stack.py:162-427, with the interface

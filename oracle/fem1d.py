@@ -23,6 +23,7 @@ Follows (reference paths relative to the reference repository):
   eigensolve_1d    backend/fortran/valpr_lapack_1d.f:43-150 (dense LU,
                    shift-invert Arnoldi for the nval largest |mu|)
   fields_1d        backend/fortran/array_sol_1d.f:1-301
+  sol_p2_1d        backend/fortran/array_sol_P2_1d.f:13-61
   overlap_1d       backend/fortran/orthogonal_1d.f:1-252
   normalise_1d     backend/fortran/normalisation_1d.f:1-34
   pw_order_inv_1d  backend/fortran/pw_ordering_1d.f:1-80
@@ -268,6 +269,23 @@ def fields_1d(m, V, order, kx, n_k):
     return sol / den[None, None, :]
 
 
+def sol_p2_1d(sol):
+    """sol_P2 (3, 3, nval, nel) of array_sol_P2_1d.f:13-61 from the element
+    DOF values sol (nel, 11, nval): E_x at the three P2 nodes (DOFs 0-2); E_y
+    and E_z (DOFs 3-6 and 7-10, P3 nodes at the two ends, then 1/3 and 2/3)
+    at the ends, and at the mid-node the P3 interpolant, whose nodal weights
+    at x = 1/2 are (-1, -1, 9, 9) / 16."""
+    w = np.array([-1.0, -1.0, 9.0, 9.0]) / 16.0
+    nel, _, nval = sol.shape
+    out = np.zeros((3, 3, nval, nel), complex)
+    out[0] = np.transpose(sol[:, 0:3, :], (1, 2, 0))
+    for c, base in ((1, 3), (2, 7)):
+        out[c, 0] = sol[:, base].T
+        out[c, 1] = sol[:, base + 1].T
+        out[c, 2] = np.einsum('j,ejn->ne', w, sol[:, base:base + 4])
+    return out
+
+
 def overlap_1d(m, pp, ky, beta1, sol1, sol2, k0):
     """mat_overlap(i, j) = sum_e sol2_e(:, i) . M2_e (beta1_j sol1_e(:, j)) / k0
     (orthogonal_1d.f:108-189)."""
@@ -384,10 +402,13 @@ def pw_1d_to_2d(J, Jd, ordre, period_x, kx, ky, k0):
 # ----------------------------------------------------------------- driver
 
 
-def calc_modes_1d(wl_norm, nval, ordre, m, n_eff, kx, ky, shift, world_1d=True, biorth='diag'):
+def calc_modes_1d(wl_norm, nval, ordre, m, n_eff, kx, ky, shift, world_1d=True, biorth='diag',
+                  return_sol=False):
     """beta (nval,), J (2P x nval), J^dagger (nval x 2P) as returned by
     calc_modes_1d (py_calc_modes_1d.f); P = 2 ordre + 1, or the 2D count
-    when world_1d is False."""
+    when world_1d is False.  return_sol: also return sol_P2 of the prime
+    modes after normalisation_1d (py_calc_modes_1d.f:547-551) as the last
+    element of the tuple."""
     n_eff = np.asarray(n_eff, complex)
     eps = n_eff ** 2
     if np.all(eps[:-1] == eps[1:]):
@@ -431,6 +452,5 @@ def calc_modes_1d(wl_norm, nval, ordre, m, n_eff, kx, ky, shift, world_1d=True, 
         if Jdc is not None:
             Jdc = pw_1d_to_2d(J, Jdc, ordre, m.period_x, kx, ky, k0)[1]
         J = J2
-    if biorth == 'both':
-        return b1, J, Jd, Jdc
-    return b1, J, (Jdc if biorth == 'cluster' else Jd)
+    out = (b1, J, Jd, Jdc) if biorth == 'both' else (b1, J, (Jdc if biorth == 'cluster' else Jd))
+    return out + (sol_p2_1d(s1),) if return_sol else out

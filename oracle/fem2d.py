@@ -19,6 +19,7 @@ It restates, step by step, the f2py Fortran driver ``calc_modes_2d``
   eigen-solve           valpr_umf_64.f (OP = (K - shift M)^-1 M, ARPACK 'LM')
   beta branch / sort    py_calc_modes.f:664-683, z_indexx.f:27
   fields + normalise    array_sol.f:9-347
+  mode_pol              mode_energy.f:10-141 (adjoint family, py_calc_modes.f:709-714)
   orthogonality         orthogonal.f:3-337 (incl. conjugate-pair swap)
   adjoint normalise     normalisation.f:2-35
   PW ordering           pw_ordering.f:2-92
@@ -630,6 +631,35 @@ def fields_at_nodes(m, V, bloch):
     return sol, z
 
 
+def core_type(eps):
+    """n_core of py_calc_modes.f:546-551: material 1 when Re eps_1 > Re eps_2,
+    else material 2 (1-based)."""
+    eps = np.asarray(eps, complex)
+    return 1 if len(eps) < 2 or eps[0].real > eps[1].real else 2
+
+
+def mode_energy(m, sol, beta, eps):
+    """mode_pol (4, nval) of mode_energy.f:60-141: on the 16-point rule,
+    w |det| eps(type) |E_c|^2 with E the P2 interpolant of the six node values
+    of sol (3, 13, nval, nel) and E_z scaled by beta; rows x, y, z and the
+    sum over the core material, divided by the x + y + z total."""
+    eps = np.asarray(eps, complex)
+    xq, yq, wq = quad_triangle()
+    phi, _, _ = phi2(xq, yq)  # nq, 6
+    _, _, det, _ = element_geometry(m)
+    coef = np.abs(det) * eps[m.type_el - 1]  # nel
+    core = m.type_el == core_type(eps)
+    nval = sol.shape[2]
+    pol = np.zeros((4, nval), complex)
+    for q in range(len(wq)):
+        v = np.einsum('i,cine->cne', phi[q], sol[:, :6])  # 3, nval, nel
+        v[2] = v[2] * np.asarray(beta)[:, None]
+        e2 = wq[q] * coef[None, None, :] * np.abs(v) ** 2
+        pol[:3] += e2.sum(axis=2)
+        pol[3] += e2[:, :, core].sum(axis=(0, 2))
+    return pol / (pol[0] + pol[1] + pol[2])[None, :]
+
+
 def overlap_matrix(m, sol1, sol2, beta1, k0, pp):
     """orthogonal.f:95-248: overlap_L(i,j) = sum_e sol2_e(i)^T S_e sol1_e(j) beta1(j)."""
     xq, yq, wq = quad_triangle()
@@ -826,5 +856,8 @@ def calc_modes_2d(wl_norm, nval, ordre_ls, mesh, n_eff, bloch_vec, shift,
     J, Jd_raw = j_overlaps(m, sol1, sol2, ordre_ls, bloch_vec, k0)
     Jd = adjoint_normaliser(L, biorth) @ Jd_raw
     if return_debug:
-        return beta1, J, Jd, dict(res=res, L=L, sol1=sol1, sol2=sol2, k0=k0, Jd_raw=Jd_raw)
+        # mode_pol: the reference's last writer is mode_energy on the adjoint
+        # family in its own sorted order (before the pair swap)
+        pol = mode_energy(m, res[2][1], res[2][0], eps)
+        return beta1, J, Jd, dict(res=res, L=L, sol1=sol1, sol2=sol2, k0=k0, Jd_raw=Jd_raw, mode_pol=pol)
     return beta1, J, Jd
