@@ -14,15 +14,18 @@ A step solves the modes of all three layers for a batch of wavelength
 points (prime and adjoint problems, batched library calls per layer) and
 runs the star-product cascade with fluxes for every point (one batched
 cascade on the GPU).  `value` is wavelength-points per second of the
-precision form given by --precision (default: the mixed form: the Krylov
-solves read a complex64 copy of the FP64 inverted factor, and every beta is
-the FP64 two-sided Rayleigh quotient); one step of the other form on the
-same batch gives the FP64/mixed ratio and the R/T/A difference between the
-two, and both are compared with the oracle's FP64 result at the fixture
-wavelength (tests/golden/configs/config3.npz) when the batch holds it.
+precision form given by --precision (default: FP64 throughout).  One step of
+the other form on the same batch gives the mixed/FP64 ratio and the R/T/A
+difference between the two (mixed: the Krylov solves read a complex64 copy
+of the FP64 inverted factor, and every beta is the FP64 two-sided Rayleigh
+quotient), and both are compared with the oracle's FP64 result at the
+fixture wavelength (tests/golden/configs/config3.npz) when the batch holds
+it.  The default is FP64 because the mixed form misses the 1e-6 R/T/A bar
+on this workload (its eigenvectors carry the complex64 operator's
+first-order error).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
-                    [--precision mixed|fp64] [--no-cpu] [--no-compare]
+                    [--precision fp64|mixed] [--no-cpu] [--no-compare]
 """
 import argparse
 import json
@@ -91,7 +94,7 @@ def layers_for(mesh_paths):
     return ns, film, sub, sup
 
 
-def run_step(ns, film, sub, sup, wls, groups=1, ncv=0, workers=None, precision='mixed'):
+def run_step(ns, film, sub, sup, wls, groups=1, ncv=0, workers=None, precision='fp64'):
     """All three layers for the wavelength batch `wls` (`groups` concurrent
     sub-batches per layer, Krylov dimension ncv), then the cascade.  Returns
     R/T/A per point and the summed solver stats."""
@@ -252,7 +255,7 @@ def main():
     ap.add_argument('--workers', type=int, default=3,
                     help='host threads (contexts) for the layer sub-batches (0: groups + 1); each context '
                          'holds ~40 GB at 8 points per sub-batch, and 16 host cores serve the Schur steps')
-    ap.add_argument('--precision', choices=('mixed', 'fp64'), default='mixed')
+    ap.add_argument('--precision', choices=('fp64', 'mixed'), default='fp64')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline sample')
     ap.add_argument('--no-compare', action='store_true', help='skip the step of the other precision form')
     ap.add_argument('--workdir', default=os.path.join(tempfile.gettempdir(), 'emustack_amd_bench_meshes'),
@@ -355,6 +358,8 @@ def main():
                 'op_applications_per_instance_other': float(st2[0]) / (args.batch * 6.0),
                 'mixed_over_fp64': (value / v_other) if other == 'fp64' else (v_other / value),
                 'rta_max_abs_diff_mixed_vs_fp64': float(np.abs(by['mixed'] - by['fp64']).max()),
+                'rta_median_abs_diff_mixed_vs_fp64': float(np.median(np.abs(by['mixed'] - by['fp64']).max(axis=1))),
+                'points_over_1e-6': int((np.abs(by['mixed'] - by['fp64']).max(axis=1) > 1e-6).sum()),
                 'points_compared': args.batch}
         if fx is not None:
             ref = np.asarray(fx['rta_cluster'], float)

@@ -6,7 +6,10 @@ tests/golden/make_config_fixtures.py):
   config2  1D grating, 60 Bloch modes, 2 wavelengths x 2 polar angles,
            one batch;
   config3  the bench's stacked gratings (three 8,000-triangle layers, 200
-           Bloch modes) at one wavelength.
+           Bloch modes) at one wavelength;
+  case5    the reference's extraordinary-transmission case (square holes in
+           a 200 nm Au film, its own 2,528-triangle mesh, 100 Bloch modes,
+           theta = 5 deg, TM), whose own fixture file cannot be read.
 
 Tolerances (FP64 form): beta 1e-9 relative after the reference's sort; the
 mode-subspace projector J J^dagger 1e-7 of its largest entry; R/T/A 1e-8
@@ -175,3 +178,28 @@ def test_config3_against_oracle_and_batch_invariant():
     spread = np.abs(_rta(batch[i][0]) - ref).max()
     print('config3 batched vs alone: R/T/A spread %.2e' % spread, flush=True)
     assert spread <= 1e-8
+
+
+def test_case5_eot_against_oracle():
+    """tests/test_case_5_EOT_5_degrees.py.bak:44-74 through the drop-in
+    objects: a lossy metal background (Au, n = 0.25 + 6.81i) with TM light
+    at 5 degrees."""
+    from emustack_amd import objects, materials, Stack
+    fx = _load('case5')
+    C = cf.C5
+    light = objects.Light(C['wl'], max_order_PWs=C['order'], theta=C['theta'], phi=C['phi'])
+    assert abs(complex(materials.Au.n(C['wl'])) - cf.case5_au_index()) < 1e-12
+    holes = objects.NanoStruct('2D_array', C['period'], 266, height_nm=C['h'], inclusion_a=materials.Air,
+                               background=materials.Au, loss=True, square=True, make_mesh_now=False,
+                               mesh_file=os.path.join(cf.MESHES, C['mesh']))
+    sim = holes.calc_modes(light, num_BMs=C['nval'])
+    _check_beta(sim.k_z, fx['beta'][0], 'case5')
+    _check_proj(sim.J, sim.J_dag, fx['JJd_cluster'][0], 'case5')
+    sup = objects.ThinFilm(C['period'], height_nm='semi_inf', material=materials.Air, loss=False)
+    sub = objects.ThinFilm(C['period'], height_nm='semi_inf', material=materials.Air, loss=False)
+    st = Stack((sub.calc_modes(light), sim, sup.calc_modes(light)))
+    st.calc_scat(pol='TM')
+    err = np.abs(_rta(st) - fx['rta_cluster']).max()
+    print('case5: R/T/A %s, err vs oracle %.2e (per-mode normalisation of the oracle: %.2e)'
+          % (_rta(st), err, np.abs(_rta(st) - fx['rta']).max()), flush=True)
+    assert err <= 1e-8

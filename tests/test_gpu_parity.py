@@ -198,3 +198,20 @@ def test_stack_batch_equals_per_layer(gpu_ctx):
             np.testing.assert_allclose(np.asarray(sb.k_z), np.asarray(sa.k_z), rtol=1e-9)
             P1, P2 = np.asarray(sa.J) @ np.asarray(sa.J_dag), np.asarray(sb.J) @ np.asarray(sb.J_dag)
             np.testing.assert_allclose(P2, P1, rtol=0, atol=1e-8 * np.abs(P1).max())
+
+
+@pytest.mark.parametrize('block', [1, 2, 4])
+def test_small_blocks_match_oracle(case1_500, gpu_ctx, mesh600, block):
+    """Block sizes below 8 take the non-delayed block CGS2 path of the
+    Krylov-Schur expansion; the eigenpairs must not depend on it."""
+    (beta8, J8, Jd8, _), (bo, Jo, Jdo), (wln, kpn) = case1_500
+    mh, m = mesh600
+    import json
+    d = json.load(open(os.path.join(GOLDEN, 'materials.json')))
+    nsi = complex(*d['Si_c']['500.0'])
+    _, _, n_eff, _, shift = _params(500.0, 600., nsi)
+    beta, J, Jd, st = gpu_ctx.calc_modes_batch(mh, [wln], 40, 2, [n_eff], [kpn], [shift], 13, block=block)
+    print('block %d: OP applications %d, restarts %d' % (block, st[0], st[1]), flush=True)
+    np.testing.assert_allclose(beta[0], bo, rtol=1e-9, atol=1e-9 * np.abs(bo).max())
+    Po = Jo @ Jdo
+    assert np.abs(J[0] @ Jd[0] - Po).max() <= 1e-8 * np.abs(Po).max()
