@@ -26,6 +26,7 @@ first-order error).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
                     [--precision fp64|mixed] [--no-cpu] [--no-compare]
+                    [--copy-calibration]
 """
 import argparse
 import json
@@ -258,9 +259,19 @@ def main():
     ap.add_argument('--precision', choices=('fp64', 'mixed'), default='fp64')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline sample')
     ap.add_argument('--no-compare', action='store_true', help='skip the step of the other precision form')
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='only copy 2 GiB device to device (the reference kernel for scaling HBM counters)')
     ap.add_argument('--workdir', default=os.path.join(tempfile.gettempdir(), 'emustack_amd_bench_meshes'),
                     help='where the seeded bench meshes are written (regenerated when missing)')
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.complex128, device='cuda')
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes': int(x.numel() * x.element_size())}))
+        return
     if args.gpus != 1:
         print('bench.py: this build runs on one GPU; the workload is independent wavelength points and '
               'would shard by wavelength across GPUs, but only --gpus 1 is measured.',

@@ -46,7 +46,7 @@ def build(verbose=False):
             sys.stderr.write(err)
     objs = [o for o, _ in results]
     tmp = LIB + '.tmp'
-    cmd = [HIPCC, '-shared', '-fPIC', '--offload-arch=' + ARCH, '-o', tmp] + objs + ['-L/opt/rocm/lib', '-lrocblas', '-lpthread']
+    cmd = [HIPCC, '-shared', '-fPIC', '--offload-arch=' + ARCH, '-o', tmp] + objs + ['-lpthread']
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError('link failed: %s\n%s\n%s' % (' '.join(cmd), r.stdout, r.stderr))

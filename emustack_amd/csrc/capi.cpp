@@ -267,6 +267,7 @@ static int modes1d_impl(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const doub
     }
     if (opts && opts[1] != 0) a.precision = (int64_t)opts[1];
     if (a.precision != 0 && a.precision != 1) return fail(EMU_E_ARG, "precision must be 0 (FP64) or 1 (mixed)");
+    a.block = 1;  // single vectors on the small banded 1D pencils
     if (opts && opts[2] > 0) a.block = (int64_t)opts[2];
     if (opts && opts[3] > 0) a.ncv = (int64_t)opts[3];
     if (a.block != 1 && a.block != 2 && a.block != 4 && a.block != 8)

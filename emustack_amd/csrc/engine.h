@@ -59,8 +59,10 @@ struct KTime {
 // Live kernel timing (enabled by emu_profile).  A "sweep" is one solve or one
 // Gram-Schmidt pass; every `every`-th sweep is recorded.  Classes: 0 forward
 // solve (k_fwd), 1 backward solve (k_bwd), 2 Gram-Schmidt projection
-// (k_atb_tall), 3 Gram-Schmidt update (k_sub_vh_blk), 4 sparse M product
-// (k_spmm); per class: dispatches, milliseconds, algorithmic bytes.
+// (k_atb_mfma16; k_atb_tall below 8-column blocks), 3 Gram-Schmidt update
+// (k_combine; k_sub_vh_blk below 8-column blocks), 4 sparse M product of the
+// Krylov block (k_spmm_il; k_spmm below 8 columns); per class: dispatches,
+// milliseconds, algorithmic bytes.
 struct Profiler {
   bool on = false;
   int every = 8;
@@ -279,21 +281,24 @@ void launch_permute_vec(DevMesh &dm, int nb, int ncol, const zd *src, zd *dst, b
 
 // ---------------- dense / vector kernels (kernels_krylov.hip) ----------------
 // y[b][r] = sum_k A[b][r][k] x[b][col k] for CSR (n x n), ncol columns
+// M times one 8-column block X (column-major, ld n) through the
+// row-interleaved scratch xi (nb x 8 n): y column-major (ld n)
+void launch_spmm8_il(int nb, int64_t n, const int64_t *rp, const int32_t *ci, const zd *vals, int64_t vstride,
+                     const zd *x, int64_t xstride, zd *xi, zd *y, int64_t ystride, hipStream_t s, KTime kt = {});
+// xi (optional, nb x n x ncol): scratch for the row-major copy that the
+// many-column form (ncol > 8) works from
 void launch_spmm(int nb, int64_t n, int ncol, const int64_t *rp, const int32_t *ci, const zd *vals,
                  int64_t vstride, const zd *x, int64_t xstride, zd *y, int64_t ystride,
-                 hipStream_t s, KTime kt = {});
+                 hipStream_t s, KTime kt = {}, zd *xi = nullptr);
 // C[b] (M x N, ldc) = op(A[b])^T B[b] with K rows; op = conj if conjA, split
 // over K (partial products per K chunk in scratch, then a reduction).  N in
 // {1, 2, 4, 8, 16}: the tall-skinny kernel (the one kt times); 8 x 8 Gram
-// matrices of a block: a dedicated kernel; other N: 32 x 32 output tiles.
+// matrices of a block: a dedicated kernel; other N: 64 x 64 MFMA output tiles.
 void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64_t sA,
                 const zd *B, int64_t ldb, int64_t sB, zd *C, int64_t ldc, int64_t sC,
                 bool conjA, zd *scratch, size_t scratch_elems, hipStream_t s, KTime kt = {});
-// C[b] = alpha op(A[b]) op(B[b]) + beta C[b], op 'N', 'T' or 'C', column-major (rocBLAS)
-void launch_gemm(int nb, char opA, char opB, int64_t M, int64_t N, int64_t K, zd alpha, const zd *A, int64_t lda,
-                 int64_t sA, const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,
-                 hipStream_t s);
 // C[b] (M x N) = alpha * A[b] (M x K) * B[b] (K x N) + beta*C[b], column-major
+// (k_zgemm_nn: 64 x 64 tiles on the FP64 matrix cores); C must not alias A or B
 void launch_gemm_nn(int nb, int64_t M, int N, int K, zd alpha, const zd *A, int64_t lda, int64_t sA,
                     const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,
                     hipStream_t s);

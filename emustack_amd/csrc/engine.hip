@@ -746,6 +746,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   zd *t1 = c->buf<zd>("ks_t1", (size_t)nb * bq * n);
   zd *ty = c->buf<zd>("ks_ty", (size_t)nb * bq * n);
   zd *wv = c->buf<zd>("ks_wv", (size_t)nb * bq * n);
+  zd *xil = bq == 8 ? c->buf<zd>("ks_xil", (size_t)nb * 8 * n) : nullptr;  // interleaved SpMM input
   zd *tw = c->buf<zd>("ks_w", (size_t)nb * bq * solve_ws(dm));
   zd *Hd = c->buf<zd>("ks_H", (size_t)nb * sH);
   zd *H2 = c->buf<zd>("ks_H2", (size_t)nb * ld * bq);
@@ -828,8 +829,12 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
       zd *Wn = V + (int64_t)(j + bq) * n;  // next block slot
       if (pf) pf->start_sweep();
       const KTime kspmv = pf ? pf->time(4, nbd * (20.0 * nzd + 8.0 * (nd + 1) + 32.0 * nd * bq)) : KTime{};
-      launch_spmm(nact, n, bq, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, t1, (int64_t)bq * n, s,
-                  kspmv);
+      if (bq == 8)
+        launch_spmm8_il(nact, n, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, xil, t1, (int64_t)bq * n,
+                        s, kspmv);
+      else
+        launch_spmm(nact, n, bq, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, t1, (int64_t)bq * n, s,
+                    kspmv);
       if (inv32)
         lu_solve(dm, nact, bq, inv32, gx, t1, ty, wv, tw, s, pf);
       else
@@ -1359,7 +1364,7 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   const Symbolic &sy = mh.sym;
   // 5. field normalisation (array_sol.f:297-339)
   zd *zmax = c->buf<zd>("zmax", (size_t)nb * nval);
-  size_t nparts = (size_t)((nel + 255) / 256);
+  size_t nparts = (size_t)((nel + 63) / 64);  // k_field_max_partial's 64-element chunks
   double *fms = c->buf<double>("fmax_scratch", (size_t)nb * nval * nparts * 4);
   launch_field_max(dm, nb, nval, Xn, n, n * nval, ve, false, zmax, fms, s);
   launch_scale_cols(nb, n, nval, Xn, n, n * nval, zmax, nval, true, s);
@@ -1402,7 +1407,8 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   }
   // 6. orthogonality overlap L = -(1/k0) X2^T M_prime X1 diag(beta1) (orthogonal.f:95-248)
   zd *Y = c->buf<zd>("ovl_Y", (size_t)nl * n * nval);
-  launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, m2, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s);
+  zd *Xrm = c->buf<zd>("spmm_xi", (size_t)nl * n * nval);  // row-major copy for the many-column SpMM
+  launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, m2, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s, {}, Xrm);
   zd *Ld = c->buf<zd>("ovl_L", (size_t)nl * nval * nval);
   size_t atb_elems = (size_t)nl * nval * nval * 64 + (size_t)nb * 2 * 113 * nval * 64;
   zd *atbs = c->buf<zd>("post_atb", atb_elems);
@@ -1414,7 +1420,7 @@ void calc_modes(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   // for the two-sided Rayleigh quotients below
   std::vector<cplx> LAh;
   {
-    launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, m1, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s);
+    launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, m1, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s, {}, Xrm);
     zd *LAd = c->buf<zd>("ovl_LA", (size_t)nl * nval * nval);
     launch_atb(nl, n, nval, nval, Xn + n * nval, n, 2 * n * nval, Y, n, n * nval, LAd, nval, (int64_t)nval * nval,
                false, atbs, atb_elems, s);
@@ -1633,7 +1639,8 @@ void calc_modes_1d(Ctx *c, DevMesh &dm, const ModesArgs &a) {
   launch_scale_cols(nb, n, nval, Xn, n, n * nval, den, nval, true, s);
   // 6. overlap L = X2^T Mov X1 diag(beta1) / k0 of the prime instances
   zd *Y = c->buf<zd>("ovl_Y", (size_t)nl * n * nval);
-  launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, mov, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s);
+  zd *Xrm = c->buf<zd>("spmm_xi", (size_t)nl * n * nval);  // row-major copy for the many-column SpMM
+  launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, mov, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s, {}, Xrm);
   zd *Ld = c->buf<zd>("ovl_L", (size_t)nl * nval * nval);
   size_t atb_elems = (size_t)nl * nval * nval * 64 + (size_t)nb * 2 * P * nval * 64;
   zd *atbs = c->buf<zd>("post_atb", atb_elems);
@@ -1648,7 +1655,8 @@ void calc_modes_1d(Ctx *c, DevMesh &dm, const ModesArgs &a) {
     zd *LAd = c->buf<zd>("ovl_LA", (size_t)nl * nval * nval);
     zd *LBd = c->buf<zd>("ovl_LB", (size_t)nl * nval * nval);
     for (int w = 0; w < 2; ++w) {
-      launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, w ? m1 : m2, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s);
+      launch_spmm(nl, n, nval, dm.row_ptr.p, dm.col_ind.p, w ? m1 : m2, 2 * nnz, Xn, 2 * n * nval, Y, n * nval, s,
+                  {}, Xrm);
       launch_atb(nl, n, nval, nval, Xn + n * nval, n, 2 * n * nval, Y, n, n * nval, w ? LAd : LBd, nval,
                  (int64_t)nval * nval, false, atbs, atb_elems, s);
     }

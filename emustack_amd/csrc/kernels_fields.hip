@@ -48,8 +48,8 @@ __device__ __forceinline__ void node_fields(int64_t e, const int32_t *__restrict
 __global__ __launch_bounds__(256) void k_fields_2d(int64_t nel, int nval, const int32_t *__restrict__ ldof,
                                                    const double *__restrict__ gt, const zd *__restrict__ X,
                                                    int64_t ldx, const zd *__restrict__ ve, zd *__restrict__ sol) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
+  const int64_t e = blockIdx.y * (int64_t)blockDim.x + threadIdx.x;
+  const int k = blockIdx.x;  // modes fastest: the element range's data stays in L2
   if (e >= nel) return;
   zd f[6][3], co[22];
   node_fields(e, ldof, gt, X + (int64_t)k * ldx, ve, f, co);
@@ -71,8 +71,8 @@ __global__ __launch_bounds__(256) void k_energy_2d(int64_t nel, int nval, const 
                                                    const zd *__restrict__ X, int64_t ldx, const zd *__restrict__ ve,
                                                    const zd *__restrict__ beta, const zd *__restrict__ eps,
                                                    zd *__restrict__ part) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
+  const int64_t e = blockIdx.y * (int64_t)blockDim.x + threadIdx.x;
+  const int k = blockIdx.x;  // modes fastest: the element range's data stays in L2
   if (e >= nel) return;
   zd f[6][3], co[22];
   node_fields(e, ldof, gt, X + (int64_t)k * ldx, ve, f, co);
@@ -143,7 +143,7 @@ static void fields_tables() {
 void launch_fields_2d(DevMesh &dm, int nval, const zd *X, int64_t ldx, const zd *ve, zd *sol, hipStream_t s) {
   fields_tables();
   const int64_t nel = dm.host.nel;
-  hipLaunchKernelGGL(k_fields_2d, dim3((unsigned)((nel + 255) / 256), nval), dim3(256), 0, s, nel, nval, dm.ldof.p,
+  hipLaunchKernelGGL(k_fields_2d, dim3(nval, (unsigned)((nel + 255) / 256)), dim3(256), 0, s, nel, nval, dm.ldof.p,
                      dm.gt.p, X, ldx, ve, sol);
   EMU_LAUNCH_CHECK();
 }
@@ -152,7 +152,7 @@ void launch_mode_pol_2d(DevMesh &dm, int nval, const zd *X, int64_t ldx, const z
                         const zd *eps, int core_t, zd *part, zd *pol, hipStream_t s) {
   fields_tables();
   const int64_t nel = dm.host.nel;
-  hipLaunchKernelGGL(k_energy_2d, dim3((unsigned)((nel + 255) / 256), nval), dim3(256), 0, s, nel, nval, dm.ldof.p,
+  hipLaunchKernelGGL(k_energy_2d, dim3(nval, (unsigned)((nel + 255) / 256)), dim3(256), 0, s, nel, nval, dm.ldof.p,
                      dm.gt.p, dm.det.p, dm.type_el.p, core_t, X, ldx, ve, beta, eps, part);
   EMU_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_mode_pol, dim3(nval), dim3(256), 0, s, nel, part, pol);
@@ -165,8 +165,8 @@ void launch_mode_pol_2d(DevMesh &dm, int nval, const zd *X, int64_t ldx, const z
 __global__ __launch_bounds__(256) void k_sol_p2_1d(int64_t nel, int nval, const int32_t *__restrict__ ldof,
                                                    const zd *__restrict__ X, int64_t ldx, double kx, double dxw,
                                                    zd *__restrict__ out) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
+  const int64_t e = blockIdx.y * (int64_t)blockDim.x + threadIdx.x;
+  const int k = blockIdx.x;  // modes fastest: the element range's data stays in L2
   if (e >= nel) return;
   const zd *x = X + (int64_t)k * ldx;
   zd v[11];
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_sol_p2_1d(int64_t nel, int nval, const 
 
 void launch_sol_p2_1d(DevMesh &dm, int nval, const zd *X, int64_t ldx, double kx, zd *out, hipStream_t s) {
   const int64_t nel = dm.host.nel;
-  hipLaunchKernelGGL(k_sol_p2_1d, dim3((unsigned)((nel + 255) / 256), nval), dim3(256), 0, s, nel, nval, dm.ldof.p, X,
+  hipLaunchKernelGGL(k_sol_p2_1d, dim3(nval, (unsigned)((nel + 255) / 256)), dim3(256), 0, s, nel, nval, dm.ldof.p, X,
                      ldx, kx, dm.host.period_x, out);
   EMU_LAUNCH_CHECK();
 }

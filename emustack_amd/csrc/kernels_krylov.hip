@@ -1,7 +1,5 @@
 // Dense / vector kernels of the Krylov-Schur eigensolver and of the overlap
 // contractions, batched over instances (blockIdx.y / blockIdx.z = instance).
-#include <rocblas/rocblas.h>
-
 #include <map>
 #include <mutex>
 #include <string>
@@ -81,8 +79,104 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, int ncol, const int64_t
   if (r < n && lane8 < nc) y[b * ystride + (int64_t)(c0 + lane8) * n + r] = out;
 }
 
+// ------------------------------------------ SpMM with many columns (modes)
+// Post-processing multiplies M (or K - shift M) by all nval mode vectors at
+// once.  The mode block is first transposed to row-major (xi[col][c], one
+// pass through LDS tiles), so that a nonzero's nval values are one
+// contiguous run; a wavefront then owns a row and its lanes the columns
+// (lane, lane + 64, ...), reads the row's values and column indices once
+// (wave-uniform) and streams the column runs coalesced, four nonzeros at a
+// time.  A workgroup's 8 rows are gathered in LDS and written back column by
+// column (128-byte runs).  Column chunks of 256 (grid z) bound the LDS tile.
+constexpr int SR_ROWS = 8, SR_COLS = 256;
+
+__global__ __launch_bounds__(256) void k_transpose_rm(int64_t n, int ncol, const zd *__restrict__ X, int64_t ldx,
+                                                      int64_t sX, zd *__restrict__ xi, int64_t sI) {
+  __shared__ zd T[32][33];
+  const int b = blockIdx.z;
+  const int64_t r0 = (int64_t)blockIdx.x * 32;
+  const int c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const zd *x = X + b * sX;
+  for (int k = ty; k < 32; k += 8) {
+    const int c = c0 + k;
+    const int64_t r = r0 + tx;
+    if (c < ncol && r < n) T[k][tx] = x[r + (int64_t)c * ldx];
+  }
+  __syncthreads();
+  zd *o = xi + b * sI;
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t r = r0 + k;
+    const int c = c0 + tx;
+    if (c < ncol && r < n) o[r * ncol + c] = T[tx][k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_spmm_rows(int64_t n, int ncol, const int64_t *__restrict__ rp,
+                                                   const int32_t *__restrict__ ci, const zd *__restrict__ vals,
+                                                   int64_t vstride, const zd *__restrict__ xi, int64_t sI,
+                                                   zd *__restrict__ y, int64_t ldy, int64_t sY) {
+  __shared__ zd Ys[SR_ROWS][SR_COLS + 1];
+  const int b = blockIdx.y;
+  const int c0 = blockIdx.z * SR_COLS;
+  const int nc = min(SR_COLS, ncol - c0);
+  const int64_t rbase = (int64_t)blockIdx.x * SR_ROWS;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const zd *v = vals + b * vstride;
+  const zd *xb = xi + b * sI + c0;
+  for (int i = 0; i < SR_ROWS / 4; ++i) {
+    const int rr = (SR_ROWS / 4) * w + i;
+    const int64_t r = rbase + rr;
+    zd acc[SR_COLS / 64];
+#pragma unroll
+    for (int q = 0; q < SR_COLS / 64; ++q) acc[q] = zmk(0, 0);
+    if (r < n) {
+      const int64_t k0 = rp[r], k1 = rp[r + 1];
+      // four nonzeros per step: their column runs are all requested before
+      // the first is used
+      for (int64_t kb = k0; kb < k1; kb += 4) {
+        zd a[4], xv[4][SR_COLS / 64];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t k = min(kb + u, k1 - 1);
+          a[u] = kb + u < k1 ? v[k] : zmk(0, 0);
+          const zd *xr = xb + (int64_t)ci[k] * ncol;
+#pragma unroll
+          for (int q = 0; q < SR_COLS / 64; ++q) xv[u][q] = xr[min(lane + 64 * q, nc - 1)];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int q = 0; q < SR_COLS / 64; ++q) acc[q] = zfma(a[u], xv[u][q], acc[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SR_COLS / 64; ++q) Ys[rr][lane + 64 * q] = acc[q];
+  }
+  __syncthreads();
+  zd *yb = y + b * sY + (int64_t)c0 * ldy;
+  for (int idx = tid; idx < SR_ROWS * nc; idx += 256) {
+    const int c = idx / SR_ROWS, rr = idx % SR_ROWS;
+    const int64_t r = rbase + rr;
+    if (r < n) yb[(int64_t)c * ldy + r] = Ys[rr][c];
+  }
+}
+
 void launch_spmm(int nb, int64_t n, int ncol, const int64_t *rp, const int32_t *ci, const zd *vals,
-                 int64_t vstride, const zd *x, int64_t xstride, zd *y, int64_t ystride, hipStream_t s, KTime kt) {
+                 int64_t vstride, const zd *x, int64_t xstride, zd *y, int64_t ystride, hipStream_t s, KTime kt,
+                 zd *xi) {
+  if (xi && ncol > 8) {  // many columns: row-major copy, then one wavefront per row
+    dim3 gt((unsigned)((n + 31) / 32), (unsigned)((ncol + 31) / 32), nb);
+    hipLaunchKernelGGL(k_transpose_rm, gt, dim3(256), 0, s, n, ncol, x, n, xstride, xi, n * ncol);
+    EMU_LAUNCH_CHECK();
+    dim3 g((unsigned)((n + SR_ROWS - 1) / SR_ROWS), nb, (unsigned)((ncol + SR_COLS - 1) / SR_COLS));
+    hipExtLaunchKernelGGL(k_spmm_rows, g, dim3(256), 0, s, kt.a, kt.b, 0, n, ncol, rp, ci, vals, vstride, xi,
+                          n * ncol, y, n, ystride);
+    EMU_LAUNCH_CHECK();
+    return;
+  }
   // 8 * per launch indices cover the (n + 31) / 32 row blocks; rows >= n idle
   const int per = (int)(((n + 31) / 32 + 7) / 8);
   dim3 g((unsigned)(8 * per), nb, (unsigned)((ncol + SM_C - 1) / SM_C));
@@ -92,58 +186,161 @@ void launch_spmm(int nb, int64_t n, int ncol, const int64_t *rp, const int32_t *
 }
 
 
-// ----------------------------------------------- C = op(A)^T B, split over K
-constexpr int AT = 32;     // output tile
-constexpr int AKC = 32;    // K sub-chunk staged in LDS
+// ------------------------------------------- SpMM on a row-interleaved block
+// The Krylov loop multiplies M by one 8-column block per step.  Gathering
+// x[col] of eight columns stored n apart touches eight cache lines per
+// nonzero; with the block stored row-interleaved (xi[8 col + c]) the eight
+// values of a nonzero's column are one 128-byte line, read whole.  The block
+// is interleaved by one pass (k_interleave8, 2 x 16 n bytes per column
+// block) before the product.
+__global__ __launch_bounds__(256) void k_interleave8(int64_t n, const zd *__restrict__ X, int64_t sX,
+                                                     zd *__restrict__ xi, int64_t sI) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (r >= n) return;
+  const zd *x = X + b * sX;
+  zd v[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) v[c] = ld_stream(x + (int64_t)c * n + r);
+  zd *o = xi + b * sI + 8 * r;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) o[c] = v[c];
+}
 
-__global__ __launch_bounds__(256) void k_atb_partial(int64_t K, int M, int N, int64_t kchunk,
-                                                     const zd *__restrict__ A, int64_t lda, int64_t sA,
-                                                     const zd *__restrict__ B, int64_t ldb, int64_t sB,
-                                                     int conjA, zd *__restrict__ part, int tilesM) {
-  const int tile = blockIdx.x;
-  const int tm = tile % tilesM, tn = tile / tilesM;
-  const int chunk = blockIdx.y;
+__global__ __launch_bounds__(256) void k_spmm_il(int64_t n, const int64_t *__restrict__ rp,
+                                                 const int32_t *__restrict__ ci, const zd *__restrict__ vals,
+                                                 int64_t vstride, const zd *__restrict__ xi, int64_t xstride,
+                                                 zd *__restrict__ y, int64_t ystride, int per) {
+  // Eight lanes share a row and lane c owns output column c.  Per step the
+  // group takes the row's next eight nonzeros (lane j loads value and column
+  // of nonzero j, coalesced), then for each of them lane c loads x[col][c]:
+  // the group reads one 128-byte line per nonzero, eight lines per wavefront
+  // load, and accumulates without a final cross-lane reduction.
+  const int lane8 = threadIdx.x & 7;
+  const int gbase = threadIdx.x & 63 & ~7;  // first lane of the group in the wavefront
+  const int64_t blk = (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int64_t r = blk * 32 + (threadIdx.x >> 3);
+  const int b = blockIdx.y;
+  const zd *v = vals + b * vstride;
+  const zd *xb = xi + b * xstride;
+  zd acc = zmk(0, 0);
+  const int64_t k0 = r < n ? rp[r] : 0, k1 = r < n ? rp[r + 1] : 0;
+  for (int64_t kb = k0; kb < k1; kb += 8) {
+    const int64_t k = kb + lane8;
+    const bool ok = k < k1;
+    const int64_t kc = ok ? k : k1 - 1;
+    zd a = ld_stream(v + kc);
+    const int col = ld_stream(ci + kc);
+    if (!ok) a = zmk(0, 0);
+    const int cnt = (int)min((int64_t)8, k1 - kb);
+    zd xv[8], av[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int cq = __shfl(col, gbase + q, 64);
+      av[q] = zmk(__shfl(a.x, gbase + q, 64), __shfl(a.y, gbase + q, 64));
+      xv[q] = xb[8 * (int64_t)cq + lane8];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < cnt) acc = zfma(av[q], xv[q], acc);
+  }
+  if (r < n) y[b * ystride + (int64_t)lane8 * n + r] = acc;
+}
+
+void launch_spmm8_il(int nb, int64_t n, const int64_t *rp, const int32_t *ci, const zd *vals, int64_t vstride,
+                     const zd *x, int64_t xstride, zd *xi, zd *y, int64_t ystride, hipStream_t s, KTime kt) {
+  hipLaunchKernelGGL(k_interleave8, dim3((unsigned)((n + 255) / 256), nb), dim3(256), 0, s, n, x, xstride, xi,
+                     8 * n);
+  EMU_LAUNCH_CHECK();
+  const int per = (int)(((n + 31) / 32 + 7) / 8);
+  hipExtLaunchKernelGGL(k_spmm_il, dim3((unsigned)(8 * per), nb), dim3(256), 0, s, kt.a, kt.b, 0, n, rp, ci, vals,
+                        vstride, xi, 8 * n, y, ystride, per);
+  EMU_LAUNCH_CHECK();
+}
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// -------------------------------- C = op(A)^T B, wide, on the matrix cores
+// Mode overlaps and plane-wave contractions: M, N of tens to hundreds, K the
+// DOF count.  A workgroup owns a 64 x 64 output tile and one K chunk; per
+// 32-row step it stages the 32 x 64 tiles of A and B in LDS (the next pair is
+// fetched into registers meanwhile), and wave w accumulates output columns
+// 16 w .. 16 w + 15 against all 64 rows with FP64 16x16x4 MFMAs (four real
+// products per complex one).  Partials per chunk: part[b][chunk][n][m].
+template <bool CONJ>
+__global__ __launch_bounds__(256) void k_atb_mfma_wide(int64_t K, int M, int N, int64_t kchunk,
+                                                       const zd *__restrict__ A, int64_t lda, int64_t sA,
+                                                       const zd *__restrict__ B, int64_t ldb, int64_t sB,
+                                                       zd *__restrict__ part, int tilesM) {
+  constexpr int TT = 64, TK = 32;
+  const int tm = blockIdx.x % tilesM, tn = blockIdx.x / tilesM;
+  const int m0 = tm * TT, n0 = tn * TT;
+  const int chunk = blockIdx.y, nchunk = gridDim.y;
   const int b = blockIdx.z;
-  const int nchunk = gridDim.y;
-  const int tid = threadIdx.x;
-  const int m0 = tm * AT, n0 = tn * AT;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const zd *Ab = A + b * sA;
   const zd *Bb = B + b * sB;
-  __shared__ zd As[AKC][AT + 1];
-  __shared__ zd Bs[AKC][AT + 1];
-  const int tx = tid % 16, ty = tid / 16;  // outputs (ty + 16 i, tx + 16 j), i, j in {0,1}
-  zd acc[2][2] = {{zmk(0, 0), zmk(0, 0)}, {zmk(0, 0), zmk(0, 0)}};
+  __shared__ zd As[TK][TT + 1];
+  __shared__ zd Bs[TK][TT + 1];
   const int64_t kb = chunk * kchunk, ke = min(K, kb + kchunk);
-  for (int64_t k0 = kb; k0 < ke; k0 += AKC) {
-    // unconditional loads from clamped addresses (a load inside a branch
-    // is waited for before the branch merges)
+  const int lr = tid & 31, lc = tid >> 5;  // staging: row lr, columns lc + 8 i
+  zd ra[8], rb[8];
+  auto fetch = [&](int64_t k0) {
+    const int64_t k = k0 + lr;
+    const int64_t kc = k < ke ? k : ke - 1;
 #pragma unroll
-    for (int i = 0; i < AKC * AT / 256; ++i) {
-      const int idx = tid + 256 * i, kk = idx % AKC, mm = idx / AKC;
-      const int64_t k = k0 + kk;
-      const int64_t kc = k < ke ? k : ke - 1;
-      const zd a = Ab[kc + (int64_t)min(m0 + mm, M - 1) * lda];
-      const zd bb = Bb[kc + (int64_t)min(n0 + mm, N - 1) * ldb];
-      As[kk][mm] = (k < ke && m0 + mm < M) ? (conjA ? zconj(a) : a) : zmk(0, 0);
-      Bs[kk][mm] = (k < ke && n0 + mm < N) ? bb : zmk(0, 0);
-    }
-    __syncthreads();
-#pragma unroll 8
-    for (int kk = 0; kk < AKC; ++kk) {
-      zd a0 = As[kk][ty], a1 = As[kk][ty + 16];
-      zd b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
-      acc[0][0] = zfma(a0, b0, acc[0][0]);
-      acc[0][1] = zfma(a0, b1, acc[0][1]);
-      acc[1][0] = zfma(a1, b0, acc[1][0]);
-      acc[1][1] = zfma(a1, b1, acc[1][1]);
-    }
-    __syncthreads();
+    for (int i = 0; i < 8; ++i) ra[i] = Ab[kc + (int64_t)min(m0 + lc + 8 * i, M - 1) * lda];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) rb[i] = Bb[kc + (int64_t)min(n0 + lc + 8 * i, N - 1) * ldb];
+  };
+  v4d pr[4], pi[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    pr[t] = (v4d){0, 0, 0, 0};
+    pi[t] = (v4d){0, 0, 0, 0};
   }
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) {
-      int m = m0 + ty + 16 * i, nn = n0 + tx + 16 * j;
-      if (m < M && nn < N) part[(((int64_t)b * nchunk + chunk) * N + nn) * M + m] = acc[i][j];
+  if (kb < ke) fetch(kb);
+  const int fi = lane & 15, fk = lane >> 4;
+  for (int64_t k0 = kb; k0 < ke; k0 += TK) {
+    __syncthreads();
+    const bool okk = k0 + lr < ke;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) As[lr][lc + 8 * i] = (okk && m0 + lc + 8 * i < M) ? ra[i] : zmk(0, 0);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) Bs[lr][lc + 8 * i] = (okk && n0 + lc + 8 * i < N) ? rb[i] : zmk(0, 0);
+    __syncthreads();
+    if (k0 + TK < ke) fetch(k0 + TK);
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ++ks) {
+      const int kk = ks * 4 + fk;
+      const zd xb = Bs[kk][16 * wv + fi];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const zd va = As[kk][16 * t + fi];
+        pr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va.x, xb.x, pr[t], 0, 0, 0);
+        pi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va.x, xb.y, pi[t], 0, 0, 0);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const zd va = As[kk][16 * t + fi];
+        // CONJ: Re += ar br + ai bi, Im += ar bi - ai br
+        pr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(CONJ ? va.y : -va.y, xb.y, pr[t], 0, 0, 0);
+        pi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(CONJ ? -va.y : va.y, xb.x, pi[t], 0, 0, 0);
+      }
     }
+  }
+  const int nn = n0 + 16 * wv + fi;
+  if (nn < N) {
+    zd *pp = part + (((int64_t)b * nchunk + chunk) * N + nn) * M;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + 16 * t + fk + 4 * r;
+        if (m < M) pp[m] = zmk(pr[t][r], pi[t][r]);
+      }
+  }
 }
 
 // ------------------------------------------ C = op(A)^T B for N <= 8 columns
@@ -293,7 +490,6 @@ void launch_swap_i32(int64_t n, int32_t *a, int32_t *b, hipStream_t s) {
 // by k_atb_reduce).  Fragment layouts (FP64 16x16x4): A[i][k] in lane
 // i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in lane j + 16 (i % 4),
 // register i / 4.
-typedef double v4d __attribute__((ext_vector_type(4)));
 
 template <bool CONJ>
 __global__ __launch_bounds__(256) void k_atb_mfma16(int64_t K, int M, int64_t kchunk, const zd *__restrict__ A,
@@ -436,19 +632,24 @@ void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64
                 size_t scratch_elems, hipStream_t s, KTime kt) {
   if (M <= 0 || N <= 0 || nb <= 0) return;
   const bool tall = N == 1 || N == 2 || N == 4 || N == 8 || N == 16;
-  if (!tall) {  // wide products (mode overlaps, J contractions): 32 x 32 output tiles, split over K
-    const int tiles = ((M + AT - 1) / AT) * ((N + AT - 1) / AT);
+  if (!tall) {  // wide products (mode overlaps, J contractions): 64 x 64 MFMA tiles, split over K
+    const int tilesM = (M + 63) / 64, tiles = tilesM * ((N + 63) / 64);
     const int64_t per = (int64_t)nb * M * N;
     const int64_t maxchunk = std::max<int64_t>(1, (int64_t)(scratch_elems / std::max<int64_t>(per, 1)));
     const int64_t want = std::max<int64_t>(1, 2048 / std::max(1, tiles * nb));
-    int64_t nchunk = std::min<int64_t>(std::min<int64_t>(want, maxchunk), (K + AKC - 1) / AKC);
+    int64_t nchunk = std::min<int64_t>(std::min<int64_t>(want, maxchunk), (K + 31) / 32);
     nchunk = std::max<int64_t>(1, std::min<int64_t>(nchunk, 65535));
     int64_t kchunk = (K + nchunk - 1) / nchunk;
-    kchunk = ((kchunk + AKC - 1) / AKC) * AKC;
+    kchunk = ((kchunk + 31) / 32) * 32;
     nchunk = (K + kchunk - 1) / kchunk;
     if ((size_t)(nchunk * per) > scratch_elems) throw HipError("atb: scratch too small");
-    hipLaunchKernelGGL(k_atb_partial, dim3(tiles, (unsigned)nchunk, nb), dim3(256), 0, s, K, M, N, kchunk, A, lda, sA,
-                       B, ldb, sB, conjA ? 1 : 0, scratch, (M + AT - 1) / AT);
+    dim3 g((unsigned)tiles, (unsigned)nchunk, nb);
+    if (conjA)
+      hipLaunchKernelGGL(k_atb_mfma_wide<true>, g, dim3(256), 0, s, K, M, N, kchunk, A, lda, sA, B, ldb, sB, scratch,
+                         tilesM);
+    else
+      hipLaunchKernelGGL(k_atb_mfma_wide<false>, g, dim3(256), 0, s, K, M, N, kchunk, A, lda, sA, B, ldb, sB,
+                         scratch, tilesM);
     EMU_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_atb_reduce, dim3((unsigned)((per / nb + 63) / 64), nb), dim3(256), 0, s, M, N, (int)nchunk,
                        scratch, C, ldc, sC);
@@ -521,45 +722,114 @@ void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64
 
 
 // ------------------------------------------------ C = alpha A B + beta C (NN)
-// Plain batched GEMMs (restart combination V Q, Ritz vectors, star products)
-// go to rocBLAS' strided-batched zgemm: one handle per stream, created on
-// first use and kept for the process, atomics off so that results do not
-// depend on the run.
-static rocblas_handle blas_handle(hipStream_t s) {
-  static std::mutex mu;
-  static std::map<hipStream_t, rocblas_handle> handles;
-  std::lock_guard<std::mutex> g(mu);
-  auto it = handles.find(s);
-  if (it != handles.end()) return it->second;
-  rocblas_handle h = nullptr;
-  if (rocblas_create_handle(&h) != rocblas_status_success) throw HipError("rocblas_create_handle failed");
-  if (rocblas_set_stream(h, s) != rocblas_status_success ||
-      rocblas_set_atomics_mode(h, rocblas_atomics_not_allowed) != rocblas_status_success)
-    throw HipError("rocblas handle set-up failed");
-  handles[s] = h;
-  return h;
-}
-
-void launch_gemm(int nb, char opA, char opB, int64_t M, int64_t N, int64_t K, zd alpha, const zd *A, int64_t lda,
-                 int64_t sA, const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,
-                 hipStream_t s) {
-  if (M <= 0 || N <= 0 || nb <= 0) return;
-  auto op = [](char c) {
-    return c == 'C' ? rocblas_operation_conjugate_transpose
-                    : (c == 'T' ? rocblas_operation_transpose : rocblas_operation_none);
+// Batched complex GEMM on the FP64 matrix cores, for the restart combination
+// V Q, the Ritz vectors V W, H S in the delayed Gram-Schmidt and the
+// star-product cascade.  A workgroup owns a 64 x 64 tile of C; per 16-deep K
+// step it stages the 64 x 16 tile of A (loaded along the columns of A,
+// coalesced) and the 16 x 64 tile of B in LDS (33 KB, four workgroups per
+// CU), and wave w accumulates the 64 x 16 slice of columns 16 w .. 16 w + 15
+// with 16x16x4 FP64 MFMAs (four real products per complex one, the two
+// products of each accumulator issued apart).  The next pair of tiles is fetched into
+// registers while the current one is multiplied.  Launch order runs the N
+// tiles of one M tile next to each other, so A's rows are shared in L2.
+template <bool BETA>
+__global__ __launch_bounds__(256) void k_zgemm_nn(int64_t M, int N, int K, zd alpha, const zd *__restrict__ A,
+                                                  int64_t lda, int64_t sA, const zd *__restrict__ B, int64_t ldb,
+                                                  int64_t sB, zd beta, zd *__restrict__ C, int64_t ldc, int64_t sC,
+                                                  int tilesN) {
+  constexpr int TT = 64, TK = 16;
+  const int64_t tm = blockIdx.x / tilesN;
+  const int tn = blockIdx.x % tilesN;
+  const int64_t m0 = tm * TT;
+  const int n0 = tn * TT;
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const zd *Ab = A + b * sA;
+  const zd *Bb = B + b * sB;
+  __shared__ zd As[TK][TT + 1];
+  __shared__ zd Bs[TK][TT + 1];
+  // A tile: row am = tid & 63 of the tile, K columns ak + 4 i; B tile: K row
+  // bk = tid & 15, columns bn + 16 i
+  const int am = tid & 63, ak = tid >> 6;
+  const int bk = tid & 15, bn = tid >> 4;
+  const int64_t arow = min(m0 + am, M - 1);
+  zd ra[4], rb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ra[i] = Ab[arow + (int64_t)min(k0 + ak + 4 * i, K - 1) * lda];
+    const int kb = min(k0 + bk, K - 1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) rb[i] = Bb[kb + (int64_t)min(n0 + bn + 16 * i, N - 1) * ldb];
   };
-  using rz = rocblas_double_complex;
-  const rocblas_status st = rocblas_zgemm_strided_batched_64(
-      blas_handle(s), op(opA), op(opB), M, N, K, reinterpret_cast<const rz *>(&alpha),
-      reinterpret_cast<const rz *>(A), lda, sA, reinterpret_cast<const rz *>(B), ldb, sB,
-      reinterpret_cast<const rz *>(&beta), reinterpret_cast<rz *>(C), ldc, sC, nb);
-  if (st != rocblas_status_success) throw HipError("rocblas_zgemm_strided_batched failed (status " + std::to_string((int)st) + ")");
+  v4d pr[4], pi[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    pr[t] = (v4d){0, 0, 0, 0};
+    pi[t] = (v4d){0, 0, 0, 0};
+  }
+  fetch(0);
+  const int fi = lane & 15, fk = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += TK) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int kk = ak + 4 * i;
+      As[kk][am] = (k0 + kk < K && m0 + am < M) ? ra[i] : zmk(0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Bs[bk][bn + 16 * i] = (k0 + bk < K && n0 + bn + 16 * i < N) ? rb[i] : zmk(0, 0);
+    __syncthreads();
+    if (k0 + TK < K) fetch(k0 + TK);
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ++ks) {
+      const int kk = ks * 4 + fk;
+      const zd xb = Bs[kk][16 * wv + fi];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const zd va = As[kk][16 * t + fi];
+        pr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va.x, xb.x, pr[t], 0, 0, 0);
+        pi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va.x, xb.y, pi[t], 0, 0, 0);
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const zd va = As[kk][16 * t + fi];
+        pr[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-va.y, xb.y, pr[t], 0, 0, 0);
+        pi[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(va.y, xb.x, pi[t], 0, 0, 0);
+      }
+    }
+  }
+  const int nn = n0 + 16 * wv + fi;
+  if (nn >= N) return;
+  zd *Cb = C + b * sC + (int64_t)nn * ldc;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t m = m0 + 16 * t + fk + 4 * r;
+      if (m < M) {
+        zd v = zmul(alpha, zmk(pr[t][r], pi[t][r]));
+        if (BETA) v = zfma(beta, Cb[m], v);
+        Cb[m] = v;
+      }
+    }
 }
 
 void launch_gemm_nn(int nb, int64_t M, int N, int K, zd alpha, const zd *A, int64_t lda, int64_t sA,
                     const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,
                     hipStream_t s) {
-  launch_gemm(nb, 'N', 'N', M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, s);
+  if (M <= 0 || N <= 0 || nb <= 0) return;
+  if (K <= 0) throw HipError("gemm_nn: K must be positive");
+  const int tilesN = (N + 63) / 64;
+  const int64_t tiles = ((M + 63) / 64) * tilesN;
+  if (tiles > 0x7fffffff || nb > 65535) throw HipError("gemm_nn: grid too large");
+  dim3 g((unsigned)tiles, nb), t(256);
+  if (beta.x == 0.0 && beta.y == 0.0)
+    hipLaunchKernelGGL(k_zgemm_nn<false>, g, t, 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC,
+                       tilesN);
+  else
+    hipLaunchKernelGGL(k_zgemm_nn<true>, g, t, 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC,
+                       tilesN);
+  EMU_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------- w -= V[:, :k] h

@@ -31,56 +31,82 @@ __device__ __forceinline__ void max_merge(MaxRec &x, const MaxRec &y) {
 // phi3 at the six P2 nodes
 __constant__ double c_p3node[6][10];
 
+// One workgroup per 64-element chunk of one instance: the chunk's
+// interpolation blocks (64 x 144 doubles), Bloch phases and DOF indices are
+// staged in LDS once, then wave q walks modes q, q + 4, ...; lane l holds
+// element e0 + l, computes its 18 P2-node values of the mode and the wave
+// reduces them to the chunk's first largest one (scan order element, node,
+// component).  Each mode's partials over the chunks are merged by
+// k_field_max_final.
+constexpr int FM_E = 64;
+
+__device__ __forceinline__ void max_shfl(MaxRec &x, int m) {
+  MaxRec y;
+  y.a = __shfl_xor(x.a, m, 64);
+  y.idx = __shfl_xor(x.idx, m, 64);
+  y.v = zmk(__shfl_xor(x.v.x, m, 64), __shfl_xor(x.v.y, m, 64));
+  max_merge(x, y);
+}
+
 __global__ __launch_bounds__(256) void k_field_max_partial(int64_t nel, int nval, const int32_t *__restrict__ ldof,
                                                            const double *__restrict__ gt, const zd *__restrict__ X,
                                                            int64_t ldx, int64_t sX, const zd *__restrict__ ve,
                                                            int conj_phase, MaxRec *__restrict__ part) {
-  const int c = blockIdx.y;
-  const int b = blockIdx.z;
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  MaxRec best;
-  best.a = -1.0;
-  best.idx = INT64_MAX;
-  best.v = zmk(0, 0);
-  if (e < nel) {
-    const zd *x = X + b * sX + (int64_t)c * ldx;
-    const zd *vb = ve + ((int64_t)b * nel + e) * 14;
-    zd co[22];
+  __shared__ double Gs[FM_E][145];
+  __shared__ zd Ps[FM_E][15];
+  __shared__ int32_t Ls[FM_E][23];
+  const int chunk = blockIdx.x, nparts = gridDim.x;
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wq = tid >> 6;
+  const int64_t e0 = (int64_t)chunk * FM_E;
+  const int ne = (int)min((int64_t)FM_E, nel - e0);
+  for (int i = tid; i < ne * 144; i += 256) Gs[i / 144][i % 144] = gt[144 * e0 + i];
+  const zd *vb = ve + ((int64_t)b * nel + e0) * 14;
+  for (int i = tid; i < ne * 14; i += 256) {
+    zd p = vb[i];
+    if (conj_phase) p = zconj(p);
+    Ps[i / 14][i % 14] = p;
+  }
+  for (int i = tid; i < ne * 22; i += 256) Ls[i / 22][i % 22] = ldof[22 * e0 + i];
+  __syncthreads();
+  const bool live = lane < ne;
+  const int el = live ? lane : 0;
+  const zd *xb = X + b * sX;
+  for (int c = wq; c < nval; c += 4) {
+    const zd *x = xb + (int64_t)c * ldx;
+    MaxRec best;
+    best.a = -1.0;
+    best.idx = INT64_MAX;
+    best.v = zmk(0, 0);
+    if (live) {
+      zd co[22];
 #pragma unroll
-    for (int l = 0; l < 22; ++l) {
-      zd p = vb[ent_of(l)];
-      if (conj_phase) p = zconj(p);
-      co[l] = zmul(x[ldof[22 * e + l]], p);
-    }
-    const double *G = gt + 144 * e;
-    for (int inod = 0; inod < 6; ++inod) {
-      zd f0 = zmk(0, 0), f1 = zmk(0, 0), fz = zmk(0, 0);
+      for (int l = 0; l < 22; ++l) co[l] = zmul(x[Ls[el][l]], Ps[el][ent_of(l)]);
+      const int64_t e = e0 + el;
+      for (int inod = 0; inod < 6; ++inod) {
+        zd f0 = zmk(0, 0), f1 = zmk(0, 0), fz = zmk(0, 0);
 #pragma unroll
-      for (int l = 0; l < 12; ++l) {
-        f0 = zfma(zmk(G[(2 * inod) * 12 + l], 0), co[l], f0);
-        f1 = zfma(zmk(G[(2 * inod + 1) * 12 + l], 0), co[l], f1);
-      }
+        for (int l = 0; l < 12; ++l) {
+          f0 = zfma(zmk(Gs[el][(2 * inod) * 12 + l], 0), co[l], f0);
+          f1 = zfma(zmk(Gs[el][(2 * inod + 1) * 12 + l], 0), co[l], f1);
+        }
 #pragma unroll
-      for (int m = 0; m < 10; ++m) fz = zfma(zmk(c_p3node[inod][m], 0), co[12 + m], fz);
-      zd fv[3] = {f0, f1, fz};
-      for (int cc = 0; cc < 3; ++cc) {
-        double a = hypot(fv[cc].x, fv[cc].y);
-        if (a > best.a) {
-          best.a = a;
-          best.idx = e * 18 + inod * 3 + cc;
-          best.v = fv[cc];
+        for (int m = 0; m < 10; ++m) fz = zfma(zmk(c_p3node[inod][m], 0), co[12 + m], fz);
+        zd fv[3] = {f0, f1, fz};
+        for (int cc = 0; cc < 3; ++cc) {
+          double a = hypot(fv[cc].x, fv[cc].y);
+          if (a > best.a) {
+            best.a = a;
+            best.idx = e * 18 + inod * 3 + cc;
+            best.v = fv[cc];
+          }
         }
       }
     }
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) max_shfl(best, m);
+    if (lane == 0) part[((int64_t)b * nval + c) * nparts + chunk] = best;
   }
-  __shared__ MaxRec red[256];
-  red[threadIdx.x] = best;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) max_merge(red[threadIdx.x], red[threadIdx.x + st]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[((int64_t)b * nval + c) * gridDim.x + blockIdx.x] = red[0];
 }
 
 __global__ void k_field_max_final(int nparts, int nval, const MaxRec *__restrict__ part, zd *__restrict__ zmax) {
@@ -107,10 +133,10 @@ void launch_field_max(DevMesh &dm, int nb, int nval, const zd *X, int64_t ldx, i
     init = true;
   }
   int64_t nel = dm.host.nel;
-  int nparts = (int)((nel + 255) / 256);
+  int nparts = (int)((nel + FM_E - 1) / FM_E);
   MaxRec *part = reinterpret_cast<MaxRec *>(scratch);
-  hipLaunchKernelGGL(k_field_max_partial, dim3(nparts, nval, nb), dim3(256), 0, s, nel, nval, dm.ldof.p, dm.gt.p,
-                     X, ldx, sX, ve, conj_phase ? 1 : 0, part);
+  hipLaunchKernelGGL(k_field_max_partial, dim3(nparts, nb), dim3(256), 0, s, nel, nval, dm.ldof.p, dm.gt.p, X, ldx,
+                     sX, ve, conj_phase ? 1 : 0, part);
   EMU_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_field_max_final, dim3((nval + 63) / 64, nb), dim3(64), 0, s, nparts, nval, part, zmax);
   EMU_LAUNCH_CHECK();

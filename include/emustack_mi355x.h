@@ -174,7 +174,10 @@ int emu_calc_modes_1d(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_t nval,
 /* Batched form of emu_calc_modes_1d, instance-major like
  * emu_calc_modes_2d_batch: bloch_vec[nbatch*2] = (x, y) per instance; the
  * 2D outputs are nbatch blocks of 2*2neq_PW_2d*nval doubles.  opts and
- * stats as for emu_calc_modes_2d_batch. */
+ * stats as for emu_calc_modes_2d_batch, except that the default Krylov block
+ * size is 1: on these small banded pencils single vectors need about half
+ * the operator applications of 8-column blocks and give the deepest
+ * evanescent modes' beta several times more accurately. */
 int emu_calc_modes_1d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *lambda,
                             int64_t nval, int64_t ordre_ls, int64_t nb_typ_el, const double *n_eff,
                             const double *bloch_vec, const double *shift, int64_t itermax,
@@ -235,8 +238,10 @@ int emu_material_index(emu_ctx *ctx, int64_t nmat, const int64_t *kind, const in
  * solve sweep; enable = 0 stops.  out[k*3 + {0,1,2}] (may be NULL) for kernel
  * class k: launches timed, total milliseconds, total algorithmic bytes of
  * those launches.  Classes: 0 forward-solve level sweep (k_fwd), 1 backward
- * level sweep (k_bwd), 2 Gram-Schmidt projections (k_atb_partial),
- * 3 Gram-Schmidt updates (k_sub_vh_blk), 4 mass-matrix SpMV (k_spmm),
+ * level sweep (k_bwd), 2 Gram-Schmidt projections (k_atb_mfma16 for the
+ * 8-column blocks, k_atb_tall for smaller ones), 3 Gram-Schmidt updates
+ * (k_combine; k_sub_vh_blk for blocks below 8 columns), 4 mass-matrix
+ * product of the Krylov block (k_spmm_il; k_spmm below 8 columns),
  * 5..7 reserved. */
 int emu_profile(emu_ctx *ctx, int enable, int every, double *out);
 
