@@ -26,7 +26,7 @@ first-order error).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
                     [--precision fp64|mixed] [--no-cpu] [--no-compare]
-                    [--copy-calibration]
+                    [--copy-calibration] [--trace-summary DIR]
 """
 import argparse
 import json
@@ -57,10 +57,12 @@ FIXTURE = os.path.join(ROOT, 'tests', 'golden', 'configs', 'config3.npz')
 # default step), each scaled by what the counter reads for a plain 2 GiB copy;
 # summary and commands in profiles/kernels_stacked_gratings_8k.md.
 TRAFFIC_PER_DISPATCH = {
-    'k_atb_mfma16<true>': 8473211998,
-    'k_combine<16>': 7324606084,
-    'k_fwd': 433422318,
-    'k_bwd': 412855311,
+    'k_atb_mfma16<true>': 8473468313,
+    'k_combine<16>': 7323729510,
+    'k_fwd': 433522995,
+    'k_bwd': 413069824,
+    'k_spmm_il': 2804337561,
+    'k_zgemm_nn': 1602755174,
 }
 TRAFFIC_SOURCE = 'profiles/kernels_stacked_gratings_8k.md'
 
@@ -231,6 +233,56 @@ def cpu_baseline(mesh_paths, wls, n_incls, ops_per_instance, cores, nops=40):
     return dict(value=value, unit='wavelength-points/s', cores=cores, kind='port', sample=sample)
 
 
+def trace_summary(trace_dir, top=30):
+    """GPU-busy fraction and per-kernel time of the timed step in a rocprofv3
+    --kernel-trace CSV of `bench.py --steps 1 --warmup 1`: the step runs from
+    the end of the warmup step's cascade (k_prop dispatches, one batched
+    cascade per step) to the end of the timed step's; busy = union of the
+    kernel intervals inside it."""
+    import csv
+    import glob
+    import collections
+    files = glob.glob(os.path.join(trace_dir, '**', '*kernel_trace.csv'), recursive=True)
+    if not files:
+        raise SystemExit('no kernel_trace.csv under %s' % trace_dir)
+    rows = []
+    with open(files[0]) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r['Start_Timestamp']), int(r['End_Timestamp']), r['Kernel_Name']))
+    rows.sort()
+    casc = [i for i, r in enumerate(rows) if 'k_prop' in r[2]]
+    groups = []
+    for i in casc:
+        if groups and i - groups[-1][-1] <= 1000:
+            groups[-1].append(i)
+        else:
+            groups.append([i])
+    if len(groups) < 2:
+        raise SystemExit('expected a warmup and a timed step (cascade groups: %d)' % len(groups))
+    lo, hi = rows[groups[-2][-1]][1], rows[groups[-1][-1]][1]
+    win = sorted((max(a, lo), min(b, hi), n) for a, b, n in rows if b > lo and a < hi)
+    busy, cs, ce = 0, None, None
+    for a, b, _ in win:
+        if ce is None or a > ce:
+            if ce is not None:
+                busy += ce - cs
+            cs, ce = a, b
+        else:
+            ce = max(ce, b)
+    busy += (ce - cs) if ce is not None else 0
+    agg = collections.defaultdict(lambda: [0, 0])
+    for a, b, n in win:
+        k = agg[n.split('(')[0]]
+        k[0] += 1
+        k[1] += b - a
+    tot = sum(v[1] for v in agg.values())
+    print('timed step %.1f ms, GPU busy %.1f ms = %.1f%%, kernel time %.1f ms'
+          % ((hi - lo) / 1e6, busy / 1e6, 100.0 * busy / (hi - lo), tot / 1e6))
+    for name, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+        print('| `%s` | %d | %.1f | %.1f | %.1f%% |' % (name.replace('void ', '').replace('emu::', ''), c, t / 1e6,
+                                                       t / 1e3 / c, 100.0 * t / tot))
+
+
 def host_cores():
     """Cores this process may use (the box grants 16 per GPU)."""
     try:
@@ -261,9 +313,14 @@ def main():
     ap.add_argument('--no-compare', action='store_true', help='skip the step of the other precision form')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only copy 2 GiB device to device (the reference kernel for scaling HBM counters)')
+    ap.add_argument('--trace-summary', metavar='DIR',
+                    help='only summarise a rocprofv3 kernel trace of a --steps 1 --warmup 1 run (no GPU use)')
     ap.add_argument('--workdir', default=os.path.join(tempfile.gettempdir(), 'emustack_amd_bench_meshes'),
                     help='where the seeded bench meshes are written (regenerated when missing)')
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(args.trace_summary)
+        return
     if args.copy_calibration:
         import torch
         x = torch.ones(1 << 27, dtype=torch.complex128, device='cuda')
@@ -326,7 +383,7 @@ def main():
     cnt, ms, byt = prof[dom]
     achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     kernel_name = {'solve_fwd': 'k_fwd', 'solve_bwd': 'k_bwd', 'gs_project': 'k_atb_mfma16<true>',
-                   'gs_update': 'k_combine<16>', 'spmv_M': 'k_spmm'}[dom]
+                   'gs_update': 'k_combine<16>', 'spmv_M': 'k_spmm_il'}[dom]
     m = args.ncv or 2 * NUM_BMS + NUM_BMS // 2 + 3
     m = ((m + 7) // 8) * 8
     keep = min(m - 8, NUM_BMS + (m - NUM_BMS) // 2)

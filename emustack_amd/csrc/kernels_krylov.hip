@@ -492,7 +492,9 @@ void launch_swap_i32(int64_t n, int32_t *a, int32_t *b, hipStream_t s) {
 // register i / 4.
 
 template <bool CONJ>
-__global__ __launch_bounds__(256) void k_atb_mfma16(int64_t K, int M, int64_t kchunk, const zd *__restrict__ A,
+// three waves per SIMD (164 registers, no spills): more loads in flight
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+void k_atb_mfma16(int64_t K, int M, int64_t kchunk, const zd *__restrict__ A,
                                                     int64_t lda, int64_t sA, const zd *__restrict__ B, int64_t ldb,
                                                     int64_t sB, zd *__restrict__ part) {
   constexpr int TM = 64, TK = 32, NQ = 16;
@@ -733,7 +735,8 @@ void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64
 // registers while the current one is multiplied.  Launch order runs the N
 // tiles of one M tile next to each other, so A's rows are shared in L2.
 template <bool BETA>
-__global__ __launch_bounds__(256) void k_zgemm_nn(int64_t M, int N, int K, zd alpha, const zd *__restrict__ A,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+void k_zgemm_nn(int64_t M, int N, int K, zd alpha, const zd *__restrict__ A,
                                                   int64_t lda, int64_t sA, const zd *__restrict__ B, int64_t ldb,
                                                   int64_t sB, zd beta, zd *__restrict__ C, int64_t ldc, int64_t sC,
                                                   int tilesN) {
@@ -814,11 +817,122 @@ __global__ __launch_bounds__(256) void k_zgemm_nn(int64_t M, int N, int K, zd al
     }
 }
 
+// Tall products (M = the DOF count: restart combination, Ritz vectors): a
+// 128 x 64 tile per workgroup, waves in a 2 x 2 grid of 64 x 32 slices, so
+// that every A fragment read from LDS feeds two column blocks (six LDS reads
+// per 32 MFMAs instead of five per 16) and B is staged once per 128 rows.
+template <bool BETA>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+void k_zgemm_nn_tall(int64_t M, int N, int K, zd alpha, const zd *__restrict__ A,
+                                                       int64_t lda, int64_t sA, const zd *__restrict__ B,
+                                                       int64_t ldb, int64_t sB, zd beta, zd *__restrict__ C,
+                                                       int64_t ldc, int64_t sC, int tilesN) {
+  constexpr int TM = 128, TN = 64, TK = 16;
+  const int64_t tm = blockIdx.x / tilesN;
+  const int tn = blockIdx.x % tilesN;
+  const int64_t m0 = tm * TM;
+  const int n0 = tn * TN;
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wm = wv & 1, wn = wv >> 1;
+  const zd *Ab = A + b * sA;
+  const zd *Bb = B + b * sB;
+  __shared__ zd As[TK][TM + 1];
+  __shared__ zd Bs[TK][TN + 1];
+  // A tile: row am = tid & 127, K columns ak + 2 i; B tile: K row bk =
+  // tid & 15, columns bn + 16 i
+  const int am = tid & 127, ak = tid >> 7;
+  const int bk = tid & 15, bn = tid >> 4;
+  const int64_t arow = min(m0 + am, M - 1);
+  zd ra[8], rb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) ra[i] = Ab[arow + (int64_t)min(k0 + ak + 2 * i, K - 1) * lda];
+    const int kb = min(k0 + bk, K - 1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) rb[i] = Bb[kb + (int64_t)min(n0 + bn + 16 * i, N - 1) * ldb];
+  };
+  v4d pr[4][2], pi[4][2];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      pr[t][j] = (v4d){0, 0, 0, 0};
+      pi[t][j] = (v4d){0, 0, 0, 0};
+    }
+  fetch(0);
+  const int fi = lane & 15, fk = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += TK) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int kk = ak + 2 * i;
+      As[kk][am] = (k0 + kk < K && m0 + am < M) ? ra[i] : zmk(0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Bs[bk][bn + 16 * i] = (k0 + bk < K && n0 + bn + 16 * i < N) ? rb[i] : zmk(0, 0);
+    __syncthreads();
+    if (k0 + TK < K) fetch(k0 + TK);
+#pragma unroll
+    for (int ks = 0; ks < TK / 4; ++ks) {
+      const int kk = ks * 4 + fk;
+      zd xb[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) xb[j] = Bs[kk][32 * wn + 16 * j + fi];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const zd va = As[kk][64 * wm + 16 * t + fi];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          pr[t][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(va.x, xb[j].x, pr[t][j], 0, 0, 0);
+          pi[t][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(va.x, xb[j].y, pi[t][j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          pr[t][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-va.y, xb[j].y, pr[t][j], 0, 0, 0);
+          pi[t][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(va.y, xb[j].x, pi[t][j], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int nn = n0 + 32 * wn + 16 * j + fi;
+    if (nn >= N) continue;
+    zd *Cb = C + b * sC + (int64_t)nn * ldc;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t m = m0 + 64 * wm + 16 * t + fk + 4 * r;
+        if (m < M) {
+          zd v = zmul(alpha, zmk(pr[t][j][r], pi[t][j][r]));
+          if (BETA) v = zfma(beta, Cb[m], v);
+          Cb[m] = v;
+        }
+      }
+  }
+}
+
 void launch_gemm_nn(int nb, int64_t M, int N, int K, zd alpha, const zd *A, int64_t lda, int64_t sA,
                     const zd *B, int64_t ldb, int64_t sB, zd beta, zd *C, int64_t ldc, int64_t sC,
                     hipStream_t s) {
   if (M <= 0 || N <= 0 || nb <= 0) return;
   if (K <= 0) throw HipError("gemm_nn: K must be positive");
+  if (M >= 4096 && N >= 64) {  // tall: 128 x 64 tiles
+    const int tilesN = (N + 63) / 64;
+    const int64_t tiles = ((M + 127) / 128) * tilesN;
+    if (tiles > 0x7fffffff || nb > 65535) throw HipError("gemm_nn: grid too large");
+    dim3 g((unsigned)tiles, nb), t(256);
+    if (beta.x == 0.0 && beta.y == 0.0)
+      hipLaunchKernelGGL(k_zgemm_nn_tall<false>, g, t, 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
+                         sC, tilesN);
+    else
+      hipLaunchKernelGGL(k_zgemm_nn_tall<true>, g, t, 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
+                         sC, tilesN);
+    EMU_LAUNCH_CHECK();
+    return;
+  }
   const int tilesN = (N + 63) / 64;
   const int64_t tiles = ((M + 63) / 64) * tilesN;
   if (tiles > 0x7fffffff || nb > 65535) throw HipError("gemm_nn: grid too large");

@@ -14,6 +14,12 @@ path at those shapes without running the oracle on the GPU box.
   config3  the stacked-gratings workload of bench.py (three ~8,000-triangle
            layers r = 100 / 150 / 80 nm, two 50 nm films, 200 Bloch modes,
            max_order_PWs = 4) at one wavelength, normal incidence, TE.
+  config4  the uneven batch at reduced size: 3 distinct cells drawn from seed 4
+           (period U[400, 800] nm, elliptical inclusion with semi-axes
+           U[50, 250] nm capped at 0.4 period, rotation U[0, 90] deg, n
+           U[1.8, 3.8] + i U[0, 0.1], slab U[100, 300] nm on n = 1.5 under
+           air), meshes of 1,000-2,000 triangles (committed), 100 Bloch modes,
+           max_order_PWs = 2, wavelengths 500 and 700 nm, normal incidence, TE.
   case5    the reference's extraordinary-transmission case
            (tests/test_case_5_EOT_5_degrees.py.bak:44-74): square air holes
            d = 266 nm in a 200 nm Au film, p = 940 nm, the reference's own
@@ -26,7 +32,7 @@ path at those shapes without running the oracle on the GPU box.
 The oracle is the CPU restatement in oracle/ (scipy SuperLU + ARPACK);
 config 3 takes about an hour of CPU per layer.
 
-    python tests/golden/make_config_fixtures.py config1 config2 config3 case5
+    python tests/golden/make_config_fixtures.py config1 config2 config3 config4 case5
 """
 import hashlib
 import os
@@ -54,6 +60,7 @@ C2 = dict(period=400.0, diam=(80.0, 120.0), n=(1.45, 2.0, 3.6 + 0.005j), h=150.0
 # bench.py's workload; the wavelength is entry 64 of linspace(400, 1000, 512)
 C3 = dict(period=500.0, radii=(100.0, 150.0, 80.0), heights=(100.0, 150.0, 120.0), ntri=8000, nval=200, order=4,
           wl=float(np.linspace(400.0, 1000.0, 512)[64]))
+C4 = dict(seed=4, ncell=3, nval=100, order=2, wls=(500.0, 700.0))
 C5 = dict(period=940.0, h=200.0, wl=1.11 * 940, theta=5.0, phi=0.0, nval=100, order=4, mesh='4testing-940_266_sq.mail')
 
 
@@ -211,6 +218,60 @@ def config3(pool):
         out['JJd' + tag] = np.array([r[1] @ r[k] for r in res])
         out['rta' + tag] = np.array([o['a_list'][-1], o['r_list'][-1], o['t_list'][-1]], float)
     np.savez(os.path.join(OUT, 'config3.npz'), **out)
+
+
+def config4_cells():
+    """Cell parameters of the reduced uneven batch (seed 4)."""
+    rng = np.random.default_rng(C4['seed'])
+    cells = []
+    for i in range(C4['ncell']):
+        period = float(rng.uniform(400.0, 800.0))
+        a = min(float(rng.uniform(50.0, 250.0)), 0.4 * period)
+        b = min(float(rng.uniform(50.0, 250.0)), 0.4 * period)
+        rot = float(rng.uniform(0.0, 90.0))
+        n = complex(rng.uniform(1.8, 3.8), rng.uniform(0.0, 0.1))
+        h = float(rng.uniform(100.0, 300.0))
+        ntri = int(rng.integers(1000, 2001))
+        cells.append(dict(period=period, a=a, b=b, rot=rot, n=n, h=h, ntri=ntri,
+                          mesh='config4_cell%d.mail' % i))
+    return cells
+
+
+def config4_meshes():
+    """The committed meshes of the reduced uneven batch, written by the project
+    mesher when missing and checked against MESH_SHA256."""
+    from emustack_amd import mesher
+    out = []
+    for i, c in enumerate(config4_cells()):
+        p = os.path.join(MESHES, c['mesh'])
+        if not os.path.exists(p):
+            mesher.write_mail(p, mesher.ellipse_cell_ntri(c['period'], c['period'], c['a'], c['b'], c['ntri'],
+                                                          rotation_deg=c['rot'], seed=40 + i))
+        out.append(_checked(p) if c['mesh'] in MESH_SHA256 else p)
+    return out
+
+
+def config4(pool):
+    from oracle import stack_ref
+    cells = config4_cells()
+    paths = config4_meshes()
+    jobs = [(p, wl, c['period'], [1.0, c['n']], C4['nval'], C4['order']) for p, c in zip(paths, cells)
+            for wl in C4['wls']]
+    res = pool.map(_modes_2d, jobs)
+    nw = len(C4['wls'])
+    out = dict(wl=np.array(C4['wls']))
+    out['beta'] = np.array([[res[i * nw + k][0] for k in range(nw)] for i in range(len(cells))])
+    for tag, kk in (('', 2), ('_cluster', 3)):
+        rta = np.zeros((len(cells), nw, 3))
+        for i, c in enumerate(cells):
+            for k, wl in enumerate(C4['wls']):
+                r = res[i * nw + k]
+                L = [dict(kind='tf', n=1.5, h=None), dict(kind='ns', kz=r[0], J=r[1], Jd=r[kk], h=c['h'] / c['period']),
+                     dict(kind='tf', n=1.0, h=None)]
+                o = stack_ref.calc_scat(L, wl / c['period'], k_par_norm(wl, 0, 0, c['period']), C4['order'], pol='TE')
+                rta[i, k] = [o['a_list'][-1], o['r_list'][-1], o['t_list'][-1]]
+        out['rta' + tag] = rta
+    np.savez(os.path.join(OUT, 'config4.npz'), **out)
 
 
 def case5_au_index():

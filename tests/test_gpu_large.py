@@ -1,0 +1,97 @@
+"""The metal hole-array workload at its full size, on the GPU: a 200 nm
+Drude film (hbar omega_p = 9.0 eV, hbar gamma = 0.07 eV) with circular air
+holes r = 140 nm, square period 750 nm, on n = 1.5 under air; a ~30,000
+triangle mesh (project mesher, seeded); 400 Bloch modes, max_order_PWs = 6
+(169 plane waves per polarisation).  No oracle finishes this size on the CPU,
+so the test checks what holds at any size (parity unpinned at this size; the
+same path is pinned against the oracle on the smaller workloads):
+
+* every instance converges within the reference's 30 restarts;
+* R, T and A are physical (in [0, 1], R + T + A = 1 to 1e-9) for TE and TM;
+* R/T/A do not depend on the batch (a point solved alone and next to
+  another wavelength) or on the Krylov dimension, to 1e-8;
+* beta is invariant to the batch to 1e-9 relative.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HBAR_EV_S = 6.582119569e-16
+PERIOD, R_HOLE, H_FILM, NTRI, SEED = 750.0, 140.0, 200.0, 30000, 5
+NUM_BMS, ORDER = 400, 6
+
+
+def _mesh():
+    from emustack_amd import mesher
+    d = os.path.join(tempfile.gettempdir(), 'emustack_amd_large_meshes')
+    os.makedirs(d, exist_ok=True)
+    p = os.path.join(d, 'holes_p750_r140_s%d.mail' % SEED)
+    if not os.path.exists(p):
+        tmp = p + '.%d.tmp' % os.getpid()
+        mesher.write_mail(tmp, mesher.ellipse_cell_ntri(PERIOD, PERIOD, R_HOLE, R_HOLE, NTRI, seed=SEED))
+        os.replace(tmp, p)
+    return p
+
+
+def _drude():
+    from emustack_amd import materials
+    return materials.Material(np.array([9.0 / HBAR_EV_S, 0.07 / HBAR_EV_S, 1.0]))
+
+
+def _rta(st):
+    return np.array([float(st.a_list[-1]), float(st.r_list[-1]), float(st.t_list[-1])])
+
+
+@pytest.fixture(scope='module')
+def holes():
+    from emustack_amd import objects, materials
+    return objects.NanoStruct('2D_array', PERIOD, 2 * R_HOLE, height_nm=H_FILM, background=_drude(),
+                              inclusion_a=materials.Air, loss=True, make_mesh_now=False, mesh_file=_mesh())
+
+
+def _solve(holes, wls, ncv):
+    from emustack_amd import objects
+    from emustack_amd.batch import calc_modes_many
+    lights = [objects.Light(wl, max_order_PWs=ORDER, theta=0.0, phi=0.0) for wl in wls]
+    sims = calc_modes_many(holes, lights, num_BMs=NUM_BMS, ncv=ncv)
+    st = calc_modes_many.last_stats.copy()
+    print('wavelengths %s, Krylov dimension %d: restarts %d of 30, OP applications %d'
+          % (list(wls), ncv, st[1], st[0]), flush=True)
+    assert st[1] <= 30
+    return lights, sims
+
+
+def _stack_rta(light, sim, pol):
+    from emustack_amd import objects, materials, Stack
+    sub = objects.ThinFilm(PERIOD, height_nm='semi_inf', material=materials.Material(1.5 + 0j), loss=False)
+    sup = objects.ThinFilm(PERIOD, height_nm='semi_inf', material=materials.Air, loss=False)
+    st = Stack((sub.calc_modes(light), sim, sup.calc_modes(light)))
+    st.calc_scat(pol=pol)
+    return _rta(st)
+
+
+def test_metal_hole_array_full_size(holes):
+    lights, sims = _solve(holes, [800.0, 1000.0], 640)
+    for light, sim in zip(lights, sims):
+        for pol in ('TE', 'TM'):
+            a, r, t = _stack_rta(light, sim, pol)
+            print('%g nm %s: A %.6f R %.6f T %.6f' % (light.wl_nm, pol, a, r, t), flush=True)
+            for v in (a, r, t):
+                assert -1e-9 <= v <= 1 + 1e-9
+            assert abs(a + r + t - 1) <= 1e-9
+    # the same point alone, and alone with another Krylov dimension
+    l1, s1 = _solve(holes, [800.0], 640)
+    l2, s2 = _solve(holes, [800.0], 704)
+    eb = np.abs(np.asarray(s1[0].k_z) - np.asarray(sims[0].k_z)).max() / np.abs(sims[0].k_z).max()
+    print('beta, alone vs batched: %.2e' % eb, flush=True)
+    assert eb <= 1e-9
+    for pol in ('TE', 'TM'):
+        ref = _stack_rta(lights[0], sims[0], pol)
+        d1 = np.abs(_stack_rta(l1[0], s1[0], pol) - ref).max()
+        d2 = np.abs(_stack_rta(l2[0], s2[0], pol) - ref).max()
+        print('%s: R/T/A alone vs batched %.2e, Krylov dimension 704 vs 640 %.2e' % (pol, d1, d2), flush=True)
+        assert d1 <= 1e-8 and d2 <= 1e-8
