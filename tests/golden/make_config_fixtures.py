@@ -72,6 +72,9 @@ MESH_SHA256 = {
     'config3_r100_s1.mail': '58df7d1022fc5358c6ab3b4dc8bea27516ed925280101baa346cc9a40a45e7b7',
     'config3_r150_s2.mail': '8a4c307eaf79968979e36084474af9a5671c76172570d28dacf8dbd41728a28e',
     'config3_r80_s3.mail': '4508806418cf1cefe0da010b1527843336747521a977b29e8b3f8013dcea5729',
+    'config4_cell0.mail': 'b5980f95b45408d4365697b9717c8d5d500463d613b2030ba0bb55a5ad17fed5',
+    'config4_cell1.mail': '282566be319e64dcc85afefc56cc43e8b5f31e1c4b772f52054e768dbae8307c',
+    'config4_cell2.mail': '6f50d37fa577256eded939b43c4db0e0d45c81335ec392b830891626681c88b2',
     '4testing-940_266_sq.mail': 'f7a751275e8c0b1de983f554f75e30f55cb5ba4811928e00a8e62824480091d2',
 }
 MESH_CACHE = os.path.join(tempfile.gettempdir(), 'emustack_amd_config_meshes')
@@ -247,7 +250,7 @@ def config4_meshes():
         if not os.path.exists(p):
             mesher.write_mail(p, mesher.ellipse_cell_ntri(c['period'], c['period'], c['a'], c['b'], c['ntri'],
                                                           rotation_deg=c['rot'], seed=40 + i))
-        out.append(_checked(p) if c['mesh'] in MESH_SHA256 else p)
+        out.append(_checked(p))
     return out
 
 

@@ -7,6 +7,10 @@ tests/golden/make_config_fixtures.py):
            one batch;
   config3  the bench's stacked gratings (three 8,000-triangle layers, 200
            Bloch modes) at one wavelength;
+  config4  the uneven batch at reduced size: three distinct cells (periods,
+           ellipses, rotations, indices, slab heights and mesh sizes drawn from
+           a seed), 100 Bloch modes, two wavelengths each, all solved in one
+           calc_modes_stack call (concurrent sub-batches of different meshes);
   case5    the reference's extraordinary-transmission case (square holes in
            a 200 nm Au film, its own 2,528-triangle mesh, 100 Bloch modes,
            theta = 5 deg, TM), whose own fixture file cannot be read.
@@ -180,6 +184,32 @@ def test_config3_against_oracle_and_batch_invariant():
     spread = np.abs(_rta(batch[i][0]) - ref).max()
     print('config3 batched vs alone: R/T/A spread %.2e' % spread, flush=True)
     assert spread <= 1e-8
+
+
+def test_config4_uneven_batch_against_oracle():
+    from emustack_amd import objects, materials, Stack
+    from emustack_amd.batch import calc_modes_stack
+    fx = _load('config4')
+    cells = cf.config4_cells()
+    paths = cf.config4_meshes()
+    ns = [objects.NanoStruct('2D_array', c['period'], 2 * c['a'], height_nm=c['h'], background=materials.Air,
+                             inclusion_a=materials.Material(c['n']), loss=True, make_mesh_now=False, mesh_file=p)
+          for c, p in zip(cells, paths)]
+    lights = [objects.Light(wl, max_order_PWs=cf.C4['order'], theta=0.0, phi=0.0) for wl in fx['wl']]
+    sims = calc_modes_stack(ns, lights, num_BMs=cf.C4['nval'], groups=2, workers=3)
+    _restarts('config4', calc_modes_stack.last_stats)
+    for i, c in enumerate(cells):
+        sub, sup = _films(c['period'])
+        for k, light in enumerate(lights):
+            lab = 'config4 cell %d (%d triangles) wl %g' % (i, c['ntri'], light.wl_nm)
+            sim = sims[i][k]
+            _check_beta(sim.k_z, fx['beta'][i][k], lab)
+            st = Stack((sub.calc_modes(light), sim, sup.calc_modes(light)))
+            st.calc_scat(pol='TE')
+            err = np.abs(_rta(st) - fx['rta_cluster'][i][k]).max()
+            print('%s: R/T/A err %.2e (per-mode normalisation of the oracle: %.2e)'
+                  % (lab, err, np.abs(_rta(st) - fx['rta'][i][k]).max()), flush=True)
+            assert err <= 1e-8
 
 
 def test_case5_eot_against_oracle():
