@@ -40,6 +40,7 @@ EXPORTS = [
     'emu_mesh_geometry', 'emu_calc_modes_2d', 'emu_calc_modes_2d_batch', 'emu_assemble', 'emu_lu_solve',
     'emu_profile', 'emu_mesh_create_1d', 'emu_calc_modes_1d', 'emu_calc_modes_1d_batch', 'emu_star_cascade_batch',
     'emu_material_index', 'emu_assemble_noalloc', 'emu_calc_modes_2d_fields', 'emu_calc_modes_1d_fields',
+    'emu_calc_modes_2d_submit', 'emu_job_wait',
 ]
 
 
@@ -145,6 +146,10 @@ def lib():
                                               c_i64p]
         L.emu_material_index.argtypes = [c_vp, c_i64, c_i64p, c_i64p, c_dp, c_dp, c_i64, c_i64p, c_dp, c_dp]
         L.emu_assemble_noalloc.argtypes = [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]
+        L.emu_calc_modes_2d_submit.argtypes = [c_vp, c_vp, c_i64, c_dp, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_i64,
+                                               c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_i64p,
+                                               ctypes.POINTER(ctypes.c_void_p)]
+        L.emu_job_wait.argtypes = [c_vp]
         L.emu_calc_modes_2d_fields.argtypes = [c_vp, c_vp, ctypes.c_double, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp,
                                                c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_dp]
         L.emu_calc_modes_1d_fields.argtypes = [c_vp, c_vp, ctypes.c_double, c_i64, c_i64, c_i64, c_dp,
@@ -175,6 +180,29 @@ def as_f64(a):
     """Complex array -> contiguous interleaved float64 view."""
     a = np.ascontiguousarray(a, dtype=np.complex128)
     return a.view(np.float64)
+
+
+class Job:
+    """A batch queued with Context.calc_modes_batch_submit: owns the output
+    arrays (and keeps the mesh handle alive) until wait() returns them."""
+
+    def __init__(self, nl, nval, neq_pw, mesh):
+        self.beta = np.zeros((nl, nval), np.complex128)
+        self.J = np.zeros((nl, nval, 2 * neq_pw), np.complex128)
+        self.Jd = np.zeros((nl, 2 * neq_pw, nval), np.complex128)
+        self.st = np.zeros(8, np.int64)
+        self.mesh = mesh
+        self.h = None
+
+    def wait(self):
+        if self.h is None:
+            raise RuntimeError('job already waited on (or never submitted)')
+        h, self.h = self.h, None
+        try:
+            check(lib().emu_job_wait(h))
+        finally:
+            self._blas.__exit__(None, None, None)
+        return self.beta, self.J.transpose(0, 2, 1), self.Jd.transpose(0, 2, 1), self.st
 
 
 class Context:
@@ -234,6 +262,37 @@ class Context:
                                                 ptr(beta.view(np.float64)), ptr(J.view(np.float64)),
                                                 ptr(Jd.view(np.float64)), ptr(st, c_i64p)))
         return beta, J.transpose(0, 2, 1), Jd.transpose(0, 2, 1), st
+
+    def calc_modes_batch_submit(self, mesh, lam, nval, ordre, n_eff, bloch, shift, neq_pw, E_H_field=1,
+                                itermax=30, tol=0.0, block=0, ncv=0, precision=0):
+        """calc_modes_batch queued on this context's worker thread; returns a
+        Job whose wait() gives (beta, J, Jd, stats) as calc_modes_batch does."""
+        lam = np.ascontiguousarray(np.atleast_1d(np.asarray(lam, np.float64)))
+        nl = lam.shape[0]
+        n_eff = np.ascontiguousarray(np.asarray(n_eff, np.complex128).reshape(nl, -1))
+        ntyp = n_eff.shape[1]
+        bloch = np.ascontiguousarray(np.asarray(bloch, np.float64).reshape(nl, 2))
+        shift = np.ascontiguousarray(np.asarray(shift, np.complex128).reshape(nl))
+        job = Job(nl, nval, neq_pw, mesh)
+        job.ctx = self  # the context (and its worker thread) must outlive the job
+        opts = np.array([tol, float(precision), float(block), float(ncv)])
+        h = ctypes.c_void_p()
+        # single-threaded host LAPACK from now until the job is waited on: the
+        # library's worker thread may reach its Schur steps at any time
+        job._blas = _host_lapack_threads(nl)
+        job._blas.__enter__()
+        try:
+            check(lib().emu_calc_modes_2d_submit(self.h, mesh.h, nl, ptr(lam), nval, ordre, ntyp,
+                                                 ptr(n_eff.view(np.float64)), ptr(bloch),
+                                                 ptr(shift.view(np.float64)), E_H_field, itermax, neq_pw, ptr(opts),
+                                                 ptr(job.beta.view(np.float64)), ptr(job.J.view(np.float64)),
+                                                 ptr(job.Jd.view(np.float64)), ptr(job.st, c_i64p),
+                                                 ctypes.byref(h)))
+        except BaseException:
+            job._blas.__exit__(None, None, None)
+            raise
+        job.h = h
+        return job
 
     def calc_modes_2d_fields(self, mesh, lam, nval, ordre, n_eff, bloch, shift, neq_pw, E_H_field=1, itermax=30):
         """One point with the plotting fields: beta (nval,), J (2P, nval),

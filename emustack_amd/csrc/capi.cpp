@@ -1,18 +1,57 @@
 // extern "C" entry points of libemu_mi355x.so (declared in
 // include/emustack_mi355x.h).  Converts C++ exceptions to status codes and a
 // thread-local message.
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <exception>
+#include <functional>
+#include <mutex>
 #include <new>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/emustack_mi355x.h"
 #include "common.h"
 #include "engine_api.h"
 
+// A context runs one engine call at a time (run_mu).  Submitted jobs queue on
+// the context's worker thread, started on the first submission and joined
+// (after draining the queue) by emu_ctx_destroy.
 struct emu_ctx {
-  emu::Ctx *c;
+  emu::Ctx *c = nullptr;
+  std::mutex run_mu;
+  std::mutex q_mu;
+  std::condition_variable q_cv;
+  std::deque<std::function<void()>> q;
+  bool stop = false;
+  std::thread worker;
 };
+
+// A submitted batch: owned copies of the inputs, its status and message
+struct emu_job {
+  std::mutex mu;
+  std::condition_variable cv;
+  bool done = false;
+  int status = 0;
+  std::string err;
+  std::vector<double> lambda, n_eff, bloch, shift, opts;
+};
+
+static void worker_loop(emu_ctx *ctx) {
+  for (;;) {
+    std::function<void()> f;
+    {
+      std::unique_lock<std::mutex> lk(ctx->q_mu);
+      ctx->q_cv.wait(lk, [&] { return ctx->stop || !ctx->q.empty(); });
+      if (ctx->q.empty()) return;  // stop requested and nothing left
+      f = std::move(ctx->q.front());
+      ctx->q.pop_front();
+    }
+    f();
+  }
+}
 struct emu_mesh {
   emu::DevMesh *m;
 };
@@ -69,6 +108,12 @@ int emu_ctx_create(int device, emu_ctx **out) {
 
 void emu_ctx_destroy(emu_ctx *ctx) {
   if (!ctx) return;
+  {
+    std::lock_guard<std::mutex> lk(ctx->q_mu);
+    ctx->stop = true;
+  }
+  ctx->q_cv.notify_all();
+  if (ctx->worker.joinable()) ctx->worker.join();
   emu::ctx_destroy(ctx->c);
   delete ctx;
 }
@@ -154,6 +199,7 @@ static int modes2d_impl(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const doub
   if (E_H_field != 1 && E_H_field != 2) return fail(EMU_E_ARG, "E_H_field must be 1 or 2");
   for (int64_t i = 0; i < nbatch; ++i)
     if (!(lambda[i] > 0)) return fail(EMU_E_ARG, "lambda must be positive");
+  std::lock_guard<std::mutex> run(ctx->run_mu);
   EMU_GUARD({
     emu::ModesArgs a;
     a.nl = nbatch;
@@ -215,6 +261,62 @@ int emu_calc_modes_2d_fields(emu_ctx *ctx, emu_mesh *mesh, double lambda, int64_
                       neq_PW, nullptr, beta1, overlap_J, overlap_J_dagger, nullptr, sol1, mode_pol);
 }
 
+int emu_calc_modes_2d_submit(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *lambda, int64_t nval,
+                             int64_t ordre_ls, int64_t nb_typ_el, const double *n_eff, const double *bloch_vec,
+                             const double *shift, int64_t E_H_field, int64_t itermax, int64_t neq_PW,
+                             const double *opts, double *beta1, double *overlap_J, double *overlap_J_dagger,
+                             int64_t *stats, emu_job **out) {
+  if (!out) return fail(EMU_E_ARG, "emu_calc_modes_2d_submit: out is NULL");
+  *out = nullptr;
+  if (!ctx || !mesh || !lambda || !n_eff || !bloch_vec || !shift || !beta1 || !overlap_J || !overlap_J_dagger)
+    return fail(EMU_E_ARG, "emu_calc_modes_2d_submit: NULL argument");
+  if (nbatch < 1 || nb_typ_el < 1) return fail(EMU_E_ARG, "emu_calc_modes_2d_submit: invalid size");
+  EMU_GUARD({
+    emu_job *job = new emu_job;
+    job->lambda.assign(lambda, lambda + nbatch);
+    job->n_eff.assign(n_eff, n_eff + 2 * nbatch * nb_typ_el);  // complex per (instance, material)
+    job->bloch.assign(bloch_vec, bloch_vec + 2 * nbatch);
+    job->shift.assign(shift, shift + 2 * nbatch);
+    if (opts) job->opts.assign(opts, opts + 4);
+    auto run = [=]() {
+      const int st = modes2d_impl(ctx, mesh, nbatch, job->lambda.data(), nval, ordre_ls, nb_typ_el, job->n_eff.data(),
+                                  job->bloch.data(), job->shift.data(), E_H_field, itermax, neq_PW,
+                                  job->opts.empty() ? nullptr : job->opts.data(), beta1, overlap_J,
+                                  overlap_J_dagger, stats, nullptr, nullptr);
+      std::string msg = st == EMU_OK ? std::string() : g_err;
+      {
+        std::lock_guard<std::mutex> lk(job->mu);
+        job->status = st;
+        job->err = msg;
+        job->done = true;
+      }
+      job->cv.notify_all();
+    };
+    {
+      std::lock_guard<std::mutex> lk(ctx->q_mu);
+      if (!ctx->worker.joinable()) ctx->worker = std::thread(worker_loop, ctx);
+      ctx->q.push_back(run);
+    }
+    ctx->q_cv.notify_one();
+    *out = job;
+  })
+}
+
+int emu_job_wait(emu_job *job) {
+  if (!job) return fail(EMU_E_ARG, "emu_job_wait: NULL job");
+  int st;
+  std::string msg;
+  {
+    std::unique_lock<std::mutex> lk(job->mu);
+    job->cv.wait(lk, [&] { return job->done; });
+    st = job->status;
+    msg = job->err;
+  }
+  delete job;
+  if (st != EMU_OK) return fail(st, msg);
+  return EMU_OK;
+}
+
 int emu_mesh_create_1d(emu_ctx *ctx, int64_t npt_P2, int64_t nel, const int64_t *table_nod,
                        const int64_t *type_el, const double *x_P2, int leaf_size, emu_mesh **out) {
   if (!ctx || !table_nod || !type_el || !x_P2 || !out) return fail(EMU_E_ARG, "emu_mesh_create_1d: NULL argument");
@@ -248,6 +350,7 @@ static int modes1d_impl(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const doub
     return fail(EMU_E_ARG, "world_1d = 0 needs the 2D overlap outputs and neq_PW_2d");
   for (int64_t i = 0; i < nbatch; ++i)
     if (!(lambda[i] > 0)) return fail(EMU_E_ARG, "lambda must be positive");
+  std::lock_guard<std::mutex> run(ctx->run_mu);
   EMU_GUARD({
     emu::ModesArgs a;
     a.nl = nbatch;

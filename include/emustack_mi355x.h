@@ -145,6 +145,24 @@ int emu_calc_modes_2d_batch(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch,
                             int64_t neq_PW, const double *opts, double *beta1,
                             double *overlap_J, double *overlap_J_dagger, int64_t *stats);
 
+/* Asynchronous form of emu_calc_modes_2d_batch (the batch submit / wait pair
+ * of the drop-in boundary; the reference's Python runs one blocking f2py call
+ * per point, mode_calcs.py:371-378).  Copies the inputs, queues the batch on
+ * the context's worker thread and returns at once with a job handle; the
+ * outputs (caller-owned, kept alive until the wait) are complete when
+ * emu_job_wait returns.  A context runs its jobs, and any other calculation
+ * called on it, one at a time in submission order; batches on different
+ * contexts run concurrently.  Every job must be waited on exactly once;
+ * emu_job_wait returns the batch's status (and sets emu_last_error) and
+ * frees the handle. */
+typedef struct emu_job emu_job;
+int emu_calc_modes_2d_submit(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, const double *lambda, int64_t nval,
+                             int64_t ordre_ls, int64_t nb_typ_el, const double *n_eff, const double *bloch_vec,
+                             const double *shift, int64_t E_H_field, int64_t itermax, int64_t neq_PW,
+                             const double *opts, double *beta1, double *overlap_J, double *overlap_J_dagger,
+                             int64_t *stats, emu_job **out);
+int emu_job_wait(emu_job *job);
+
 /* 1D gratings.  Create a mesh handle from the arrays the reference passes to
  * calc_modes_1d (py_calc_modes_1d.f:2-12, built by objects.py:1054-1530):
  * x_P2[npt_P2] node coordinates in [0, 1], table_nod[3*nel] (element-major,

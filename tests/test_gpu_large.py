@@ -2,9 +2,11 @@
 Drude film (hbar omega_p = 9.0 eV, hbar gamma = 0.07 eV) with circular air
 holes r = 140 nm, square period 750 nm, on n = 1.5 under air; a ~30,000
 triangle mesh (project mesher, seeded); 400 Bloch modes, max_order_PWs = 6
-(169 plane waves per polarisation).  No oracle finishes this size on the CPU,
-so the test checks what holds at any size (parity unpinned at this size; the
-same path is pinned against the oracle on the smaller workloads):
+(113 plane waves per polarisation); normal incidence at 800 and 1000 nm and
+20 degrees at 800 nm in one batch (TE and TM share the mode solve).  No
+oracle finishes this size on the CPU, so the test checks what holds at any
+size (parity unpinned at this size; the same path is pinned against the
+oracle on the smaller workloads):
 
 * every instance converges within the reference's 30 restarts;
 * R, T and A are physical (in [0, 1], R + T + A = 1 to 1e-9) for TE and TM;
@@ -53,10 +55,11 @@ def holes():
                               inclusion_a=materials.Air, loss=True, make_mesh_now=False, mesh_file=_mesh())
 
 
-def _solve(holes, wls, ncv):
+def _solve(holes, wls, ncv, thetas=None):
     from emustack_amd import objects
     from emustack_amd.batch import calc_modes_many
-    lights = [objects.Light(wl, max_order_PWs=ORDER, theta=0.0, phi=0.0) for wl in wls]
+    thetas = thetas or [0.0] * len(wls)
+    lights = [objects.Light(wl, max_order_PWs=ORDER, theta=th, phi=0.0) for wl, th in zip(wls, thetas)]
     sims = calc_modes_many(holes, lights, num_BMs=NUM_BMS, ncv=ncv)
     st = calc_modes_many.last_stats.copy()
     print('wavelengths %s, Krylov dimension %d: restarts %d of 30, OP applications %d'
@@ -75,11 +78,12 @@ def _stack_rta(light, sim, pol):
 
 
 def test_metal_hole_array_full_size(holes):
-    lights, sims = _solve(holes, [800.0, 1000.0], 640)
-    for light, sim in zip(lights, sims):
+    thetas = [0.0, 0.0, 20.0]
+    lights, sims = _solve(holes, [800.0, 1000.0, 800.0], 640, thetas=thetas)
+    for light, sim, th in zip(lights, sims, thetas):
         for pol in ('TE', 'TM'):
             a, r, t = _stack_rta(light, sim, pol)
-            print('%g nm %s: A %.6f R %.6f T %.6f' % (light.wl_nm, pol, a, r, t), flush=True)
+            print('%g nm, theta %g, %s: A %.6f R %.6f T %.6f' % (light.wl_nm, th, pol, a, r, t), flush=True)
             for v in (a, r, t):
                 assert -1e-9 <= v <= 1 + 1e-9
             assert abs(a + r + t - 1) <= 1e-9

@@ -215,3 +215,26 @@ def test_small_blocks_match_oracle(case1_500, gpu_ctx, mesh600, block):
     np.testing.assert_allclose(beta[0], bo, rtol=1e-9, atol=1e-9 * np.abs(bo).max())
     Po = Jo @ Jdo
     assert np.abs(J[0] @ Jd[0] - Po).max() <= 1e-8 * np.abs(Po).max()
+
+
+def test_submitted_batches_equal_blocking_calls(gpu_ctx, mesh600, mats):
+    """emu_calc_modes_2d_submit / emu_job_wait: two batches queued on one
+    context and one on a second context give what the blocking calls give."""
+    from emustack_amd import _lib
+    mh, m = mesh600
+    wls = [500.0, 700.0, 1000.0]
+    P = [_params(wl, 600., mats('Si_c', wl), 20, 40) for wl in wls]
+    ref = [gpu_ctx.calc_modes_batch(mh, [p[0]], 20, 1, [p[2]], [p[3]], [p[4]], 5) for p in P]
+    ctx2 = _lib.Context(0)
+    mh2 = _lib.MeshHandle(MESH_600, ctx2)
+    jobs = [gpu_ctx.calc_modes_batch_submit(mh, [P[0][0]], 20, 1, [P[0][2]], [P[0][3]], [P[0][4]], 5),
+            gpu_ctx.calc_modes_batch_submit(mh, [P[1][0]], 20, 1, [P[1][2]], [P[1][3]], [P[1][4]], 5),
+            ctx2.calc_modes_batch_submit(mh2, [P[2][0]], 20, 1, [P[2][2]], [P[2][3]], [P[2][4]], 5)]
+    for j, r in zip(jobs, ref):
+        b, J, Jd, st = j.wait()
+        np.testing.assert_allclose(b[0], r[0][0], rtol=1e-12)
+        np.testing.assert_allclose(J[0] @ Jd[0], r[1][0] @ r[2][0], rtol=0,
+                                   atol=1e-10 * np.abs(r[1][0] @ r[2][0]).max())
+        assert st[1] == r[3][1]
+    with pytest.raises(RuntimeError):
+        jobs[0].wait()
