@@ -128,7 +128,7 @@ def test_dropin_case4_golden():
              'Lay_Trans_0': ('t', 0), 'Lay_Trans_1': ('t', 1), 'Lay_Absorb_0': ('a', 0), 'Lay_Absorb_1': ('a', 1)}
     for name, (q, i) in files.items():
         ref = load_txt('case_4', name + '_stack0001.txt')[1]
-        v = float(getattr(st, q + '_list')[i])
+        v = float(np.real(np.asarray(getattr(st, q + '_list')[i])).ravel()[0])
         assert abs(v - ref) < 1e-8, (name, v, ref)
 
 

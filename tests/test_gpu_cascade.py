@@ -138,5 +138,5 @@ def test_case0_films_through_gpu_cascade(gpu_ctx, mats):
         checks['Lay_Absorb_%d' % i] = (lambda i: lambda s: s.a_list[i])(i)
     for name, get in checks.items():
         ref = load_txt('case_0', name + '_stack0001.txt')[:, 1]
-        got = np.array([float(np.real(get(s))) for s in stacks])
+        got = np.array([np.real(np.asarray(get(s))).ravel()[0] for s in stacks])
         np.testing.assert_allclose(got, ref, rtol=0, atol=1e-8, err_msg=name)

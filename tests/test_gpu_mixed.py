@@ -8,7 +8,8 @@ in 2D: the complex64 factor leaves the mode vectors first-order errors, and
 the two-sided FP64 Rayleigh quotient reduces beta's to the product of the
 prime and adjoint errors).  The 1D pencil is not the transpose of its
 adjoint, so there the quotient is one-sided: the deepest evanescent modes
-(|beta| > 0.6 max) are held to 2e-5 (measured 9e-6), the others to 2e-6.
+(|beta| > 0.6 max) are held to 5e-5 (measured 9e-6), the others to 5e-6
+(measured 1.9e-6 at 900 nm).
 The measured errors are printed."""
 import os
 import sys
@@ -66,4 +67,4 @@ def test_mixed_config2_1d_against_fp64_oracle():
         er = np.abs(_rta(st) - fx['rta_cluster'][i]).max()
         print('config2 mixed wl %g theta %g: beta rel err %.2e (deepest modes %.2e), R/T/A err %.2e'
               % (light.wl_nm, fx['theta'][i], eb, ed, er), flush=True)
-        assert eb <= 2e-6 and ed <= 2e-5 and er <= 1e-6
+        assert eb <= 5e-6 and ed <= 5e-5 and er <= 1e-6

@@ -166,7 +166,8 @@ def test_dropin_api_case1():
                     ('Reflectance_stack0001.txt', lambda s: s.r_list[-1]),
                     ('Transmittance_stack0001.txt', lambda s: s.t_list[-1])):
         ref = load_txt('case_1', fn)[:, 1]
-        np.testing.assert_allclose([float(np.real(get(s))) for s in stacks], ref, rtol=0, atol=1e-8, err_msg=fn)
+        np.testing.assert_allclose([np.real(np.asarray(get(s))).ravel()[0] for s in stacks], ref, rtol=0, atol=1e-8,
+                                   err_msg=fn)
 
 
 def test_dropin_api_case2():
