@@ -102,13 +102,15 @@ def run_step(ns, film, sub, sup, wls, groups=1, ncv=0, workers=None, precision='
     sub-batches per layer, Krylov dimension ncv), then the cascade.  Returns
     R/T/A per point and the summed solver stats."""
     from emustack_amd import objects, Stack
-    from emustack_amd.batch import calc_modes_stack, calc_scat_many
+    from emustack_amd.batch import calc_films_many, calc_modes_stack, calc_scat_many
     lights = [objects.Light(wl, max_order_PWs=ORDER, theta=0.0, phi=0.0) for wl in wls]
     sims = calc_modes_stack(ns, lights, num_BMs=NUM_BMS, groups=groups, ncv=ncv, workers=workers,
                             precision=precision)
     stats = calc_modes_stack.last_stats.copy()
-    stacks = [Stack((sub.calc_modes(light), sims[0][i], film.calc_modes(light), sims[1][i],
-                     film.calc_modes(light), sims[2][i], sup.calc_modes(light))) for i, light in enumerate(lights)]
+    # film modes of all points, one launch per film layer
+    fl = [calc_films_many(f, lights) for f in (sub, film, film, sup)]
+    stacks = [Stack((fl[0][i], sims[0][i], fl[1][i], sims[1][i], fl[2][i], sims[2][i], fl[3][i]))
+              for i in range(len(lights))]
     calc_scat_many(stacks, pol='TE')  # batched star-product cascade on the GPU
     out = np.array([(float(st.a_list[-1]), float(st.r_list[-1]), float(st.t_list[-1])) for st in stacks])
     return out, stats

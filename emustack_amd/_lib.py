@@ -40,7 +40,7 @@ EXPORTS = [
     'emu_mesh_geometry', 'emu_calc_modes_2d', 'emu_calc_modes_2d_batch', 'emu_assemble', 'emu_lu_solve',
     'emu_profile', 'emu_mesh_create_1d', 'emu_calc_modes_1d', 'emu_calc_modes_1d_batch', 'emu_star_cascade_batch',
     'emu_material_index', 'emu_assemble_noalloc', 'emu_calc_modes_2d_fields', 'emu_calc_modes_1d_fields',
-    'emu_calc_modes_2d_submit', 'emu_job_wait',
+    'emu_calc_modes_2d_submit', 'emu_job_wait', 'emu_film_modes_batch',
 ]
 
 
@@ -145,6 +145,7 @@ def lib():
                                               c_i64, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_dp, c_dp,
                                               c_i64p]
         L.emu_material_index.argtypes = [c_vp, c_i64, c_i64p, c_i64p, c_dp, c_dp, c_i64, c_i64p, c_dp, c_dp]
+        L.emu_film_modes_batch.argtypes = [c_vp, c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_i64p, c_dp, c_dp]
         L.emu_assemble_noalloc.argtypes = [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]
         L.emu_calc_modes_2d_submit.argtypes = [c_vp, c_vp, c_i64, c_dp, c_i64, c_i64, c_i64, c_dp, c_dp, c_dp, c_i64,
                                                c_i64, c_i64, c_dp, c_dp, c_dp, c_dp, c_i64p,
@@ -432,6 +433,25 @@ class Context:
                                        ptr(xs) if xs.size else None, ptr(ys.view(np.float64)) if ys.size else None,
                                        qmat.size, ptr(qmat, c_i64p), ptr(qwl), ptr(out.view(np.float64))))
         return out
+
+    def film_modes(self, n, wl_norm, alpha, beta, order):
+        """emu_film_modes_batch: k_z and Z (nbatch x 2 npw complex, TE block
+        then TM block) of homogeneous films; n, wl_norm per instance, alpha,
+        beta (nbatch x npw) per order, order (nbatch x npw) the stored order."""
+        n = np.ascontiguousarray(n, np.complex128).reshape(-1)
+        wl_norm = np.ascontiguousarray(wl_norm, np.float64).reshape(-1)
+        alpha = np.ascontiguousarray(alpha, np.float64)
+        beta = np.ascontiguousarray(beta, np.float64)
+        order = np.ascontiguousarray(order, np.int64)
+        nb, npw = alpha.shape
+        if n.size != nb or wl_norm.size != nb or beta.shape != (nb, npw) or order.shape != (nb, npw):
+            raise ValueError("film_modes: inconsistent shapes")
+        kz = np.zeros((nb, 2 * npw), np.complex128)
+        Z = np.zeros((nb, 2 * npw), np.complex128)
+        check(lib().emu_film_modes_batch(self.h, nb, npw, ptr(n.view(np.float64)), ptr(wl_norm), ptr(alpha),
+                                         ptr(beta), ptr(order, c_i64p), ptr(kz.view(np.float64)),
+                                         ptr(Z.view(np.float64))))
+        return kz, Z
 
     PROFILE_CLASSES = ('solve_fwd', 'solve_bwd', 'gs_project', 'gs_update', 'spmv_M')
 

@@ -15,7 +15,7 @@ import numpy as np
 from . import EMUstack, paths
 from .mode_calcs import Simmo
 
-__all__ = ['calc_modes_many', 'calc_modes_stack', 'calc_scat_many']
+__all__ = ['calc_modes_many', 'calc_modes_stack', 'calc_films_many', 'calc_scat_many']
 
 _pools = {}
 _pool_lock = threading.Lock()
@@ -161,6 +161,45 @@ def preload_meshes(structures, groups=2, workers=None):
 
 calc_modes_stack.last_stats = None
 calc_modes_many.last_stats = None
+
+
+def calc_films_many(film, lights):
+    """ThinFilm.calc_modes for many lights (mode_calcs.py:110-212 of the
+    reference, once per light there): the k_z and impedances of every
+    (light, order) in one GPU launch (emu_film_modes_batch).  The orders,
+    their in-plane wavevectors and the air reference's ordering are host
+    bookkeeping; a light whose orders include a grazing one (k_z exactly 0)
+    is evaluated again with k_parallel scaled by 1 - 1e-9, as Anallo does.
+    Returns one Anallo per light, as film.calc_modes(light) would."""
+    from .mode_calcs import Anallo
+    if not lights:
+        return []
+    films = [Anallo(film, light) for light in lights]
+    waves = [f.wavevectors() for f in films]
+    grazing = np.zeros(len(films), bool)
+    for _ in range(2):
+        n = np.array([f.n() for f in films], np.complex128)
+        wl = np.array([f.wl_norm() for f in films])
+        alpha = np.array([w[2] for w in waves])
+        beta = np.array([np.broadcast_to(w[3], w[2].shape) for w in waves])
+        order = np.array([f.air_ref().sort_order for f in films])
+        if alpha.ndim != 2 or order.shape != alpha.shape:
+            raise AssertionError("film and air reference disagree on the number of orders")
+        kz, Z = EMUstack.context().film_modes(n, wl, alpha, beta, order)
+        hit = np.any(kz == 0, axis=1) & ~grazing
+        if not hit.any():
+            break
+        for i in np.flatnonzero(hit):
+            waves[i] = films[i].wavevectors(grazing=True)
+        grazing |= hit
+    npw = alpha.shape[1]
+    for i, f in enumerate(films):
+        px, py, a, b = waves[i]
+        f.k_z = f.take_kz(kz[i, :npw], px, py, a, b, presorted=True)
+        f.k_z = np.concatenate((f.k_z, f.k_z))
+        f._device_Z = Z[i]
+        f.structure.num_pw_per_pol = npw
+    return films
 
 
 def calc_scat_many(stacks, pol='TE', want_nets=False):

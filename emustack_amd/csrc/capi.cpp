@@ -477,6 +477,14 @@ int emu_material_index(emu_ctx *ctx, int64_t nmat, const int64_t *kind, const in
   EMU_GUARD(emu::material_index(ctx->c, nmat, kind, offset, xs, ys, nq, material, wl_nm, n_out))
 }
 
+int emu_film_modes_batch(emu_ctx *ctx, int64_t nbatch, int64_t npw, const double *n, const double *wl_norm,
+                         const double *alpha, const double *beta, const int64_t *order, double *k_z, double *Z) {
+  if (!ctx || !n || !wl_norm || !alpha || !beta || !order || !k_z || !Z)
+    return fail(EMU_E_ARG, "emu_film_modes_batch: NULL argument");
+  if (nbatch < 1 || npw < 1) return fail(EMU_E_ARG, "emu_film_modes_batch: invalid size");
+  EMU_GUARD(emu::film_modes(ctx->c, nbatch, npw, n, wl_norm, alpha, beta, order, k_z, Z))
+}
+
 int emu_assemble(emu_ctx *ctx, emu_mesh *mesh, int64_t nbatch, int64_t nb_typ_el, const double *pp,
                  const double *qq, const double *shift, const double *bloch, double *mat1, double *mat2) {
   if (!ctx || !mesh || !pp || !qq || !shift || !bloch) return fail(EMU_E_ARG, "emu_assemble: NULL argument");

@@ -103,5 +103,7 @@ void assemble_noalloc(Ctx *c, DevMesh &dm, int nb, int ntyp, const double *pp, c
 // n(lambda) of nq (material, wavelength) queries (kernels_material.hip)
 void material_index(Ctx *c, int64_t nmat, const int64_t *kind, const int64_t *off, const double *xs,
                     const double *ys, int64_t nq, const int64_t *qm, const double *qw, double *out);
+void film_modes(Ctx *c, int64_t nb, int64_t npw, const double *n, const double *wl_norm, const double *alpha,
+                const double *beta, const int64_t *order, double *kz, double *Z);
 
 }  // namespace emu

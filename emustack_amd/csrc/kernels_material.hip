@@ -98,4 +98,75 @@ void material_index(Ctx *c, int64_t nmat, const int64_t *kind, const int64_t *of
   EMU_HIP(hipStreamSynchronize(s));
 }
 
+// Analytic plane-wave modes of homogeneous films (reference mode_calcs.py:
+// 110-186 Anallo.calc_modes / calc_kz, 195-212 Anallo.k / Anallo.Z) for nb (film, light) instances, one
+// thread per (instance, retained order): k = 2 pi n / wl_norm, k_z =
+// sqrt(k^2 - alpha^2 - beta^2) (principal branch) of order order[b][i],
+// stored at position i (the air reference's order) of both polarisation
+// blocks; Z = (1/n) k / k_z (TE block), (1/n) k_z / k (TM block).  An order
+// with k_z exactly 0 (grazing) gets Z = 0 in place of the infinite
+// impedance; the caller finds it in k_z, moves k_parallel and evaluates
+// again, as Anallo does.
+__global__ void k_film_modes(int64_t nb, int64_t npw, const zd *__restrict__ n, const double *__restrict__ wl_norm,
+                             const double *__restrict__ alpha, const double *__restrict__ beta,
+                             const int64_t *__restrict__ order, zd *__restrict__ kz, zd *__restrict__ Z) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nb * npw) return;
+  const int64_t b = q / npw, i = q % npw;
+  const int64_t o = order[q];
+  const zd nb_ = n[b];
+  const double w = wl_norm[b], tp = 2.0 * 3.141592653589793;
+  const zd k = zmk(tp * nb_.x / w, tp * nb_.y / w);
+  const double a = alpha[b * npw + o], be = beta[b * npw + o];
+  const zd k2 = zmul(k, k);
+  const zd kzv = zsqrt_principal(zmk(k2.x - a * a - be * be, k2.y));
+  zd *kb = kz + b * 2 * npw;
+  zd *Zb = Z + b * 2 * npw;
+  kb[i] = kzv;
+  kb[npw + i] = kzv;
+  if (kzv.x == 0.0 && kzv.y == 0.0) {
+    Zb[i] = Zb[npw + i] = zmk(0.0, 0.0);
+    return;
+  }
+  const zd inv_n = zdiv(zmk(1.0, 0.0), nb_);
+  const zd ratio = zdiv(k, kzv);
+  Zb[i] = zmul(inv_n, ratio);
+  Zb[npw + i] = zdiv(inv_n, ratio);
+}
+
+void film_modes(Ctx *c, int64_t nb, int64_t npw, const double *n, const double *wl_norm, const double *alpha,
+                const double *beta, const int64_t *order, double *kz, double *Z) {
+  EMU_HIP(hipSetDevice(c->device));
+  for (int64_t b = 0; b < nb; ++b) {
+    if (!(wl_norm[b] > 0.0)) throw std::invalid_argument("film_modes: wl_norm must be positive");
+    if (n[2 * b] == 0.0 && n[2 * b + 1] == 0.0) throw std::invalid_argument("film_modes: zero refractive index");
+    std::vector<char> seen(npw, 0);
+    for (int64_t i = 0; i < npw; ++i) {
+      const int64_t o = order[b * npw + i];
+      if (o < 0 || o >= npw || seen[o]) throw std::invalid_argument("film_modes: order must be a permutation");
+      seen[o] = 1;
+    }
+  }
+  hipStream_t s = c->stream;
+  const size_t nq = (size_t)(nb * npw);
+  zd *dn = c->buf<zd>("fm_n", (size_t)nb);
+  double *dw = c->buf<double>("fm_wl", (size_t)nb);
+  double *da = c->buf<double>("fm_alpha", nq);
+  double *dbeta = c->buf<double>("fm_beta", nq);
+  int64_t *dord = c->buf<int64_t>("fm_order", nq);
+  zd *dkz = c->buf<zd>("fm_kz", 2 * nq);
+  zd *dZ = c->buf<zd>("fm_Z", 2 * nq);
+  EMU_HIP(hipMemcpyAsync(dn, n, sizeof(zd) * nb, hipMemcpyHostToDevice, s));
+  EMU_HIP(hipMemcpyAsync(dw, wl_norm, sizeof(double) * nb, hipMemcpyHostToDevice, s));
+  EMU_HIP(hipMemcpyAsync(da, alpha, sizeof(double) * nq, hipMemcpyHostToDevice, s));
+  EMU_HIP(hipMemcpyAsync(dbeta, beta, sizeof(double) * nq, hipMemcpyHostToDevice, s));
+  EMU_HIP(hipMemcpyAsync(dord, order, sizeof(int64_t) * nq, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_film_modes, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, s, nb, npw, dn, dw, da, dbeta,
+                     dord, dkz, dZ);
+  EMU_LAUNCH_CHECK();
+  EMU_HIP(hipMemcpyAsync(kz, dkz, sizeof(zd) * 2 * nq, hipMemcpyDeviceToHost, s));
+  EMU_HIP(hipMemcpyAsync(Z, dZ, sizeof(zd) * 2 * nq, hipMemcpyDeviceToHost, s));
+  EMU_HIP(hipStreamSynchronize(s));
+}
+
 }  // namespace emu

@@ -174,60 +174,148 @@ def ellipse_cell_ntri(period, period_y, a_nm, b_nm, ntri, rotation_deg=0.0, seed
 
 # --------------------------------------------------------------- 1D gratings
 
-def grating_1d(period, diameters, lc_bkg, small_space=None, split_touching_incs=False, inclusion_edge='code'):
-    """Uniform P2 mesh of a 1D grating cell (objects.py:1054-1530 of the
-    reference, one or two inclusions): nel = round(1/lc_bkg) elements on
-    [0, 1]; nodes 2i-1 (left), 2i (mid), 2i+1 (right) of element i; the
-    material type of each element follows from its end points exactly as the
-    reference decides it (inclusion a centred, inclusion b at the cell edges
-    or touching a).  Returns x_arr (2 nel + 1,), table_nod (3, nel) 1-based,
-    type_el (nel,) 1-based and the reference's mesh name.
+def _dec_float_str(v):
+    """The reference's mesh-name number format: str(v) with '.' -> '_'."""
+    return str(v).replace('.', '_')
 
-    inclusion_edge: 'code' applies the reference's current test for the
-    centre inclusion (objects.py:1502-1503: an element whose left end lies up
-    to one element width left of 0.5 - r still counts as inclusion); 'exact'
-    takes only the elements inside [0.5 - r, 0.5 + r], the rule with which
-    the reference's stacked-gratings golden R/T files (tests/ref/case_4)
-    were generated."""
-    ds = [d for d in diameters if d > 0]
-    if not ds:
+
+def grating_layout_1d(period, diameters, edge_spacing=False, split_touching_incs=False, small_space=None):
+    """Material-type rules of a 1D grating cell with up to six inclusions,
+    as intervals of the unit cell (objects.py:1054-1506 of the reference).
+
+    Returns a list of (type, lo, hi, slack): an element [x1, x2] takes the
+    type of the first rule with lo - slack * dx <= x1, x2 <= hi (dx the
+    element width; lo = -inf / hi = +inf for one-sided rules); elements no
+    rule takes are background (type 1).  The number of inclusions N is the
+    index of the last positive diameter (earlier zero diameters stay in
+    place with zero width).  Layouts: inclusion 1 (type 2) is centred on
+    0.5; then
+      * N = 2: inclusion 2 (type 3) touching inclusion 1 on both sides
+        (split_touching_incs), or at the cell edges after a gap small_space
+        on the left and the rest of the cell on the right;
+      * N >= 3, edge_spacing: inclusions alternate left / right of the
+        centre with equal gaps i_d between them;
+      * N >= 3, split_touching_incs: inclusion k touches k - 1 on both sides;
+      * N >= 3 otherwise: inclusion k centred at 0.5 -/+ (k // 2) / N,
+        alternating left / right, for N = 4 and N = 6 the last inclusion
+        split across the cell edges.
+    The bounds are evaluated in the reference's order of operations, so that
+    an element end that meets a bound compares the same way.  What the
+    reference's rules do, including the irregular ones, is kept:
+      * N = 3 edge_spacing: i_d = (1 - d1 - d2 - d3) / 3 from the diameters
+        in nm (objects.py:1388), not the normalised widths;
+      * N = 4 edge_spacing: the fourth inclusion is not placed; elements past
+        the second gap on either side take types 4 and 3 (objects.py:1318-1323);
+      * N = 5 edge_spacing: the fifth inclusion takes type 4 (objects.py:1230-1232);
+      * N = 6 edge_spacing: a type-6 band of width r5 left of inclusion 4 and
+        inclusion 6 (type 7) on the right (objects.py:1118-1126);
+      * the slack dx widens the centre rule always and the touching rules
+        for N = 2 and N = 3 only.
+    """
+    d = [float(v) for v in diameters] + [0.0] * (6 - len(diameters))
+    if len(d) > 6:
+        raise ValueError("1D gratings take at most six inclusions (diameter1..diameter6)")
+    pos = [k for k in range(6) if d[k] > 0]
+    if not pos:
         raise ValueError("Must have at least one grating of nonzero width.")
-    if len(ds) > 2:
-        raise NotImplementedError("1D gratings with more than two inclusions")
+    N = pos[-1] + 1
+    r = [0.0] + [v / (2.0 * period) for v in d]  # r[k]: half-width of inclusion k, unit cell
+    inf = np.inf
+    rules = [(2, 0.5 - r[1], 0.5 + r[1], 1)]
+    if N == 1:
+        return rules
+    # touching layout: cumulative edges, accumulated term by term
+    if split_touching_incs and (N == 2 or not edge_spacing):
+        slack = 1 if N <= 3 else 0
+        right, left = 0.5 + r[1], 0.5 - r[1]
+        for k in range(2, N + 1):
+            rules.append((k + 1, right, right + r[k], slack))
+            rules.append((k + 1, left - r[k], left, slack))
+            right, left = right + r[k], left - r[k]
+        return rules
+    if N == 2:
+        if small_space is None:
+            small = large = 0.5 - r[1] - r[2]
+        else:
+            small = small_space
+            large = 1.0 - small_space - (2 * r[1]) - (2 * r[2])
+        rules.append((3, -inf, 0.5 - r[1] - small, 0))
+        rules.append((3, 0.5 + large + r[1], inf, 0))
+        return rules
+    if edge_spacing:
+        if N == 3:
+            g = (1.0 - d[0] - d[1] - d[2]) / 3.0
+            rules += [(3, 0.5 - g - 2.0 * r[2] - r[1], 0.5 - g - r[1], 0),
+                      (4, 0.5 + g + r[1], 0.5 + g + 2.0 * r[3] + r[1], 0)]
+            return rules
+        rem = 0.5
+        for k in range(1, N + 1):
+            rem = rem - r[k]
+        g = 2.0 * rem / float(N)
+        if N == 4:
+            rules += [(3, 0.5 - g - r[1] - 2.0 * r[2], 0.5 - g - r[1], 0),
+                      (4, 0.5 + g + r[1], 0.5 + g + r[1] + 2.0 * r[3], 0),
+                      (4, 0.5 + 2.0 * g + r[1] + 2.0 * r[3], inf, 0),
+                      (3, -inf, 0.5 - 2.0 * g - r[1] - 2.0 * r[2], 0)]
+            return rules
+        rules += [(3, 0.5 - g - r[1] - 2.0 * r[2], 0.5 - g - r[1], 0),
+                  (4, 0.5 + g + r[1], 0.5 + g + 2.0 * r[3] + r[1], 0),
+                  (5, 0.5 - 2.0 * g - 2.0 * r[2] - 2.0 * r[4] - r[1], 0.5 - 2.0 * g - 2.0 * r[2] - r[1], 0),
+                  (4 if N == 5 else 6, 0.5 + 2.0 * g + 2.0 * r[3] + r[1],
+                   0.5 + 2.0 * g + 2.0 * r[3] + 2.0 * r[5] + r[1], 0)]
+        if N == 6:
+            h = 0.5 - 3.0 * g - r[1] - 2.0 * r[2] - 2.0 * r[4]
+            right = 0.5 + 3.0 * g + r[1] + 2.0 * r[3] + 2.0 * r[5]
+            rules += [(6, h - r[5], h, 0), (7, right, right + 2.0 * r[6], 0)]
+        return rules
+    g = 1.0 / float(N)
+    centres = [None, None, 0.5 - g, 0.5 + g, 0.5 - 2.0 * g, 0.5 + 2.0 * g]
+    last = N if N in (3, 5) else N - 1  # N = 4, 6: the last inclusion sits at the cell edges
+    for k in range(2, last + 1):
+        rules.append((k + 1, centres[k] - r[k], centres[k] + r[k], 0))
+    if last < N:
+        rules += [(N + 1, -inf, r[N], 0), (N + 1, 1.0 - r[N], inf, 0)]
+    return rules
+
+
+def grating_1d(period, diameters, lc_bkg, small_space=None, split_touching_incs=False, inclusion_edge='code',
+               edge_spacing=False):
+    """Uniform P2 mesh of a 1D grating cell with up to six inclusions
+    (objects.py:1054-1530 of the reference): nel = round(1/lc_bkg) elements
+    on [0, 1]; nodes 2i-1 (left), 2i (mid), 2i+1 (right) of element i; the
+    material type of each element from grating_layout_1d's rules.  Returns
+    x_arr (2 nel + 1,), table_nod (3, nel) 1-based, type_el (nel,) 1-based
+    and the reference's mesh name.
+
+    inclusion_edge: 'code' applies the reference's current tests, whose
+    centre rule (objects.py:1502-1503) also takes an element whose left end
+    lies up to one element width left of 0.5 - r; 'exact' drops that slack
+    (only elements inside the inclusion's interval), the rule with which the
+    reference's stacked-gratings golden R/T files (tests/ref/case_4) were
+    generated."""
+    if inclusion_edge not in ('code', 'exact'):
+        raise ValueError("inclusion_edge must be 'code' or 'exact'")
+    rules = grating_layout_1d(period, diameters, edge_spacing, split_touching_incs, small_space)
     nel = int(np.round(1.0 / lc_bkg))
     delta_x = (1.0 - 0.0) / nel
     x = np.zeros(2 * nel + 1)
-    for i in range(nel):
-        x0 = 0.0 + i * delta_x
-        x[2 * i] = x0
-        x[2 * i + 1] = x0 + delta_x / 2.0
+    left = 0.0 + np.arange(nel) * delta_x
+    x[0:2 * nel:2] = left
+    x[1:2 * nel:2] = left + delta_x / 2.0
     x[2 * nel] = 0.0 + nel * delta_x
     table = np.empty((3, nel), dtype=np.int64, order='F')
     table[0] = 2 * np.arange(1, nel + 1) - 1
     table[1] = 2 * np.arange(1, nel + 1) + 1
     table[2] = 2 * np.arange(1, nel + 1)
-    r1 = ds[0] / (2.0 * period)
-    r2 = ds[1] / (2.0 * period) if len(ds) > 1 else 0.0
-    if len(ds) > 1:
-        small = 0.5 - r1 - r2 if small_space is None else small_space
-        large = 0.5 - r1 - r2 if small_space is None else 1.0 - small_space - 2 * r1 - 2 * r2
-    if inclusion_edge not in ('code', 'exact'):
-        raise ValueError("inclusion_edge must be 'code' or 'exact'")
+    x1, x2 = x[0:2 * nel:2], x[2::2]
+    dx = x2 - x1 if inclusion_edge == 'code' else np.zeros(nel)
     type_el = np.ones(nel, dtype=np.int64)
-    for i in range(nel):
-        x1, x2 = x[2 * i], x[2 * i + 2]
-        dx = x2 - x1 if inclusion_edge == 'code' else 0.0
-        if (x1 <= 0.5 + r1 and 0.5 - r1 - dx <= x1 and x2 <= 0.5 + r1 and 0.5 - r1 - dx <= x2):
-            type_el[i] = 2
-        elif len(ds) == 2 and split_touching_incs:
-            if ((x1 <= 0.5 + r1 + r2 and 0.5 + r1 - dx <= x1 and x2 <= 0.5 + r1 + r2 and 0.5 + r1 - dx <= x2)
-                    or (x1 <= 0.5 - r1 and 0.5 - r1 - r2 - dx <= x1 and x2 <= 0.5 - r1
-                        and 0.5 - r1 - r2 - dx <= x2)):
-                type_el[i] = 3
-        elif len(ds) == 2:
-            if (x1 <= 0.5 - r1 - small and x2 <= 0.5 - r1 - small) or (x1 >= 0.5 + large + r1
-                                                                        and x2 >= 0.5 + large + r1):
-                type_el[i] = 3
-    fmt = lambda v: ('%.6f' % v).rstrip('0').rstrip('.') if v != int(v) else str(int(v))
-    name = '1D_' + '_'.join(fmt(v) for v in [period] + ds)
+    free = np.ones(nel, dtype=bool)
+    for t, lo, hi, slack in rules:
+        lo_e = lo - dx if slack else lo
+        hit = free & (x1 <= hi) & (lo_e <= x1) & (x2 <= hi) & (lo_e <= x2)
+        type_el[hit] = t
+        free &= ~hit
+    n_inc = max(k + 1 for k in range(len(diameters)) if diameters[k] > 0)
+    name = ('1D_' if n_inc <= 2 else '') + '_'.join(_dec_float_str(v) for v in [period] + list(diameters[:n_inc]))
     return x, table, type_el, name

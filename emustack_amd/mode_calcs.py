@@ -159,22 +159,34 @@ class Anallo(Modes):
         alphas, betas (unsorted), sort_order, specular_order and
         num_prop_pw_per_pol.  An order with k_z exactly 0 (grazing) moves
         k_parallel by a factor 1 - 1e-9."""
+        px, py, alpha, beta = self.wavevectors()
+        kz = np.sqrt(self.k() ** 2 - alpha ** 2 - beta ** 2)
+        if np.count_nonzero(kz) < kz.size:
+            px, py, alpha, beta = self.wavevectors(grazing=True)
+            kz = np.sqrt(self.k() ** 2 - alpha ** 2 - beta ** 2)
+        return self.take_kz(kz, px, py, alpha, beta)
+
+    def wavevectors(self, grazing=False):
+        """Orders (px, py) and their in-plane wavevectors (alpha, beta) in
+        units of 1/period; grazing=True scales k_parallel by 1 - 1e-9 (the
+        reference's step off a Wood anomaly, with its warning)."""
         st = self.structure
         if st.world_1d is not True and st.world_1d is not False:
             raise ValueError("ThinFilm.world_1d must be set to True or False.")
-        ratio_y = st.period_y / st.period
         # the circle test in nm, as the reference evaluates it (the rounding
         # of boundary orders depends on the units)
         px, py = grating_orders(self.max_order_PWs, st.period, st.period_y, st.world_1d)
-        k2 = self.k() ** 2
         k_par = np.asarray(self.k_pll_norm(), float)
-        alpha, beta = order_wavevectors(k_par, px, py, ratio_y, st.world_1d)
-        kz = np.sqrt(k2 - alpha ** 2 - beta ** 2)
-        if np.count_nonzero(kz) < kz.size:
+        if grazing:
             print("Warning: [k_x, k_y] falls on a Wood anomaly of order (px, py); "
                   "k_parallel is scaled by (1 - 1e-9).")
-            alpha, beta = order_wavevectors((1 - 1e-9) * k_par, px, py, ratio_y, st.world_1d)
-            kz = np.sqrt(k2 - alpha ** 2 - beta ** 2)
+            k_par = (1 - 1e-9) * k_par
+        alpha, beta = order_wavevectors(k_par, px, py, st.period_y / st.period, st.world_1d)
+        return px, py, alpha, beta
+
+    def take_kz(self, kz, px, py, alpha, beta, presorted=False):
+        """Record the orders' k_z (unsorted, or already in the air reference's
+        order when presorted) with their wavevectors; returns k_z sorted."""
         self.alphas, self.betas = alpha, beta
         if self.is_air_ref:
             if hasattr(self, 'sort_order'):
@@ -188,7 +200,7 @@ class Anallo(Modes):
         zeroth = (px[order] == 0) & (py[order] == 0)
         self.specular_order = int(np.flatnonzero(zeroth)[0])
         self.num_prop_pw_per_pol = int(np.count_nonzero(kz.imag == 0))
-        return kz[order]
+        return kz if presorted else kz[order]
 
     def n(self):
         """Refractive index at the light's wavelength (real part only for a
@@ -202,7 +214,11 @@ class Anallo(Modes):
 
     def Z(self):
         """Wave impedances relative to free space, TE block (E in the
-        interface plane: Z = (1/n) k/k_z) then TM block (Z = (1/n) k_z/k)."""
+        interface plane: Z = (1/n) k/k_z) then TM block (Z = (1/n) k_z/k).
+        Films evaluated on the GPU (batch.calc_films_many) return the
+        impedances that launch computed."""
+        if getattr(self, '_device_Z', None) is not None:
+            return self._device_Z.copy()
         half = self.k_z.size // 2
         kz = self.k_z[:half]
         if not np.array_equal(kz, self.k_z[half:]):
@@ -256,8 +272,13 @@ class Simmo(Modes):
         st = self.structure
         if st.periodicity not in ('1D_array', '2D_array'):
             raise ValueError("NanoStruct.periodicity must be '1D_array' or '2D_array', not %r" % (st.periodicity,))
+        # background, then inclusions a..f by element type (the reference stops
+        # at inclusion_e, mode_calcs.py:273-276, which leaves the sixth
+        # inclusion of a 1D grating, type 7, without a material)
         materials = (st.background, st.inclusion_a, st.inclusion_b, st.inclusion_c, st.inclusion_d,
-                     st.inclusion_e)
+                     st.inclusion_e, st.inclusion_f)
+        if st.nb_typ_el > len(materials):
+            raise ValueError("nb_typ_el = %d, but a NanoStruct has %d materials" % (st.nb_typ_el, len(materials)))
         n_effs = np.array([m.n(self.light.wl_nm) for m in materials[:st.nb_typ_el]])
         if st.loss is False:
             n_effs = n_effs.real

@@ -131,7 +131,8 @@ class NanoStruct(object):
             from . import mesher
             ds = [self.diameter1] + [getattr(self, 'diameter%d' % i) for i in range(2, 7)]
             x, table_nod, type_el, name = mesher.grating_1d(self.period, ds, self.lc, self.small_space,
-                                                            self.split_touching_incs, self.inclusion_edge)
+                                                            self.split_touching_incs, self.inclusion_edge,
+                                                            self.edge_spacing)
             self.n_msh_el = type_el.size
             self.n_msh_pts = x.size
             self.table_nod = table_nod

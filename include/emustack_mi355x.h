@@ -251,6 +251,22 @@ int emu_material_index(emu_ctx *ctx, int64_t nmat, const int64_t *kind, const in
                        const double *xs, const double *ys, int64_t nq, const int64_t *material,
                        const double *wl_nm, double *n_out);
 
+/* Analytic plane-wave modes of homogeneous films for nbatch (film, light)
+ * instances in one launch (replaces Anallo.calc_modes and Anallo.Z of
+ * backend/mode_calcs.py:110-186, 195-212, evaluated per layer and
+ * wavelength in numpy).  Per instance b: n[2*b] complex index (real part
+ * only for a lossless film, as the caller decides), wl_norm[b] = lambda /
+ * period; alpha, beta[b*npw + o] in-plane wavevector of order o (units of
+ * 1/period); order[b*npw + i] (0-based) the order stored at position i (the
+ * free-space reference's propagation order).  Output per instance, complex
+ * interleaved, TE block then TM block (2*npw each): k_z[i] = k_z[npw + i] =
+ * sqrt(k^2 - alpha^2 - beta^2), k = 2 pi n / wl_norm, principal branch;
+ * Z = k / (n k_z) (TE), k_z / (n k) (TM).  An order with k_z exactly 0
+ * (grazing) gets Z = 0: the caller moves k_parallel (Anallo scales it by
+ * 1 - 1e-9) and calls again. */
+int emu_film_modes_batch(emu_ctx *ctx, int64_t nbatch, int64_t npw, const double *n, const double *wl_norm,
+                         const double *alpha, const double *beta, const int64_t *order, double *k_z, double *Z);
+
 /* Live kernel timing with HIP events on the context's stream (benchmarks).
  * enable = 1 resets the counters and starts recording every `every`-th
  * solve sweep; enable = 0 stops.  out[k*3 + {0,1,2}] (may be NULL) for kernel

@@ -42,17 +42,108 @@ from .fem2d import beta_from_mu, biorth_clusters, nr_indexx, pair_swap
 # ------------------------------------------------------------------ mesh
 
 
+def _inside(x1, x2, lo, hi):
+    return lo <= x1 <= hi and lo <= x2 <= hi
+
+
+def _element_type(x1, x2, dx, N, r, d, small_space, edge_spacing, split):
+    """Material type of the element [x1, x2] (objects.py:1054-1506): the
+    reference's if-chain for N inclusions, r[k] the normalised half-widths
+    (r[0] unused), d the diameters in nm."""
+    if x1 <= 0.5 + r[1] and 0.5 - r[1] - dx <= x1 and x2 <= 0.5 + r[1] and 0.5 - r[1] - dx <= x2:
+        return 2
+    if N == 1:
+        return 1
+    if N == 2 or (split and not edge_spacing):
+        if not split:  # objects.py:1472-1489
+            if small_space is None:
+                sm = ld = 0.5 - r[1] - r[2]
+            else:
+                sm, ld = small_space, 1.0 - small_space - (2 * r[1]) - (2 * r[2])
+            if x1 <= 0.5 - r[1] - sm and x2 <= 0.5 - r[1] - sm:
+                return 3
+            if x1 >= 0.5 + ld + r[1] and x2 >= 0.5 + ld + r[1]:
+                return 3
+            return 1
+        # touching inclusions (objects.py:1128-1166, 1234-1264, 1327-1352,
+        # 1408-1425, 1456-1471); the element-width slack only for N <= 3
+        sl = dx if N <= 3 else 0.0
+        a, b = 0.5 + r[1], 0.5 - r[1]
+        for k in range(2, N + 1):
+            if x1 <= a + r[k] and a - sl <= x1 and x2 <= a + r[k] and a - sl <= x2:
+                return k + 1
+            if x1 <= b and b - r[k] - sl <= x1 and x2 <= b and b - r[k] - sl <= x2:
+                return k + 1
+            a, b = a + r[k], b - r[k]
+        return 1
+    if edge_spacing:
+        if N == 3:  # objects.py:1387-1407: the gap from diameters in nm
+            i_d = (1.0 - d[1] - d[2] - d[3]) / 3.0
+            if _inside(x1, x2, 0.5 - i_d - 2.0 * r[2] - r[1], 0.5 - i_d - r[1]):
+                return 3
+            if _inside(x1, x2, 0.5 + i_d + r[1], 0.5 + i_d + 2.0 * r[3] + r[1]):
+                return 4
+            return 1
+        if N == 4:  # objects.py:1304-1326
+            i_d = 2.0 * (0.5 - r[1] - r[2] - r[3] - r[4]) / 4.0
+            if _inside(x1, x2, 0.5 - i_d - r[1] - 2.0 * r[2], 0.5 - i_d - r[1]):
+                return 3
+            if _inside(x1, x2, 0.5 + i_d + r[1], 0.5 + i_d + r[1] + 2.0 * r[3]):
+                return 4
+            if x1 >= 0.5 + 2.0 * i_d + r[1] + 2.0 * r[3] and x2 >= 0.5 + 2.0 * i_d + r[1] + 2.0 * r[3]:
+                return 4
+            if x1 <= 0.5 - 2.0 * i_d - r[1] - 2.0 * r[2] and x2 <= 0.5 - 2.0 * i_d - r[1] - 2.0 * r[2]:
+                return 3
+            return 1
+        # N = 5 (objects.py:1210-1233) and N = 6 (objects.py:1098-1127)
+        if N == 5:
+            i_d = 2.0 * (0.5 - r[1] - r[2] - r[3] - r[4] - r[5]) / 5.0
+        else:
+            i_d = 2.0 * (0.5 - r[1] - r[2] - r[3] - r[4] - r[5] - r[6]) / 6.0
+        if _inside(x1, x2, 0.5 - i_d - r[1] - 2.0 * r[2], 0.5 - i_d - r[1]):
+            return 3
+        if _inside(x1, x2, 0.5 + i_d + r[1], 0.5 + i_d + 2.0 * r[3] + r[1]):
+            return 4
+        if _inside(x1, x2, 0.5 - 2.0 * i_d - 2.0 * r[2] - 2.0 * r[4] - r[1], 0.5 - 2.0 * i_d - 2.0 * r[2] - r[1]):
+            return 5
+        if _inside(x1, x2, 0.5 + 2.0 * i_d + 2.0 * r[3] + r[1], 0.5 + 2.0 * i_d + 2.0 * r[3] + 2.0 * r[5] + r[1]):
+            return 4 if N == 5 else 6
+        if N == 6:
+            top = 0.5 - 3.0 * i_d - r[1] - 2.0 * r[2] - 2.0 * r[4]
+            if _inside(x1, x2, top - r[5], top):
+                return 6
+            bot = 0.5 + 3.0 * i_d + r[1] + 2.0 * r[3] + 2.0 * r[5]
+            if _inside(x1, x2, bot, bot + 2.0 * r[6]):
+                return 7
+        return 1
+    # evenly spaced centres 0.5 -/+ i_d, 0.5 -/+ 2 i_d (objects.py:1167-1193,
+    # 1265-1285, 1353-1372, 1426-1444)
+    i_d = 1.0 / N
+    c = {2: 0.5 - i_d, 3: 0.5 + i_d, 4: 0.5 - 2.0 * i_d, 5: 0.5 + 2.0 * i_d}
+    for k in range(2, N + 1):
+        if k == N and N in (4, 6):  # the last inclusion across the cell edges
+            if (x1 <= r[k] and x2 <= r[k]) or (x1 >= 1.0 - r[k] and x2 >= 1.0 - r[k]):
+                return k + 1
+        elif _inside(x1, x2, c[k] - r[k], c[k] + r[k]):
+            return k + 1
+    return 1
+
+
 def grating_mesh_1d(period, diameters, lc_bkg, small_space=None, split_touching_incs=False,
-                    inclusion_edge='code'):
+                    inclusion_edge='code', edge_spacing=False):
     """Uniform mesh of the unit cell [0, 1] with nel = round(1/lc_bkg) P2
-    elements and the element material types of objects.py:1054-1530 for one
-    or two inclusions.  Returns x (npt,), table_nod (nel, 3) 1-based
-    [left, right, mid], type_el (nel,) 1-based."""
-    diameters = [d for d in diameters if d > 0]
-    if not diameters:
+    elements and the element material types of objects.py:1054-1506 for up
+    to six inclusions (N = index of the last positive diameter).  Returns x
+    (npt,), table_nod (nel, 3) 1-based [left, right, mid], type_el (nel,)
+    1-based.  inclusion_edge 'exact' drops the element-width slack of the
+    reference's rules (the rule of its tests/ref/case_4 golden files)."""
+    d = [0.0] + [float(v) for v in diameters] + [0.0] * (6 - len(diameters))
+    pos = [k for k in range(1, 7) if d[k] > 0]
+    if not pos:
         raise ValueError('Must have at least one grating of nonzero width.')
-    if len(diameters) > 2:
-        raise NotImplementedError('1D gratings with more than two inclusions')
+    N = pos[-1]
+    if inclusion_edge not in ('code', 'exact'):
+        raise ValueError(inclusion_edge)
     nel = int(np.round(1.0 / lc_bkg))
     npt = 2 * nel + 1
     delta_x = (1.0 - 0.0) / nel
@@ -63,37 +154,13 @@ def grating_mesh_1d(period, diameters, lc_bkg, small_space=None, split_touching_
         ls_x[2 * i_el] = x + delta_x / 2.0
     ls_x[2 * nel + 1] = 0.0 + nel * delta_x
     table = np.array([[2 * i - 1, 2 * i + 1, 2 * i] for i in range(1, nel + 1)], dtype=np.int64)
+    r = [0.0] + [d[k] / (2.0 * period) for k in range(1, 7)]
     type_el = np.ones(nel, dtype=np.int64)
-    r1 = diameters[0] / (2.0 * period)
-    r2 = diameters[1] / (2.0 * period) if len(diameters) > 1 else 0.0
-    if len(diameters) > 1 and small_space is None:
-        small = large_d = 0.5 - r1 - r2
-    elif len(diameters) > 1:
-        small = small_space
-        large_d = 1.0 - small_space - 2 * r1 - 2 * r2
-    if inclusion_edge not in ('code', 'exact'):
-        raise ValueError(inclusion_edge)
     for e in range(nel):
         x1 = ls_x[2 * (e + 1) - 1]
         x2 = ls_x[2 * (e + 1) + 1]
         dx = x2 - x1 if inclusion_edge == 'code' else 0.0
-        centre = (x1 <= 0.5 + r1 and 0.5 - r1 - dx <= x1 and x2 <= 0.5 + r1 and 0.5 - r1 - dx <= x2)
-        if len(diameters) == 1:
-            type_el[e] = 2 if centre else 1
-        elif split_touching_incs:
-            if centre:
-                type_el[e] = 2
-            elif (x1 <= 0.5 + r1 + r2 and 0.5 + r1 - dx <= x1 and x2 <= 0.5 + r1 + r2 and 0.5 + r1 - dx <= x2):
-                type_el[e] = 3
-            elif (x1 <= 0.5 - r1 and 0.5 - r1 - r2 - dx <= x1 and x2 <= 0.5 - r1 and 0.5 - r1 - r2 - dx <= x2):
-                type_el[e] = 3
-        else:
-            if centre:
-                type_el[e] = 2
-            elif x1 <= 0.5 - r1 - small and x2 <= 0.5 - r1 - small:
-                type_el[e] = 3
-            elif x1 >= 0.5 + large_d + r1 and x2 >= 0.5 + large_d + r1:
-                type_el[e] = 3
+        type_el[e] = _element_type(x1, x2, dx, N, r, d, small_space, edge_spacing, split_touching_incs)
     return ls_x[1:], table, type_el
 
 

@@ -6,7 +6,10 @@ relative; J J^dagger (mode-subspace projector, invariant to the per-mode
 normalisation) 1e-7 of its largest entry; R/T/A 1e-8 against the oracle; the
 2D embedding (world_1d = 0) equal to the oracle's; and the reference's seven
 case-4 golden R/T/A files at 1e-8 through the drop-in objects, with the
-element rule those files were generated with (inclusion_edge='exact')."""
+element rule those files were generated with (inclusion_edge='exact').
+Multi-inclusion cells (four evenly spaced inclusions; five touching ones)
+take the same beta and projector checks against the oracle on the same
+element types, and once through the drop-in NanoStruct."""
 import numpy as np
 import pytest
 
@@ -24,8 +27,8 @@ def _kp(wl=WL):
     return k * np.sin(th) * np.array([np.cos(ph), np.sin(ph)]) * PERIOD
 
 
-def _case(diam, neff, wl=WL):
-    x, tn, te = fem1d.grating_mesh_1d(PERIOD, diam, 0.005)
+def _case(diam, neff, wl=WL, **layout):
+    x, tn, te = fem1d.grating_mesh_1d(PERIOD, diam, 0.005, **layout)
     wln = wl / PERIOD
     k0 = 2 * np.pi / wln
     kp = _kp(wl)
@@ -34,12 +37,17 @@ def _case(diam, neff, wl=WL):
     return x, tn, te, wln, kp, neff, shift
 
 
+MESHES = (('g1', [190, 190], [3.61, 1.46, 1.46], {}), ('g2', [190], [3.61, 1.46], {}),
+          ('g4', [152.0, 76.0, 114.0, 60.8], [2.0, 1.46, 3.0, 1.0, 2.5], {}),
+          ('g5', [120.0, 60.0, 50.0, 40.0, 30.0], [3.61, 1.46, 2.2, 1.0, 2.8, 1.7], dict(split_touching_incs=True)))
+
+
 @pytest.fixture(scope='module')
 def gratings(gpu_ctx):
     from emustack_amd import _lib
     out = {}
-    for name, diam, neff in (('g1', [190, 190], [3.61, 1.46, 1.46]), ('g2', [190], [3.61, 1.46])):
-        x, tn, te, wln, kp, neff, shift = _case(diam, neff)
+    for name, diam, neff, layout in MESHES:
+        x, tn, te, wln, kp, neff, shift = _case(diam, neff, **layout)
         mh = _lib.MeshHandle1D(x, tn, te, gpu_ctx)
         g = gpu_ctx.calc_modes_1d_batch(mh, [wln], 46, ORDER, [neff], [kp], [shift], 2 * ORDER + 1)
         m = fem1d.build_mesh_1d(x, tn, te)
@@ -142,3 +150,22 @@ def test_dropin_api_case4(gratings):
     r = stack_ref.calc_scat([air] + lays + [air], WL / PERIOD, light.k_pll * PERIOD, ORDER, pol='TE', world_1d=True)
     np.testing.assert_allclose([float(st.r_list[-1]), float(st.t_list[-1]), float(st.a_list[-1])],
                                [r['r_list'][-1], r['t_list'][-1], r['a_list'][-1]], rtol=0, atol=1e-8)
+
+
+def test_dropin_multi_inclusion(gratings):
+    """NanoStruct('1D_array') with four inclusions (diameter1..4, materials
+    inclusion_a..d, nb_typ_el = 5) through Simmo.calc_modes: the drop-in's
+    mesh and solve against the oracle on the oracle's element types."""
+    from emustack_amd import objects, materials
+    diam = [152.0, 76.0, 114.0, 60.8]
+    neff = [2.0, 1.46, 3.0, 1.0, 2.5]
+    mats = [materials.Material(complex(n)) for n in neff]
+    g = objects.NanoStruct('1D_array', PERIOD, diameter1=diam[0], diameter2=diam[1], diameter3=diam[2],
+                           diameter4=diam[3], background=mats[0], inclusion_a=mats[1], inclusion_b=mats[2],
+                           inclusion_c=mats[3], inclusion_d=mats[4], nb_typ_el=5, height_nm=200, loss=True,
+                           lc_bkg=0.005)
+    np.testing.assert_array_equal(g.type_el, gratings['g4'][2][2])
+    light = objects.Light(WL, max_order_PWs=ORDER, theta=0.0, phi=0.0)
+    sim = g.calc_modes(light, num_BMs=46)
+    bo = gratings['g4'][1][0]
+    np.testing.assert_allclose(np.asarray(sim.k_z), bo, rtol=1e-9, atol=1e-9 * np.abs(bo).max())
