@@ -266,13 +266,15 @@ void lu_factor(DevMesh &dm, int nb, const zd *mat1, zd *pool, int32_t *piv, int3
                int32_t *gx, zd *inv, hipStream_t s);
 void lu_invert(DevMesh &dm, int nb, const zd *pool, zd *inv, hipStream_t s);
 // x = A^{-1} b, both in the new (nested-dissection) numbering, [nb][nrhs][n];
-// y (same shape) and w ([nb][nrhs][solve_ws(dm)]) are work vectors
+// y (same shape) and w ([nb][nrhs][solve_ws(dm)]) are work vectors.  sx:
+// instance stride of x (default nrhs n), so that x can be a column slot of a
+// larger per-instance array (the Krylov basis)
 int64_t solve_ws(const DevMesh &dm);
 void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *gx, const zd *b, zd *y, zd *x,
-              zd *w, hipStream_t s, Profiler *prof = nullptr);
+              zd *w, hipStream_t s, Profiler *prof = nullptr, int64_t sx = -1);
 // the same with the complex64 copy of the inverted factor (lu_narrow)
 void lu_solve(DevMesh &dm, int nb, int nrhs, const float2 *inv32, const int32_t *gx, const zd *b, zd *y, zd *x,
-              zd *w, hipStream_t s, Profiler *prof = nullptr);
+              zd *w, hipStream_t s, Profiler *prof = nullptr, int64_t sx = -1);
 void lu_narrow(const DevMesh &dm, int nb, const zd *inv, float2 *inv32, hipStream_t s);
 void launch_swap_f2(int64_t n, float2 *a, float2 *b, hipStream_t s);
 // permute vectors old <-> new numbering: dst[new] = src[perm[new]] or inverse
@@ -282,9 +284,11 @@ void launch_permute_vec(DevMesh &dm, int nb, int ncol, const zd *src, zd *dst, b
 // ---------------- dense / vector kernels (kernels_krylov.hip) ----------------
 // y[b][r] = sum_k A[b][r][k] x[b][col k] for CSR (n x n), ncol columns
 // M times one 8-column block X (column-major, ld n) through the
-// row-interleaved scratch xi (nb x 8 n): y column-major (ld n)
+// row-interleaved scratch xi (nb x 8 n): y column-major (ld n).  xi_ready:
+// xi already holds X's interleaved copy (written by launch_trmm_right)
 void launch_spmm8_il(int nb, int64_t n, const int64_t *rp, const int32_t *ci, const zd *vals, int64_t vstride,
-                     const zd *x, int64_t xstride, zd *xi, zd *y, int64_t ystride, hipStream_t s, KTime kt = {});
+                     const zd *x, int64_t xstride, zd *xi, zd *y, int64_t ystride, hipStream_t s, KTime kt = {},
+                     bool xi_ready = false);
 // xi (optional, nb x n x ncol): scratch for the row-major copy that the
 // many-column form (ncol > 8) works from
 void launch_spmm(int nb, int64_t n, int ncol, const int64_t *rp, const int32_t *ci, const zd *vals,
@@ -328,9 +332,11 @@ void launch_dcgs_coef(int nb, bool fin, int j, int c0, const zd *P, int64_t ldp,
 // Cholesky-QR of a bq-column block: G = R^H R, Rinv = R^-1, Racc = R * Rprev
 void launch_cholqr(int nb, int bq, const zd *G, zd *Rinv, const zd *Rprev, zd *Racc, int32_t *bad,
                    hipStream_t s);
-// O = W * Rinv (bq x bq upper triangular), in place allowed
+// O = W * Rinv (bq x bq upper triangular), in place allowed; for bq = 8 and
+// xi given, O is also written row-interleaved to xi (nb x 8 n, the layout
+// launch_spmm8_il reads)
 void launch_trmm_right(int nb, int64_t n, int bq, const zd *W, int64_t ldw, int64_t sW, const zd *Ri, zd *O,
-                       int64_t ldo, int64_t sO, hipStream_t s);
+                       int64_t ldo, int64_t sO, hipStream_t s, zd *xi = nullptr);
 void launch_block_copy(int nb, int rows, int cols, const zd *S, int64_t lds, int64_t sS, zd *D, int64_t ldd,
                        int64_t sD, bool add, hipStream_t s);
 void launch_fill(zd *p, int64_t n, zd v, hipStream_t s);

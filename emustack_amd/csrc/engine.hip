@@ -683,16 +683,17 @@ struct StageTimer {
 };
 
 // orthonormalise V[:, j:j+bq] in place (Cholesky-QR2); R (bq x bq) of the
-// factorisation W = Q R is written to Rout[b] (device, bq x bq) when given
+// factorisation W = Q R is written to Rout[b] (device, bq x bq) when given;
+// xi (8-column blocks): also write the row-interleaved copy of the result
 static void block_cholqr2(Ctx *c, int nb, int64_t n, int bq, zd *W, int64_t sV, zd *G, zd *Ri, zd *R1, zd *Rout,
-                          int32_t *bad, zd *atbs, size_t atb_elems, hipStream_t s) {
+                          int32_t *bad, zd *atbs, size_t atb_elems, hipStream_t s, zd *xi = nullptr) {
   const int64_t sG = (int64_t)bq * bq;
   launch_atb(nb, n, bq, bq, W, n, sV, W, n, sV, G, bq, sG, true, atbs, atb_elems, s);
   launch_cholqr(nb, bq, G, Ri, nullptr, R1, bad, s);
   launch_trmm_right(nb, n, bq, W, n, sV, Ri, W, n, sV, s);
   launch_atb(nb, n, bq, bq, W, n, sV, W, n, sV, G, bq, sG, true, atbs, atb_elems, s);
   launch_cholqr(nb, bq, G, Ri, R1, Rout, bad, s);
-  launch_trmm_right(nb, n, bq, W, n, sV, Ri, W, n, sV, s);
+  launch_trmm_right(nb, n, bq, W, n, sV, Ri, W, n, sV, s, xi);
 }
 
 // First-come admission of host Schur steps to the cores: at most one running
@@ -745,7 +746,6 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   zd *V2 = c->buf<zd>("ks_V2", (size_t)nb * sV);
   zd *t1 = c->buf<zd>("ks_t1", (size_t)nb * bq * n);
   zd *ty = c->buf<zd>("ks_ty", (size_t)nb * bq * n);
-  zd *wv = c->buf<zd>("ks_wv", (size_t)nb * bq * n);
   zd *xil = bq == 8 ? c->buf<zd>("ks_xil", (size_t)nb * 8 * n) : nullptr;  // interleaved SpMM input
   zd *tw = c->buf<zd>("ks_w", (size_t)nb * bq * solve_ws(dm));
   zd *Hd = c->buf<zd>("ks_H", (size_t)nb * sH);
@@ -824,6 +824,10 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   for (int it = 0; it <= itermax; ++it) {
     if (!turn.owns_lock()) turn.lock();
     const double nbd = nact;
+    // xil holds the interleaved copy of the block to expand once the
+    // previous block's Cholesky-QR has written it (not for the first block
+    // of an expansion: the restart or the start vectors wrote that one)
+    bool xil_ready = false;
     for (int j = k; j < m; j += bq) {
       zd *Xj = V + (int64_t)j * n;
       zd *Wn = V + (int64_t)(j + bq) * n;  // next block slot
@@ -831,15 +835,15 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
       const KTime kspmv = pf ? pf->time(4, nbd * (20.0 * nzd + 8.0 * (nd + 1) + 32.0 * nd * bq)) : KTime{};
       if (bq == 8)
         launch_spmm8_il(nact, n, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, xil, t1, (int64_t)bq * n,
-                        s, kspmv);
+                        s, kspmv, xil_ready);
       else
         launch_spmm(nact, n, bq, dm.prow_ptr.p, dm.pcol_ind.p, mvals, dm.host.nnz, Xj, sV, t1, (int64_t)bq * n, s,
                     kspmv);
+      // the solve writes the next block slot of the basis directly
       if (inv32)
-        lu_solve(dm, nact, bq, inv32, gx, t1, ty, wv, tw, s, pf);
+        lu_solve(dm, nact, bq, inv32, gx, t1, ty, Wn, tw, s, pf, sV);
       else
-        lu_solve(dm, nact, bq, inv, gx, t1, ty, wv, tw, s, pf);
-      launch_copy(nact, bq * n, wv, (int64_t)bq * n, Wn, sV, s);
+        lu_solve(dm, nact, bq, inv, gx, t1, ty, Wn, tw, s, pf, sV);
       if (delayed) {
         // Xj is the provisional block Z; one pass over V[:, :j] for Z's
         // reorthogonalisation and W's projection, then [Q_j, Zraw] in place
@@ -852,7 +856,8 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
         launch_dcgs_coef(nact, false, j, c0, Pd, ldp, sP, A0d, ldp, sA0, Hd, ld, sH, Ed, sP, Ld, bad, s);
         launch_combine(nact, n, j, 2 * bq, V, n, sV, Ed, sP, Ld, Xj, n, sV, s,
                        pf ? pf->time(3, nbd * 16.0 * nd * (j + 4 * bq)) : KTime{});
-        block_cholqr2(c, nact, n, bq, Wn, sV, G, Ri, R1, Rf, bad, atbs, atb_elems, s);
+        block_cholqr2(c, nact, n, bq, Wn, sV, G, Ri, R1, Rf, bad, atbs, atb_elems, s, bq == 8 ? xil : nullptr);
+        xil_ready = bq == 8;
         launch_block_copy(nact, bq, bq, Rf, bq, (int64_t)bq * bq, Hd + (int64_t)j * ld + j + bq, ld, sH, false, s);
         res.ops += (int64_t)nact * bq;
         res.steps += 1;

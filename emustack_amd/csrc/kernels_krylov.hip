@@ -249,10 +249,13 @@ __global__ __launch_bounds__(256) void k_spmm_il(int64_t n, const int64_t *__res
 }
 
 void launch_spmm8_il(int nb, int64_t n, const int64_t *rp, const int32_t *ci, const zd *vals, int64_t vstride,
-                     const zd *x, int64_t xstride, zd *xi, zd *y, int64_t ystride, hipStream_t s, KTime kt) {
-  hipLaunchKernelGGL(k_interleave8, dim3((unsigned)((n + 255) / 256), nb), dim3(256), 0, s, n, x, xstride, xi,
-                     8 * n);
-  EMU_LAUNCH_CHECK();
+                     const zd *x, int64_t xstride, zd *xi, zd *y, int64_t ystride, hipStream_t s, KTime kt,
+                     bool xi_ready) {
+  if (!xi_ready) {
+    hipLaunchKernelGGL(k_interleave8, dim3((unsigned)((n + 255) / 256), nb), dim3(256), 0, s, n, x, xstride, xi,
+                       8 * n);
+    EMU_LAUNCH_CHECK();
+  }
   const int per = (int)(((n + 31) / 32 + 7) / 8);
   hipExtLaunchKernelGGL(k_spmm_il, dim3((unsigned)(8 * per), nb), dim3(256), 0, s, kt.a, kt.b, 0, n, rp, ci, vals,
                         vstride, xi, 8 * n, y, ystride, per);
@@ -1412,11 +1415,13 @@ void launch_cholqr(int nb, int bq, const zd *G, zd *Rinv, const zd *Rprev, zd *R
   EMU_LAUNCH_CHECK();
 }
 
-// Wout[b][:, 0:bq] = W[b][:, 0:bq] * Rinv[b]  (bq x bq upper triangular)
+// Wout[b][:, 0:bq] = W[b][:, 0:bq] * Rinv[b]  (bq x bq upper triangular);
+// with xi, the row-interleaved copy of Wout as well (xi[b][8 r + j], the
+// operand of the next Krylov block's SpMM, saving k_interleave8's pass)
 template <int BQ>
 __global__ __launch_bounds__(256) void k_trmm_right(int64_t n, const zd *W, int64_t ldw, int64_t sW,
                                                     const zd *__restrict__ Ri, zd *O, int64_t ldo,
-                                                    int64_t sO) {
+                                                    int64_t sO, zd *__restrict__ xi) {
   __shared__ zd rs[BQ][BQ];
   const int b = blockIdx.y;
   for (int idx = threadIdx.x; idx < BQ * BQ; idx += blockDim.x) rs[idx % BQ][idx / BQ] = Ri[(int64_t)b * BQ * BQ + idx];
@@ -1426,23 +1431,31 @@ __global__ __launch_bounds__(256) void k_trmm_right(int64_t n, const zd *W, int6
   zd w[BQ];
 #pragma unroll
   for (int q = 0; q < BQ; ++q) w[q] = W[b * sW + q * ldw + r];
+  zd o[BQ];
 #pragma unroll
   for (int j = 0; j < BQ; ++j) {
     zd acc = zmk(0, 0);
 #pragma unroll
     for (int k = 0; k <= j; ++k) acc = zfma(w[k], rs[k][j], acc);
+    o[j] = acc;
     O[b * sO + j * ldo + r] = acc;
+  }
+  if (xi) {
+    zd *xr = xi + (int64_t)b * BQ * n + (int64_t)BQ * r;
+#pragma unroll
+    for (int j = 0; j < BQ; ++j) xr[j] = o[j];
   }
 }
 
 void launch_trmm_right(int nb, int64_t n, int bq, const zd *W, int64_t ldw, int64_t sW, const zd *Ri, zd *O,
-                       int64_t ldo, int64_t sO, hipStream_t s) {
+                       int64_t ldo, int64_t sO, hipStream_t s, zd *xi) {
   dim3 g((unsigned)((n + 255) / 256), nb);
+  if (xi && bq != 8) throw HipError("trmm_right: the interleaved copy is for 8-column blocks");
   switch (bq) {
-    case 1: hipLaunchKernelGGL(k_trmm_right<1>, g, dim3(256), 0, s, n, W, ldw, sW, Ri, O, ldo, sO); break;
-    case 2: hipLaunchKernelGGL(k_trmm_right<2>, g, dim3(256), 0, s, n, W, ldw, sW, Ri, O, ldo, sO); break;
-    case 4: hipLaunchKernelGGL(k_trmm_right<4>, g, dim3(256), 0, s, n, W, ldw, sW, Ri, O, ldo, sO); break;
-    case 8: hipLaunchKernelGGL(k_trmm_right<8>, g, dim3(256), 0, s, n, W, ldw, sW, Ri, O, ldo, sO); break;
+    case 1: hipLaunchKernelGGL(k_trmm_right<1>, g, dim3(256), 0, s, n, W, ldw, sW, Ri, O, ldo, sO, nullptr); break;
+    case 2: hipLaunchKernelGGL(k_trmm_right<2>, g, dim3(256), 0, s, n, W, ldw, sW, Ri, O, ldo, sO, nullptr); break;
+    case 4: hipLaunchKernelGGL(k_trmm_right<4>, g, dim3(256), 0, s, n, W, ldw, sW, Ri, O, ldo, sO, nullptr); break;
+    case 8: hipLaunchKernelGGL(k_trmm_right<8>, g, dim3(256), 0, s, n, W, ldw, sW, Ri, O, ldo, sO, xi); break;
     default: throw HipError("trmm_right: block size must be 1, 2, 4 or 8");
   }
   EMU_LAUNCH_CHECK();

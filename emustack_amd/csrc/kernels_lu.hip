@@ -834,8 +834,8 @@ __global__ __launch_bounds__(256) void k_fwd_red(const int32_t *__restrict__ tas
 template <int RC, typename FT>
 __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks, const int32_t *__restrict__ uidx,
                                              const FT *__restrict__ inv, int64_t isz, int nrhs, int64_t n,
-                                             const zd *__restrict__ y, zd *__restrict__ x, zd *__restrict__ part,
-                                             int nslot, int span) {
+                                             const zd *__restrict__ y, zd *__restrict__ x, int64_t sx,
+                                             zd *__restrict__ part, int nslot, int span) {
   const SolveTask T = tasks[blockIdx.x];
   const int b = blockIdx.y;
   const int q0 = blockIdx.z * RC;
@@ -848,6 +848,10 @@ __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks
   const int32_t *ui = uidx + T.uoff;
   const int nq = min(RC, nrhs - q0);
   const int64_t col0 = (int64_t)b * nrhs + q0;
+  // this instance's input y (nrhs columns of n) and output x (instance
+  // stride sx: the solve may write straight into a slot of a larger array)
+  const zd *yb = y + (int64_t)b * nrhs * n;
+  zd *xb = x + (int64_t)b * sx;
   const int row = T.r0 + 64 * tile;  // first row of this wave's tile
   const int rowl = row + lane;
   const int rowc = min(rowl, s - 1);
@@ -882,11 +886,11 @@ __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks
     zd a[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) a[u] = FactorLd<FT>::ld(Fcol, rowb, (uint32_t)(min(cbase + u * ks, clast) * s) * EB);
-    const zd *src = kkc < s ? y : x;
+    const zd *src = kkc < s ? yb : xb;
     constexpr int NG = (RC + 3) / 4;
     zd gv[NG];
 #pragma unroll
-    for (int u = 0; u < NG; ++u) gv[u] = src[(col0 + min(min(qg + 4 * u, RC - 1), nq - 1)) * n + off];
+    for (int u = 0; u < NG; ++u) gv[u] = src[(q0 + min(min(qg + 4 * u, RC - 1), nq - 1)) * n + off];
 #pragma unroll
     for (int u = 0; u < NG; ++u) {
       const int qq = qg + 4 * u;
@@ -939,13 +943,13 @@ __global__ __launch_bounds__(256) void k_bwd(const SolveTask *__restrict__ tasks
   if (rowl < s)
 #pragma unroll
     for (int q = 0; q < RC; ++q)
-      if (q < nq) x[(col0 + q) * n + T.fb + rowl] = acc[q];
+      if (q < nq) xb[(int64_t)(q0 + q) * n + T.fb + rowl] = acc[q];
 }
 
 template <int RC>
 __global__ __launch_bounds__(256) void k_bwd_red(const int32_t *__restrict__ tasks, FrontArrays A, int nrhs,
                                                  int64_t n, const zd *__restrict__ part, int nslot,
-                                                 zd *__restrict__ x) {
+                                                 zd *__restrict__ x, int64_t sx) {
   const int32_t *tk = tasks + 4 * blockIdx.x;
   const int t = tk[0], r0 = tk[1], p0 = tk[2], np = tk[3];
   const int b = blockIdx.y, q0 = blockIdx.z * RC;
@@ -969,7 +973,7 @@ __global__ __launch_bounds__(256) void k_bwd_red(const int32_t *__restrict__ tas
       for (int i = 0; i < 8; ++i)
         if (c + i < np) v = zadd(v, h[i]);
     }
-    x[(col0 + q) * n + fb + row] = v;
+    x[(int64_t)b * sx + (int64_t)(q0 + q) * n + fb + row] = v;
   }
 }
 
@@ -1000,15 +1004,15 @@ static void launch_fwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, 
 
 template <int RC, typename FT>
 static void launch_bwd(const SolveTasks &T, DevMesh &dm, FrontArrays A, int nb, int nrhs, const FT *inv,
-                       const zd *y, zd *x, zd *part, hipStream_t s, KTime kt) {
+                       const zd *y, zd *x, int64_t sx, zd *part, hipStream_t s, KTime kt) {
   const Symbolic &sy = dm.host.sym;
   const unsigned nz = (unsigned)((nrhs + RC - 1) / RC);
   hipExtLaunchKernelGGL(k_bwd<RC, FT>, dim3((unsigned)T.cnt, nb, nz), dim3(256), 0, s, kt.a, kt.b, 0,
                         reinterpret_cast<const SolveTask *>(dm.tasks.p + T.off), dm.uidx.p, inv, dm.isz, nrhs,
-                        sy.n, y, x, part, (int)T.nslot, (int)std::min<int64_t>(T.span, INT32_MAX));
+                        sy.n, y, x, sx, part, (int)T.nslot, (int)std::min<int64_t>(T.span, INT32_MAX));
   if (T.rcnt > 0)
     hipLaunchKernelGGL(k_bwd_red<RC>, dim3((unsigned)T.rcnt, nb, nz), dim3(256), 0, s, dm.tasks.p + T.roff, A, nrhs,
-                       sy.n, part, (int)T.nslot, x);
+                       sy.n, part, (int)T.nslot, x, sx);
 }
 
 int64_t solve_ws(const DevMesh &dm) { return std::max<int64_t>(dm.host.sym.nw, 1) + dm.part_rhs; }
@@ -1022,8 +1026,9 @@ static const SolveTasks &pick_tasks(const SolveTasks *v, int nrhs) { return nrhs
 
 template <typename FT>
 static void lu_solve_t(DevMesh &dm, int nb, int nrhs, const FT *inv, const int32_t *gx, const zd *bnew, zd *y,
-                       zd *x, zd *w, hipStream_t s, Profiler *prof) {
+                       zd *x, zd *w, hipStream_t s, Profiler *prof, int64_t sx) {
   FrontArrays A = fa(dm);
+  if (sx < 0) sx = (int64_t)nrhs * dm.host.sym.n;
   const double vecs = (double)nb * nrhs;
   const double fscale = FactorLd<FT>::kBytes / 16.0;  // factor bytes per entry relative to complex128
   // partial sums of split tasks live behind the update vectors in w
@@ -1045,23 +1050,23 @@ static void lu_solve_t(DevMesh &dm, int nb, int nrhs, const FT *inv, const int32
     const int rc = pick_rc(nrhs, T.cnt, nb);
     KTime kt = prof ? prof->time(1, it->bwd_mat * fscale * nb + it->bwd_vec * vecs) : KTime{};
     switch (rc) {
-      case 8: launch_bwd<8>(T, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
-      case 4: launch_bwd<4>(T, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
-      case 2: launch_bwd<2>(T, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
-      default: launch_bwd<1>(T, dm, A, nb, nrhs, inv, y, x, part, s, kt); break;
+      case 8: launch_bwd<8>(T, dm, A, nb, nrhs, inv, y, x, sx, part, s, kt); break;
+      case 4: launch_bwd<4>(T, dm, A, nb, nrhs, inv, y, x, sx, part, s, kt); break;
+      case 2: launch_bwd<2>(T, dm, A, nb, nrhs, inv, y, x, sx, part, s, kt); break;
+      default: launch_bwd<1>(T, dm, A, nb, nrhs, inv, y, x, sx, part, s, kt); break;
     }
   }
   EMU_LAUNCH_CHECK();
 }
 
 void lu_solve(DevMesh &dm, int nb, int nrhs, const zd *inv, const int32_t *gx, const zd *bnew, zd *y, zd *x,
-              zd *w, hipStream_t s, Profiler *prof) {
-  lu_solve_t<zd>(dm, nb, nrhs, inv, gx, bnew, y, x, w, s, prof);
+              zd *w, hipStream_t s, Profiler *prof, int64_t sx) {
+  lu_solve_t<zd>(dm, nb, nrhs, inv, gx, bnew, y, x, w, s, prof, sx);
 }
 
 void lu_solve(DevMesh &dm, int nb, int nrhs, const float2 *inv32, const int32_t *gx, const zd *bnew, zd *y, zd *x,
-              zd *w, hipStream_t s, Profiler *prof) {
-  lu_solve_t<float2>(dm, nb, nrhs, inv32, gx, bnew, y, x, w, s, prof);
+              zd *w, hipStream_t s, Profiler *prof, int64_t sx) {
+  lu_solve_t<float2>(dm, nb, nrhs, inv32, gx, bnew, y, x, w, s, prof, sx);
 }
 
 // complex64 copy of the inverted factor (round to nearest)
