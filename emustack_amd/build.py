@@ -19,8 +19,11 @@ ARCH = os.environ.get('EMU_OFFLOAD_ARCH', 'gfx950')
 # packed-fp32 instructions (v_pk_*_f32) are disabled for all device code:
 # some of their forms have produced silently wrong results on this hardware
 DEVFLAGS = ['-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
+# host complex arithmetic: multiplication by the plain formula (no NaN
+# recovery call path per product, which keeps the projected-matrix loops of the
+# host Schur step from vectorising) and division by Smith's scaled algorithm
 CXXFLAGS = ['-O3', '-fPIC', '-std=c++17', '-Wall', '-Wno-unused-function', '-Wno-unused-variable',
-            '-Wno-unused-but-set-variable', '-Wno-unused-result']
+            '-Wno-unused-but-set-variable', '-Wno-unused-result', '-fcomplex-arithmetic=improved']
 
 
 def _compile(src):

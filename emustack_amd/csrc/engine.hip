@@ -501,12 +501,15 @@ static void ks_restart_blocks(KSState &S, int m, int bq, int keep, int ld, int64
       const cplx t = S.T[i + (size_t)j * m];
       S.Hk[i + (size_t)j * ld] = zmk(t.real(), t.imag());
     }
-    for (int r = 0; r < bq; ++r) {
-      cplx acc = 0;
-      if (j >= S.nlock_next)  // locked: coupling deflated to zero
-        for (int i = 0; i < m; ++i) acc += S.H[(m + r) + (size_t)i * ld] * S.Q[i + (size_t)j * m];
-      S.Hk[(keep + r) + (size_t)j * ld] = zmk(acc.real(), acc.imag());
-    }
+    cplx acc[16] = {};
+    if (bq > 16) throw std::logic_error("restart blocks: block size above 16");
+    if (j >= S.nlock_next)  // locked: coupling deflated to zero
+      for (int i = 0; i < m; ++i) {
+        const cplx q = S.Q[i + (size_t)j * m];
+        const cplx *hr = &S.H[m + (size_t)i * ld];
+        for (int r = 0; r < bq; ++r) acc[r] += hr[r] * q;
+      }
+    for (int r = 0; r < bq; ++r) S.Hk[(keep + r) + (size_t)j * ld] = zmk(acc[r].real(), acc[r].imag());
   }
 }
 
@@ -554,10 +557,12 @@ static bool ks_project(KSState &st, int m, int bq, int nev, int keep, double tol
     st.Q[j + (size_t)j * m] = 1.0;
   }
   for (int j = 0; j < na; ++j) {
-    for (int i = 0; i < a0; ++i) {
-      cplx acc = 0;
-      for (int k = 0; k < na; ++k) acc += st.H[i + (size_t)(a0 + k) * ld] * QA[k + (size_t)j * na];
-      st.T[i + (size_t)(a0 + j) * m] = acc;
+    // T[:a0, a0 + j] = H[:a0, a0:m] QA[:, j], by columns of H (contiguous)
+    cplx *tc = &st.T[(size_t)(a0 + j) * m];
+    for (int k = 0; k < na; ++k) {
+      const cplx q = QA[k + (size_t)j * na];
+      const cplx *hc = &st.H[(size_t)(a0 + k) * ld];
+      for (int i = 0; i < a0; ++i) tc[i] += hc[i] * q;
     }
     for (int i = 0; i < na; ++i) {
       st.T[(a0 + i) + (size_t)(a0 + j) * m] = TA[i + (size_t)j * na];
@@ -783,6 +788,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   for (auto &x : st) x.H.assign((size_t)sH, 0);
   EigResult res;
   res.mu.assign((size_t)nb * nev, 0);
+  std::vector<int> schur_na;  // active Schur sizes, recorded when the stage timer is on
   int k = 0;
   int keep = std::min(m - bq, nev + (m - nev) / 2);
   keep = m - ((m - keep + bq - 1) / bq) * bq;  // (m - keep) multiple of bq
@@ -892,6 +898,7 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
     if (c->prof.on) c->prof.collect();
     const bool last = (it == itermax);
     std::vector<std::thread> th;
+    std::mutex schur_mu;
     std::vector<std::string> errs(nact);
     std::vector<char> convflag(nact, 0);
     for (int b = 0; b < nact; ++b) {
@@ -907,6 +914,10 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
         } leave_gate;
         try {
           KSState &S = st[b];
+          if (tm.on) {  // active Schur sizes (printed with the stage times)
+            std::lock_guard<std::mutex> g(schur_mu);
+            schur_na.push_back(m - std::min(S.nlock, nev));
+          }
           for (int64_t i = 0; i < sH; ++i) S.H[i] = cplx(Hh[(size_t)b * sH + i].x, Hh[(size_t)b * sH + i].y);
           // Ritz vectors of the basis are formed once the pairs pass the test
           convflag[b] = ks_project(S, m, bq, nev, keep, tol, &ritz[b], nullptr, false) ? 1 : 0;
@@ -1009,6 +1020,16 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
   EMU_HIP(hipMemcpy(badh.data(), bad, sizeof(int32_t) * nb, hipMemcpyDeviceToHost));
   for (int b = 0; b < nb; ++b) res.rank_fixes += badh[b];
   tm.mark("ritz_vectors");
+  if (tm.on && !schur_na.empty()) {
+    double sum = 0, cube = 0;
+    for (int v : schur_na) {
+      sum += v;
+      cube += (double)v * v * v;
+    }
+    fprintf(stderr, "[emu timing] Schur steps %zu, active size mean %.1f min %d max %d, sum of cubes / m^3 %.2f\n",
+            schur_na.size(), sum / schur_na.size(), *std::min_element(schur_na.begin(), schur_na.end()),
+            *std::max_element(schur_na.begin(), schur_na.end()), cube / ((double)m * m * m));
+  }
   return res;
 }
 
