@@ -141,6 +141,24 @@ def test_homogeneous_layer_raises(gpu_ctx, mesh600):
     assert e.value.code == -5
 
 
+def test_nonconvergence_is_an_error_and_the_context_stays_usable(gpu_ctx, mesh600, mats):
+    """py_calc_modes.f:647-656 stops the process when ARPACK does not
+    converge; here the call returns EMU_E_NOCONV (itermax = 0 with a Krylov
+    dimension just above 40 wanted modes cannot converge in one expansion)
+    and the next call on the same context and mesh gives the converged
+    result."""
+    from emustack_amd import _lib
+    mh, m = mesh600
+    p = _params(500.0, 600., mats('Si_c', 500.0), 20, 40)
+    with pytest.raises(_lib.EmuError) as e:
+        gpu_ctx.calc_modes_batch(mh, [p[0]], 40, 1, [p[2]], [p[3]], [p[4]], 5, itermax=0, ncv=48)
+    assert e.value.code == -4
+    b1, _, _, st = gpu_ctx.calc_modes_batch(mh, [p[0]], 20, 1, [p[2]], [p[3]], [p[4]], 5)
+    b2, _, _, _ = gpu_ctx.calc_modes_batch(mh, [p[0]], 20, 1, [p[2]], [p[3]], [p[4]], 5)
+    np.testing.assert_array_equal(b1, b2)
+    assert st[1] <= 30
+
+
 def _api_stack(wl, theta, phi, nbm, order, pol):
     from emustack_amd import objects, materials, Stack
     import json
