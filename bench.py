@@ -412,7 +412,12 @@ def main():
                                       if kernel_name in TRAFFIC_PER_DISPATCH and ms > 0 else None),
                      'launches_timed': int(cnt), 'avg_launch_us': 1e3 * ms / max(cnt, 1)},
         'whole_step_roofline': whole,
-        'kernel_profile': {k: {'launches': int(v[0]), 'ms': v[1], 'GB': v[2] / 1e9} for k, v in prof.items()},
+        # every timed class with its own roofline (the four largest are within a few per cent of each
+        # other, so which one is 'dominant' can change from run to run)
+        'kernel_profile': {k: {'launches': int(v[0]), 'ms': v[1], 'GB': v[2] / 1e9,
+                               'GBps': v[2] / (v[1] * 1e-3) / 1e9 if v[1] > 0 else None,
+                               'frac': v[2] / (v[1] * 1e-3) / 1e9 / HBM_PEAK_GBS if v[1] > 0 else None}
+                           for k, v in prof.items()},
     }
     # the other precision form on one batch holding the fixture wavelength
     fx = np.load(FIXTURE, allow_pickle=False) if os.path.exists(FIXTURE) else None
