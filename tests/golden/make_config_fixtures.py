@@ -61,6 +61,9 @@ C2 = dict(period=400.0, diam=(80.0, 120.0), n=(1.45, 2.0, 3.6 + 0.005j), h=150.0
 C3 = dict(period=500.0, radii=(100.0, 150.0, 80.0), heights=(100.0, 150.0, 120.0), ntri=8000, nval=200, order=4,
           wl=float(np.linspace(400.0, 1000.0, 512)[64]))
 C4 = dict(seed=4, ncell=3, nval=100, order=2, wls=(500.0, 700.0))
+# one layer of config 3 at the oracle's reach: the first 8,000-triangle mesh
+# (96,000 unknowns), 60 Bloch modes, max_order_PWs = 2, one wavelength
+C3S = dict(layer=0, nval=60, order=2, h=100.0, wl=C3['wl'])
 C5 = dict(period=940.0, h=200.0, wl=1.11 * 940, theta=5.0, phi=0.0, nval=100, order=4, mesh='4testing-940_266_sq.mail')
 
 
@@ -221,6 +224,22 @@ def config3(pool):
         out['JJd' + tag] = np.array([r[1] @ r[k] for r in res])
         out['rta' + tag] = np.array([o['a_list'][-1], o['r_list'][-1], o['t_list'][-1]], float)
     np.savez(os.path.join(OUT, 'config3.npz'), **out)
+
+
+def config3s(pool):
+    from oracle import stack_ref
+    i = C3S['layer']
+    path, n = config3_meshes()[i], config3_indices()[i]
+    wl, per = C3S['wl'], C3['period']
+    r = _modes_2d((path, wl, per, [1.0, n], C3S['nval'], C3S['order']))
+    out = dict(wl=np.array([wl]), beta=r[0][None])
+    for tag, k in (('', 2), ('_cluster', 3)):
+        L = [dict(kind='tf', n=1.5, h=None), dict(kind='ns', kz=r[0], J=r[1], Jd=r[k], h=C3S['h'] / per),
+             dict(kind='tf', n=1.0, h=None)]
+        o = stack_ref.calc_scat(L, wl / per, k_par_norm(wl, 0, 0, per), C3S['order'], pol='TE')
+        out['JJd' + tag] = (r[1] @ r[k])[None]
+        out['rta' + tag] = np.array([o['a_list'][-1], o['r_list'][-1], o['t_list'][-1]], float)
+    np.savez(os.path.join(OUT, 'config3s.npz'), **out)
 
 
 def config4_cells():

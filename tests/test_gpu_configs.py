@@ -9,7 +9,8 @@ tests/golden/make_config_fixtures.py):
            Bloch modes) at one wavelength (the oracle's fixture for it did
            not finish within the build's CPU budget: that test skips without
            it; batch and Krylov-dimension invariance are checked at full
-           size either way);
+           size either way), and one of its layers at 60 Bloch modes
+           against the oracle (config3s);
   config4  the uneven batch at reduced size: three distinct cells (periods,
            ellipses, rotations, indices, slab heights and mesh sizes drawn from
            a seed), 100 Bloch modes, two wavelengths each, all solved in one
@@ -164,6 +165,32 @@ def _config3_stack(wls, ncv=0, groups=1):
         st.calc_scat(pol='TE')
         out.append((st, [s[i] for s in sims]))
     return out, stats
+
+
+def test_config3_layer_against_oracle():
+    """One config-3 layer (the first 8,000-triangle mesh, 96,000 unknowns)
+    at 60 Bloch modes, max_order_PWs = 2, between a substrate and air: beta,
+    J J^dagger and R/T/A against the oracle's fixture at the bench's mesh
+    size."""
+    from emustack_amd import objects, Stack
+    from emustack_amd.batch import calc_modes_many
+    fx = _load('config3s')
+    C, S = cf.C3, cf.C3S
+    i = S['layer']
+    ns = _structure_2d(cf.config3_meshes()[i], C['period'], C['radii'][i], cf.config3_indices()[i], S['h'])
+    light = objects.Light(float(fx['wl'][0]), max_order_PWs=S['order'], theta=0.0, phi=0.0)
+    sims = calc_modes_many(ns, [light], num_BMs=S['nval'])
+    _restarts('config3 layer', calc_modes_many.last_stats)
+    sim = sims[0]
+    _check_beta(sim.k_z, fx['beta'][0], 'config3 layer')
+    _check_proj(sim.J, sim.J_dag, fx['JJd_cluster'][0], 'config3 layer')
+    sub, sup = _films(C['period'])
+    st = Stack((sub.calc_modes(light), sim, sup.calc_modes(light)))
+    st.calc_scat(pol='TE')
+    err = np.abs(_rta(st) - fx['rta_cluster']).max()
+    print('config3 layer: R/T/A err %.2e (per-mode normalisation of the oracle: %.2e)'
+          % (err, np.abs(_rta(st) - fx['rta']).max()), flush=True)
+    assert err <= 1e-8
 
 
 def test_config3_batch_and_krylov_invariant():
