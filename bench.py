@@ -285,6 +285,73 @@ def trace_summary(trace_dir, top=30):
                                                        t / 1e3 / c, 100.0 * t / tot))
 
 
+def trace_levels(trace_dir, workdir):
+    """Factorisation time and TFLOP/s by elimination-tree level from a
+    rocprofv3 --kernel-trace CSV of the bench: each stream's kernels from
+    k_scatter_fronts to k_triinv are one factorisation, split into levels
+    (leaves first) after each level's k_extend_add launches; flops from the
+    symbolic count of the bench meshes (averaged over the three)."""
+    import csv
+    import glob
+    import collections
+    from emustack_amd import _lib
+    files = glob.glob(os.path.join(trace_dir, '**', '*kernel_trace.csv'), recursive=True)
+    if not files:
+        raise SystemExit('no kernel_trace.csv under %s' % trace_dir)
+    lu = ('k_scatter_fronts', 'k_panel', 'k_trsm', 'k_panel_rows', 'k_gemm_update', 'k_extend_add')
+    streams = collections.defaultdict(list)
+    with open(files[0]) as f:
+        for r in csv.DictReader(f):
+            n = r['Kernel_Name'].split('(')[0].replace('void ', '').replace('emu::', '')
+            streams[r['Stream_Id']].append((int(r['Start_Timestamp']), int(r['End_Timestamp']), n,
+                                            int(r.get('Grid_Size_Y', 1) or 1)))
+    facts = []
+    for ks in streams.values():
+        cur = None
+        for a, b, n, gy in sorted(ks):
+            if n.startswith('k_scatter_fronts'):
+                cur = dict(nb=gy, levels=[[]])
+                facts.append(cur)
+            elif cur is not None and (n.startswith('k_triinv') or n.startswith('k_gather_index')):
+                cur = None
+            elif cur is not None and any(n.startswith(x) for x in lu):
+                lv = cur['levels']
+                if n.startswith('k_extend_add') and not lv[-1] and len(lv) > 1:
+                    lv[-2].append((a, b))  # the level's second extend-add launch
+                    continue
+                lv[-1].append((a, b))
+                if n.startswith('k_extend_add'):
+                    lv.append([])
+    fl = []
+    for p in build_meshes(workdir):
+        fr = _lib.MeshHandle(p).symbolic()['fronts']
+        sz, u, lev = fr[:, 0].astype(float), fr[:, 1].astype(float), fr[:, 5]
+        fl.append([8.0 * np.sum((sz ** 3 / 3 + sz ** 2 * u + sz * u ** 2)[lev == l]) for l in range(lev.max() + 1)])
+    fl = np.mean(np.array(fl), axis=0)
+    nlev = len(fl)
+    acc = collections.defaultdict(lambda: [0.0, 0, 0])
+    for fct in facts:
+        levs = [l for l in fct['levels'] if l]
+        if len(levs) != nlev:
+            continue
+        for i, l in enumerate(levs):
+            k = acc[nlev - 1 - i]
+            k[0] += l[-1][1] - l[0][0]
+            k[1] += 1
+            k[2] = fct['nb']
+    print('| level | flops per instance | ms per factorisation | TFLOP/s |')
+    tf = tw = 0.0
+    for level in sorted(acc):
+        w, c, nb = acc[level]
+        w = w / c / 1e9
+        tf += fl[level] * nb
+        tw += w
+        print('| %d | %.2e | %.2f | %.2f |' % (level, fl[level], w * 1e3, fl[level] * nb / w / 1e12))
+    if tw:
+        print('all levels: %.2e flops per %d-instance factorisation in %.1f ms = %.2f TFLOP/s (%d factorisations)'
+              % (tf, nb, tw * 1e3, tf / tw / 1e12, acc[0][1]))
+
+
 def host_cores():
     """Cores this process may use (the box grants 16 per GPU)."""
     try:
@@ -315,6 +382,8 @@ def main():
     ap.add_argument('--no-compare', action='store_true', help='skip the step of the other precision form')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only copy 2 GiB device to device (the reference kernel for scaling HBM counters)')
+    ap.add_argument('--trace-levels', metavar='DIR',
+                    help='factorisation time and TFLOP/s per elimination-tree level from a rocprofv3 kernel trace')
     ap.add_argument('--trace-summary', metavar='DIR',
                     help='only summarise a rocprofv3 kernel trace of a --steps 1 --warmup 1 run (no GPU use)')
     ap.add_argument('--workdir', default=os.path.join(tempfile.gettempdir(), 'emustack_amd_bench_meshes'),
@@ -322,6 +391,10 @@ def main():
     args = ap.parse_args()
     if args.trace_summary:
         trace_summary(args.trace_summary)
+        return
+    if args.trace_levels:
+        os.makedirs(args.workdir, exist_ok=True)
+        trace_levels(args.trace_levels, args.workdir)
         return
     if args.copy_calibration:
         import torch
