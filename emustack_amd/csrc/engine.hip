@@ -822,9 +822,9 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
     launch_swap_i32(1, bad + a1, bad + a2, s);
   };
   zd *Wd = c->buf<zd>("ks_W", (size_t)nb * m * nev);
-  // The GPU phases of a cycle (restart combination + expansion) hold the
-  // device's turn; the host Schur phase releases it, so that concurrent
-  // sub-batches on other contexts alternate on the GPU instead of expanding
+  // The expansion of a cycle holds the device's turn; the host Schur phase
+  // and the restart products release it, so that concurrent sub-batches on
+  // other contexts alternate their expansions on the GPU instead of expanding
   // together and then leaving the GPU idle during their Schur phases.
   std::unique_lock<std::mutex> turn(gpu_turn(c->device), std::defer_lock);
   for (int it = 0; it <= itermax; ++it) {
@@ -939,7 +939,9 @@ static EigResult krylov_schur(Ctx *c, DevMesh &dm, int nb, int nev, int m, int b
       if (!errs[b].empty()) throw std::runtime_error(errs[b]);
     res.restarts = it;
     tm.mark("host_schur");
-    turn.lock();
+    // The Ritz-vector and restart products below run without the turn (it is
+    // taken again at the top of the next cycle): they are matrix-core bound
+    // and overlap another sub-batch's bandwidth-bound expansion.
     // Retire the converged instances: their Ritz vectors X = V[:, :m] W are
     // formed now, and the slot is filled with the last active instance, so
     // that the next cycles run on the active instances only.
