@@ -374,7 +374,7 @@ def main():
                     help='concurrent sub-batches (host threads, each with its own context and stream)')
     ap.add_argument('--ncv', type=int, default=384,
                     help='Krylov dimension (0: the reference rule 2 num_BMs + num_BMs/2 + 3 = 503)')
-    ap.add_argument('--workers', type=int, default=3,
+    ap.add_argument('--workers', type=int, default=4,
                     help='host threads (contexts) for the layer sub-batches (0: groups + 1); each context '
                          'holds ~40 GB at 8 points per sub-batch, and 16 host cores serve the Schur steps')
     ap.add_argument('--precision', choices=('fp64', 'mixed'), default='fp64')
