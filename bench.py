@@ -57,12 +57,12 @@ FIXTURE = os.path.join(ROOT, 'tests', 'golden', 'configs', 'config3.npz')
 # default step), each scaled by what the counter reads for a plain 2 GiB copy;
 # summary and commands in profiles/kernels_stacked_gratings_8k.md.
 TRAFFIC_PER_DISPATCH = {
-    'k_atb_mfma16<true>': 8473617203,
-    'k_combine<16>': 7323914649,
-    'k_fwd': 433475379,
-    'k_bwd': 412970291,
-    'k_spmm_il': 2803557888,
-    'k_zgemm_nn_tall': 9749818880,
+    'k_atb_mfma16<true>': 8473300000,
+    'k_combine<16>': 7323900000,
+    'k_fwd': 433500000,
+    'k_bwd': 412900000,
+    'k_spmm_il': 2810600000,
+    'k_zgemm_nn_tall': 9751900000,
 }
 TRAFFIC_SOURCE = 'profiles/kernels_stacked_gratings_8k.md'
 
@@ -352,6 +352,45 @@ def trace_levels(trace_dir, workdir):
               % (tf, nb, tw * 1e3, tf / tw / 1e12, acc[0][1]))
 
 
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir):
+    """HBM bytes per dispatch of each kernel from rocprofv3 --pmc runs: the
+    average FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of every kernel,
+    each scaled by what the same counter reads for the --copy-calibration
+    copy of known size (bytes moved / counter bytes)."""
+    import csv
+    import glob
+    import collections
+
+    def read(d):
+        files = glob.glob(os.path.join(d, '**', '*counter_collection.csv'), recursive=True)
+        if not files:
+            raise SystemExit('no counter_collection.csv under %s' % d)
+        agg = collections.defaultdict(lambda: [0, 0.0])
+        with open(files[0]) as f:
+            for r in csv.DictReader(f):
+                k = r['Kernel_Name'].split('(')[0].replace('void ', '').replace('emu::', '')
+                agg[k][0] += 1
+                agg[k][1] += float(r['Counter_Value'])
+        return agg
+
+    copy_bytes = 2 * 2 ** 30  # bench.py --copy-calibration: a 2 GiB device copy (read 2 GiB, write 2 GiB)
+    cf, cw = read(copy_fetch_dir), read(copy_write_dir)
+    kf = max(cf.values(), key=lambda v: v[1])
+    kw = max(cw.values(), key=lambda v: v[1])
+    sf = copy_bytes / (kf[1] / kf[0] * 1024.0)
+    sw = copy_bytes / (kw[1] / kw[0] * 1024.0)
+    print('copy calibration: FETCH_SIZE %.0f KiB, WRITE_SIZE %.0f KiB per 2 GiB copy -> scale %.3f / %.3f'
+          % (kf[1] / kf[0], kw[1] / kw[0], sf, sw))
+    fe, wr = read(fetch_dir), read(write_dir)
+    print('| kernel | dispatches | fetch MB / dispatch | write MB / dispatch | total GB / dispatch |')
+    print('|---|---:|---:|---:|---:|')
+    for name in sorted(fe, key=lambda k: -fe[k][1]):
+        c, v = fe[name]
+        fb = v / c * 1024.0 * sf
+        wb = (wr[name][1] / wr[name][0] * 1024.0 * sw) if name in wr else 0.0
+        print('| `%s` | %d | %.1f | %.1f | %.3f |' % (name, c, fb / 1e6, wb / 1e6, (fb + wb) / 1e9))
+
+
 def host_cores():
     """Cores this process may use (the box grants 16 per GPU)."""
     try:
@@ -382,6 +421,9 @@ def main():
     ap.add_argument('--no-compare', action='store_true', help='skip the step of the other precision form')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only copy 2 GiB device to device (the reference kernel for scaling HBM counters)')
+    ap.add_argument('--pmc-summary', nargs=4, metavar=('FETCH', 'WRITE', 'COPY_FETCH', 'COPY_WRITE'),
+                    help='HBM bytes per dispatch per kernel from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of the '
+                         'bench and of --copy-calibration')
     ap.add_argument('--trace-levels', metavar='DIR',
                     help='factorisation time and TFLOP/s per elimination-tree level from a rocprofv3 kernel trace')
     ap.add_argument('--trace-summary', metavar='DIR',
@@ -391,6 +433,9 @@ def main():
     args = ap.parse_args()
     if args.trace_summary:
         trace_summary(args.trace_summary)
+        return
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
         return
     if args.trace_levels:
         os.makedirs(args.workdir, exist_ok=True)
