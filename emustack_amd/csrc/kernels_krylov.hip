@@ -501,9 +501,20 @@ void k_atb_mfma16(int64_t K, int M, int64_t kchunk, const zd *__restrict__ A,
                                                     int64_t lda, int64_t sA, const zd *__restrict__ B, int64_t ldb,
                                                     int64_t sB, zd *__restrict__ part) {
   constexpr int TM = 64, TK = 32, NQ = 16;
-  const int m0 = blockIdx.x * TM;
-  const int chunk = blockIdx.y, nchunk = gridDim.y;
-  const int b = blockIdx.z;
+  // The column tiles of one (K chunk, instance) all read the same rows of B.
+  // Workgroups go to the 8 XCDs round-robin by linear id, each XCD with its
+  // own L2, so the linear id is renumbered to give consecutive tiles to one
+  // XCD (XCD x takes a contiguous range of the tile order, in order): B's
+  // rows are then fetched from HBM once per chunk instead of once per tile.
+  // The renumbering is a bijection of [0, T) whatever the hardware's
+  // placement, so it affects only locality.
+  const unsigned T = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned Lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const unsigned xcd = Lin & 7u, q = T >> 3, r = T & 7u;
+  const unsigned Lg = xcd * q + min(xcd, r) + (Lin >> 3);
+  const int m0 = (int)(Lg % gridDim.x) * TM;
+  const int chunk = (int)((Lg / gridDim.x) % gridDim.y), nchunk = gridDim.y;
+  const int b = (int)(Lg / (gridDim.x * gridDim.y));
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const zd *Ab = A + b * sA;
   const zd *Bb = B + b * sB;
