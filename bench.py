@@ -57,12 +57,12 @@ FIXTURE = os.path.join(ROOT, 'tests', 'golden', 'configs', 'config3.npz')
 # default step), each scaled by what the counter reads for a plain 2 GiB copy;
 # summary and commands in profiles/kernels_stacked_gratings_8k.md.
 TRAFFIC_PER_DISPATCH = {
-    'k_atb_mfma16<true>': 7545800000,
+    'k_atb_mfma16<true>': 7475500000,
     'k_combine<16>': 7323900000,
     'k_fwd': 433500000,
     'k_bwd': 412900000,
     'k_spmm_il': 2810600000,
-    'k_zgemm_nn_tall': 9751900000,
+    'k_zgemm_nn_tall': 3957700000,
 }
 TRAFFIC_SOURCE = 'profiles/kernels_stacked_gratings_8k.md'
 

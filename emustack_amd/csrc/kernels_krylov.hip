@@ -697,6 +697,8 @@ void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64
     kchunk = ((kchunk + 31) / 32) * 32;
     nchunk = (K + kchunk - 1) / kchunk;
     if ((size_t)(nchunk * per) > scratch_elems) throw HipError("atb: scratch too small");
+    // the kernel renumbers the linear id of the whole grid in 32 bits
+    if ((int64_t)tiles * nchunk * nb > 0xffffffffLL) throw HipError("atb: grid too large");
     dim3 g((unsigned)tiles, (unsigned)nchunk, nb), t(256);
     if (conjA)
       hipExtLaunchKernelGGL(k_atb_mfma16<true>, g, t, 0, s, kt.a, kt.b, 0, K, M, kchunk, A, lda, sA, B, ldb, sB, scratch);
@@ -842,11 +844,20 @@ void k_zgemm_nn_tall(int64_t M, int N, int K, zd alpha, const zd *__restrict__ A
                                                        int64_t ldb, int64_t sB, zd beta, zd *__restrict__ C,
                                                        int64_t ldc, int64_t sC, int tilesN) {
   constexpr int TM = 128, TN = 64, TK = 16;
-  const int64_t tm = blockIdx.x / tilesN;
-  const int tn = blockIdx.x % tilesN;
+  // The tilesN column tiles of a row tile read the same 128 rows of A: the
+  // linear workgroup id is renumbered (a bijection of [0, T)) so that each
+  // of the 8 XCDs, which take workgroups round-robin and have an L2 each,
+  // runs a contiguous range of the tile order, row tile by row tile.
+  const unsigned T = gridDim.x * gridDim.y;
+  const unsigned Lin = blockIdx.x + gridDim.x * blockIdx.y;
+  const unsigned xcd = Lin & 7u, q = T >> 3, r = T & 7u;
+  const unsigned Lg = xcd * q + min(xcd, r) + (Lin >> 3);
+  const unsigned bx = Lg % gridDim.x;
+  const int64_t tm = bx / tilesN;
+  const int tn = bx % tilesN;
   const int64_t m0 = tm * TM;
   const int n0 = tn * TN;
-  const int b = blockIdx.y;
+  const int b = (int)(Lg / gridDim.x);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wm = wv & 1, wn = wv >> 1;
   const zd *Ab = A + b * sA;
@@ -936,7 +947,8 @@ void launch_gemm_nn(int nb, int64_t M, int N, int K, zd alpha, const zd *A, int6
   if (M >= 4096 && N >= 64) {  // tall: 128 x 64 tiles
     const int tilesN = (N + 63) / 64;
     const int64_t tiles = ((M + 127) / 128) * tilesN;
-    if (tiles > 0x7fffffff || nb > 65535) throw HipError("gemm_nn: grid too large");
+    // the kernel renumbers the linear id of the whole grid in 32 bits
+    if (tiles * nb > 0xffffffffLL || nb > 65535) throw HipError("gemm_nn: grid too large");
     dim3 g((unsigned)tiles, nb), t(256);
     if (beta.x == 0.0 && beta.y == 0.0)
       hipLaunchKernelGGL(k_zgemm_nn_tall<false>, g, t, 0, s, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc,
