@@ -1217,21 +1217,37 @@ __global__ __launch_bounds__(256) void k_dcgs_coef(int j, int c0, const zd *__re
   __shared__ zd SS[B][B], ST[B][B], U[B][B], R[B][B], Ri[B][B], Bm[B][B], CQ[B][B], Dm[B][B], T1[B][B];
   __shared__ zd Gs[NG][NG];
   for (int idx = tid; idx < NG * NG; idx += blockDim.x) Gs[idx % NG][idx / NG] = Pb[(j + idx % NG) + (idx / NG) * ldp];
-  // 1. S^H S, S^H T, u = Hrow S (one output per thread)
-  if (tid < 3 * B * B) {
+  // 1. S^H S, S^H T over the j basis rows: 32-row chunks of P[:, :16] are
+  // staged in LDS by all threads (row tid % 32, columns tid / 32 and
+  // tid / 32 + 8), so a chunk costs one round of loads; one thread per
+  // output then sums the chunk's rows in order (the same order, hence the
+  // same rounding, as a single loop over i).  u = Hrow S after the loop.
+  __shared__ zd Pc[32][2 * B + 1];
+  {
     const int o = tid % (B * B), kind = tid / (B * B), p = o % B, q = o / B;
+    const int lr = tid & 31, lc = tid >> 5;
     zd acc = zmk(0, 0);
-    if (kind == 0) {
-#pragma unroll 8
-      for (int i = 0; i < j; ++i) acc = zfma(zconj(Pb[i + p * ldp]), Pb[i + q * ldp], acc);
-      SS[p][q] = acc;
-    } else if (kind == 1) {
-      if (!FIN) {
-#pragma unroll 8
-        for (int i = 0; i < j; ++i) acc = zfma(zconj(Pb[i + p * ldp]), Pb[i + (B + q) * ldp], acc);
+    for (int i0 = 0; i0 < j; i0 += 32) {
+      const int i = i0 + lr;
+      const bool ok = i < j;
+      const zd v0 = ok ? Pb[i + lc * ldp] : zmk(0, 0);
+      const zd v1 = (ok && !FIN) ? Pb[i + (lc + B) * ldp] : zmk(0, 0);
+      __syncthreads();  // the previous chunk has been consumed
+      Pc[lr][lc] = v0;
+      Pc[lr][lc + B] = v1;
+      __syncthreads();
+      if (kind < 2) {
+        const int nr = min(32, j - i0), cq = kind == 0 ? q : B + q;
+        for (int rr = 0; rr < nr; ++rr) acc = zfma(zconj(Pc[rr][p]), Pc[rr][cq], acc);
       }
-      ST[p][q] = acc;
-    } else {
+    }
+    if (kind == 0) SS[p][q] = acc;
+    if (kind == 1) ST[p][q] = FIN ? zmk(0, 0) : acc;
+  }
+  if (tid >= 2 * B * B && tid < 3 * B * B) {
+    const int o = tid % (B * B), p = o % B, q = o / B;
+    zd acc = zmk(0, 0);
+    {
       if (!FIN) {
 #pragma unroll 8
         for (int c = c0; c < j; ++c) acc = zfma(Hb[(j + p) + (int64_t)c * ldh], Pb[c + q * ldp], acc);
