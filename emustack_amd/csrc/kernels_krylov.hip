@@ -277,10 +277,19 @@ __global__ __launch_bounds__(256) void k_atb_mfma_wide(int64_t K, int M, int N, 
                                                        const zd *__restrict__ B, int64_t ldb, int64_t sB,
                                                        zd *__restrict__ part, int tilesM) {
   constexpr int TT = 64, TK = 32;
-  const int tm = blockIdx.x % tilesM, tn = blockIdx.x / tilesM;
+  // The output tiles of one (K chunk, instance) read the same rows of A and
+  // B: the linear workgroup id is renumbered (a bijection of [0, T)) so that
+  // each of the 8 XCDs, which take workgroups round-robin and have an L2
+  // each, runs a contiguous range of the tile order.
+  const unsigned T = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned Lin = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const unsigned xcd = Lin & 7u, q8 = T >> 3, r8 = T & 7u;
+  const unsigned Lg = xcd * q8 + min(xcd, r8) + (Lin >> 3);
+  const int bx = (int)(Lg % gridDim.x);
+  const int tm = bx % tilesM, tn = bx / tilesM;
   const int m0 = tm * TT, n0 = tn * TT;
-  const int chunk = blockIdx.y, nchunk = gridDim.y;
-  const int b = blockIdx.z;
+  const int chunk = (int)((Lg / gridDim.x) % gridDim.y), nchunk = gridDim.y;
+  const int b = (int)(Lg / (gridDim.x * gridDim.y));
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const zd *Ab = A + b * sA;
   const zd *Bb = B + b * sB;
@@ -659,6 +668,8 @@ void launch_atb(int nb, int64_t K, int M, int N, const zd *A, int64_t lda, int64
     kchunk = ((kchunk + 31) / 32) * 32;
     nchunk = (K + kchunk - 1) / kchunk;
     if ((size_t)(nchunk * per) > scratch_elems) throw HipError("atb: scratch too small");
+    // the kernel renumbers the linear id of the whole grid in 32 bits
+    if ((int64_t)tiles * nchunk * nb > 0xffffffffLL) throw HipError("atb: grid too large");
     dim3 g((unsigned)tiles, (unsigned)nchunk, nb);
     if (conjA)
       hipLaunchKernelGGL(k_atb_mfma_wide<true>, g, dim3(256), 0, s, K, M, N, kchunk, A, lda, sA, B, ldb, sB, scratch,
